@@ -1,0 +1,168 @@
+/*
+ * tjoin.h -- C ABI of libtjoin.so, the MI355X (gfx950) certified exact
+ * epsilon-join.
+ *
+ * Contract of every join entry point: given row-major FP32 point sets R
+ * [nR x d] and S [nS x d] in device memory and a squared threshold T
+ * (binary64), emit every pair (i, j) whose sequential binary64 difference
+ * form d2_seq(R_i, S_j) = sum_k fl(fl(R_ik - S_jk)^2), k ascending, no FMA,
+ * satisfies d2_seq <= T, sorted ascending by (i, j).  A self-join (S == NULL)
+ * emits i < j only.  The predicate is the one of
+ * CEJ/src/run_g2a_tensorjoin.py:193-232 (refine_ambiguous_fp64_i64) in the
+ * summation order of CEJ/src/audit_g2b_accepted_fp64.py:51-57.
+ *
+ * Conventions: all pointers are device pointers unless noted; every call
+ * enqueues on `stream` and returns without waiting, except where a comment
+ * says "synchronous".  Return value: TJ_OK or a negative TJ_ERR_* code;
+ * tj_last_error_string() describes the last failure of the calling thread.
+ * Capacity overflow is not an error of the call: the stats block reports the
+ * counts that were needed and the caller retries with larger buffers (the
+ * retry protocol of CEJ/src/run_g2a_tensorjoin.py:329-378).
+ */
+#ifndef TJOIN_H
+#define TJOIN_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TJ_OK 0
+#define TJ_ERR_INVALID (-1)     /* bad argument */
+#define TJ_ERR_HIP (-2)         /* a HIP runtime call failed */
+#define TJ_ERR_UNSUPPORTED (-3) /* shape/precision this build does not serve */
+#define TJ_ERR_WORKSPACE (-4)   /* workspace smaller than tj_plan_workspace_bytes */
+
+/* Screen precisions (the low-precision MFMA pass that certifies most pairs). */
+#define TJ_PREC_NONE 0   /* no screen: plain FP64 brute force */
+#define TJ_PREC_FP16 1   /* FP16 MFMA, one term */
+#define TJ_PREC_FP16X3 2 /* FP16 MFMA, hi/lo split: hi.hi + hi.lo + lo.hi */
+#define TJ_PREC_BF16 3   /* BF16 MFMA, one term */
+#define TJ_PREC_BF16X3 4 /* BF16 MFMA, hi/lo split */
+#define TJ_PREC_FP8 5    /* OCP e4m3 MFMA with per-row power-of-two scales */
+#define TJ_PREC_FP8X3 6  /* e4m3 hi/lo split with per-row scales */
+
+/* Join methods. */
+#define TJ_METHOD_CERTIFIED 0  /* MFMA screen + FP64 refinement of the ambiguous band */
+#define TJ_METHOD_PLAIN_FP64 1 /* FP64 brute-force tile, no certificate band */
+#define TJ_METHOD_GRID 2       /* uniform grid (d <= 3), FP64 predicate per candidate */
+
+/* Output pair formats. */
+#define TJ_OUT_I32 0 /* int32 [P, 2] */
+#define TJ_OUT_I64 1 /* int64 [P, 2] */
+
+/* Flags. */
+#define TJ_FLAG_AUDIT 1u  /* route certified accepts through FP64 too and count certificate violations */
+#define TJ_FLAG_TIMING 2u /* record HIP events between pipeline stages (read with tj_plan_stage_times) */
+
+/* Slots of the uint64 stats block (TJ_STATS_SLOTS entries, device memory).
+ * Slot meanings follow the reference counters [accept, ambiguous, overflow,
+ * fp64, reject, bitwise-equal] of CEJ/src/run_g3c_b_r1_gpu_cascade.py:108-129,
+ * extended with the screen error statistics. */
+#define TJ_ST_OUT 0           /* pairs written to the output (certified + refined) */
+#define TJ_ST_AMB 1           /* ambiguous pairs queued for FP64 */
+#define TJ_ST_CERT_ACCEPT 2   /* pairs certified inside by the screen */
+#define TJ_ST_FP64_ACCEPT 3   /* ambiguous pairs the FP64 refinement accepted */
+#define TJ_ST_CANDIDATES 4    /* pairs that reached the per-pair interval test */
+#define TJ_ST_MAX_ERR 5       /* max |d2_screen - d2_fp64| over refined pairs (binary64 bits, real units) */
+#define TJ_ST_MAX_RATIO 6     /* max |d2_screen - d2_fp64| / certified half-width (binary64 bits) */
+#define TJ_ST_VIOLATIONS 7    /* refined pairs outside their certified interval (must be 0) */
+#define TJ_ST_WORK 8          /* work-queue cursor of the persistent screen */
+#define TJ_ST_INVALID 9       /* non-finite input coordinates seen by the row metadata pass */
+#define TJ_ST_GRID_CAND 10    /* candidate evaluations of the grid join */
+#define TJ_ST_PAIRS 11        /* pair evaluations the call covers */
+#define TJ_ST_SCALE_EXP 12    /* join-unit exponent G (x_join = x * 2^-G), as int64 */
+#define TJ_STATS_SLOTS 16
+
+/* ---------------------------------------------------------------- library */
+int tj_version(void);
+const char* tj_last_error_string(void);
+int tj_device_info(int* cu_count, int* arch_major, int* arch_minor); /* synchronous */
+
+/* ------------------------------------------------------------------ plans
+ * A plan fixes shapes, method, precision and buffer capacities and lays out
+ * the caller-owned workspace.  Replaces the per-run buffer set of
+ * TG/src/g2_operator_r1.py:42-61 (Matrix.run) and
+ * CEJ/src/run_g5_tensorjoin_public.py:384-394. */
+typedef struct tj_plan tj_plan;
+
+int tj_plan_create(tj_plan** plan, int64_t nR, int64_t nS /* 0 = self-join */, int d,
+                   int method, int precision, int64_t out_capacity, int64_t amb_capacity,
+                   unsigned flags);
+int tj_plan_destroy(tj_plan* plan);
+int tj_plan_workspace_bytes(const tj_plan* plan, size_t* bytes);
+int tj_plan_describe(const tj_plan* plan, char* buf, size_t len); /* host only */
+/* Milliseconds of the stages of the last tj_join on a TJ_FLAG_TIMING plan:
+ * ms[0] metadata (scale scan, parameters, row metadata), ms[1] screen (or the
+ * plain brute force), ms[2] FP64 refinement, ms[3] sort + unpack.
+ * Synchronous (waits for the last stage event).  Counterpart of the optional
+ * per-stage events of OA/src/common.py:91-106. */
+int tj_plan_stage_times(tj_plan* plan, float* ms, int n);
+
+/* Full join: enqueue row metadata -> screen -> FP64 refinement -> sort ->
+ * unpack.  `pairs` receives min(P, out_capacity) rows in `out_format`;
+ * `stats` (uint64[TJ_STATS_SLOTS], device) receives the counters.  Replaces
+ * Matrix.run (TG/src/g2_operator_r1.py:42-61) and run_operator
+ * (G16/src/fp32_gpu_norm_operator.py:107-161). */
+int tj_join(tj_plan* plan, const float* R, const float* S, double T, void* workspace,
+            void* pairs, int out_format, uint64_t* stats, void* stream);
+
+/* ------------------------------------------------------- pipeline stages
+ * The stages tj_join runs, exported for callers that compose their own
+ * pipeline and for tests.  All counts are read from device memory so that
+ * the sequence is capturable in a hipGraph. */
+
+/* Row metadata for one side: quantized copies and the certificate terms.
+ * Replaces half_metadata (TG/src/gpu_kernels.py:33-51), gpu_quantize_metadata
+ * (G16/src/gpu_preparation.py:12-49) and gpu_norm_metadata
+ * (G16/src/gpu_norms.py:11-25).  `params` is the device parameter block
+ * written by tj_params. */
+int tj_scale_scan(const float* x, int64_t n, int d, uint32_t* maxabs_bits, uint64_t* stats,
+                  void* stream);
+int tj_params(const uint32_t* maxabs_bits, double T, int d, int precision, void* params,
+              uint64_t* stats, void* stream);
+int tj_rowmeta(const float* x, int64_t n, int d, int precision, const void* params,
+               void* q_hi, void* q_lo, float* meta, uint64_t* stats, void* stream);
+
+/* FP64 refinement of queued pair keys (i << jbits | j): the sequential
+ * difference form, appending accepted keys at out_keys[stats[TJ_ST_OUT]++].
+ * Replaces refine_ambiguous_fp64_i64 (CEJ/src/run_g2a_tensorjoin.py:193-232). */
+int tj_refine_fp64(const float* R, const float* S, int d, int jbits, double T,
+                   const uint64_t* amb_keys, const float* amb_aux, int64_t amb_capacity,
+                   const void* params, uint64_t* out_keys, int64_t out_capacity,
+                   uint64_t* stats, void* stream);
+
+/* Plain FP64 brute force (the comparison method): every pair evaluated in
+ * the sequential FP64 difference form, accepted keys appended to out_keys.
+ * Replaces the GDS brute-force kernel (GDS/kernel.cu:493-522) and the FP64
+ * control of G16/src/fp32_gpu_norm_operator.py:107-161. */
+int tj_plain_fp64(const float* R, int64_t nR, const float* S, int64_t nS, int d, int jbits,
+                  double T, uint64_t* out_keys, int64_t out_capacity, uint64_t* stats,
+                  void* stream);
+
+/* Ascending LSD radix sort of `count` (device scalar) uint64 keys over the
+ * low `bits` bits.  Two-phase scratch query as output_sort
+ * (OA/src/output.cu:26-31): call with temp == NULL to get temp_bytes.
+ * The sorted keys end in `keys` (ping-pong through temp). */
+int tj_sort_keys(void* temp, size_t* temp_bytes, uint64_t* keys, const uint64_t* count,
+                 int64_t capacity, int bits, void* stream);
+
+/* Sorted keys -> (i, j) rows of int32 or int64. */
+int tj_unpack_pairs(const uint64_t* keys, const uint64_t* count, int64_t capacity, int jbits,
+                    void* pairs, int out_format, void* stream);
+
+/* Reference output format: upper pairs of an n-point self-join -> directed
+ * u64 IDs i*n+j, both directions plus the n self pairs, sorted.
+ * Replaces output_mirror + output_sort (OA/src/output.cu:8-31) and the
+ * host canonical() of TG/src/retained_ops.py:17-19.  `ids` needs room for
+ * 2*count + n entries; temp as tj_sort_keys (query with temp == NULL). */
+int tj_canonical_u64(void* temp, size_t* temp_bytes, const int64_t* pairs_i64,
+                     const uint64_t* count, int64_t capacity, int64_t n, uint64_t* ids,
+                     void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TJOIN_H */

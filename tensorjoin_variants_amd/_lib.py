@@ -1,0 +1,104 @@
+"""ctypes binding of libtjoin.so (the C ABI declared in include/tjoin.h).
+
+The library is required: there is no CPU fallback.  Importing the package
+works without a GPU (the symbols can be inspected), but every join call goes
+through the HIP library and raises if it is missing or fails.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import re
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "libtjoin.so"
+HEADER = PKG.parent / "include" / "tjoin.h"
+
+TJ_OK = 0
+ERRORS = {-1: "TJ_ERR_INVALID", -2: "TJ_ERR_HIP", -3: "TJ_ERR_UNSUPPORTED", -4: "TJ_ERR_WORKSPACE"}
+
+PREC = {"none": 0, "fp16": 1, "fp16x3": 2, "bf16": 3, "bf16x3": 4, "fp8": 5, "fp8x3": 6}
+METHOD = {"certified": 0, "plain_fp64": 1, "grid": 2}
+OUT = {"int32": 0, "int64": 1}
+FLAG_AUDIT = 1
+FLAG_TIMING = 2
+
+STATS_SLOTS = 16
+STAT_NAMES = {
+    0: "out", 1: "ambiguous", 2: "cert_accept", 3: "fp64_accept", 4: "candidates",
+    5: "max_err", 6: "max_ratio", 7: "violations", 8: "work", 9: "invalid",
+    10: "grid_candidates", 11: "pairs", 12: "scale_exp",
+}
+DOUBLE_SLOTS = {5, 6}
+
+
+class TJError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load():
+    """Load libtjoin.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise TJError(f"{LIB_PATH} is missing: run tensorjoin_variants_amd.build.build() "
+                      "(hipcc --offload-arch=gfx950); there is no CPU fallback")
+    L = C.CDLL(str(LIB_PATH))
+    vp, i64, i32, f64, sz = C.c_void_p, C.c_int64, C.c_int, C.c_double, C.c_size_t
+    sig = {
+        "tj_version": (i32, []),
+        "tj_last_error_string": (C.c_char_p, []),
+        "tj_device_info": (i32, [C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]),
+        "tj_plan_create": (i32, [C.POINTER(vp), i64, i64, i32, i32, i32, i64, i64, C.c_uint]),
+        "tj_plan_destroy": (i32, [vp]),
+        "tj_plan_workspace_bytes": (i32, [vp, C.POINTER(sz)]),
+        "tj_plan_describe": (i32, [vp, C.c_char_p, sz]),
+        "tj_plan_stage_times": (i32, [vp, C.POINTER(C.c_float), i32]),
+        "tj_join": (i32, [vp, vp, vp, f64, vp, vp, i32, vp, vp]),
+        "tj_scale_scan": (i32, [vp, i64, i32, vp, vp, vp]),
+        "tj_params": (i32, [vp, f64, i32, i32, vp, vp, vp]),
+        "tj_rowmeta": (i32, [vp, i64, i32, i32, vp, vp, vp, vp, vp, vp]),
+        "tj_refine_fp64": (i32, [vp, vp, i32, i32, f64, vp, vp, i64, vp, vp, i64, vp, vp]),
+        "tj_plain_fp64": (i32, [vp, i64, vp, i64, i32, i32, f64, vp, i64, vp, vp]),
+        "tj_sort_keys": (i32, [vp, C.POINTER(sz), vp, vp, i64, i32, vp]),
+        "tj_unpack_pairs": (i32, [vp, vp, i64, i32, vp, i32, vp]),
+        "tj_canonical_u64": (i32, [vp, C.POINTER(sz), vp, vp, i64, i64, vp, vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def check(rc: int, what: str = "") -> None:
+    if rc != TJ_OK:
+        msg = load().tj_last_error_string().decode(errors="replace")
+        raise TJError(f"{what}: {ERRORS.get(rc, rc)}: {msg}")
+
+
+def declared_symbols() -> list[str]:
+    """Entry points declared in include/tjoin.h."""
+    text = HEADER.read_text()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(tj_[a-z0-9_]+)\s*\(", text, re.M)))
+
+
+def decode_stats(raw) -> dict:
+    """uint64 stats block -> dict (binary64 slots decoded)."""
+    import numpy as np
+    raw = np.asarray(raw, dtype=np.uint64)
+    out = {}
+    for slot, name in STAT_NAMES.items():
+        v = raw[slot]
+        if slot in DOUBLE_SLOTS:
+            out[name] = float(np.array([v], dtype=np.uint64).view(np.float64)[0])
+        elif slot == 12:
+            out[name] = int(np.array([v], dtype=np.uint64).view(np.int64)[0])
+        else:
+            out[name] = int(v)
+    return out

@@ -1,0 +1,392 @@
+// api.hip -- the C ABI of libtjoin.so: plans, the join pipeline, and the
+// exported stages.  See include/tjoin.h for the contract of each entry point.
+//
+// The pipeline of one certified join (all on the caller's stream, no host
+// synchronisation between stages, so it can be captured in a hipGraph):
+//   fill stats -> scale scan (R, S) -> parameter block -> row metadata (R, S)
+//   -> persistent MFMA screen -> FP64 refinement of the ambiguous queue
+//   -> radix sort of the output keys -> unpack to (i, j) rows.
+// It replaces the host batch loop of Matrix.run (TG/src/g2_operator_r1.py:42-61),
+// which synchronised four times per 4,096-tile batch.
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+#include <algorithm>
+#include <new>
+#include "tj_internal.h"
+
+namespace tj {
+int screen_fp16(int dp, bool self_join, const ScreenArgs& a, int grid, hipStream_t s, int query);
+int screen_fp16x3(int dp, bool self_join, const ScreenArgs& a, int grid, hipStream_t s, int query);
+int screen_bf16(int dp, bool self_join, const ScreenArgs& a, int grid, hipStream_t s, int query);
+int screen_bf16x3(int dp, bool self_join, const ScreenArgs& a, int grid, hipStream_t s, int query);
+int screen_fp8(int dp, bool self_join, const ScreenArgs& a, int grid, hipStream_t s, int query);
+int screen_fp8x3(int dp, bool self_join, const ScreenArgs& a, int grid, hipStream_t s, int query);
+int launch_mirror(const int64_t* pairs, const uint64_t* count, int64_t cap, int64_t n, uint64_t* ids, uint64_t* total,
+                  hipStream_t s);
+
+static thread_local char g_err[512] = "";
+
+void set_error(const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+}
+int hip_fail(hipError_t e, const char* what) {
+    set_error("%s: %s", what, hipGetErrorString(e));
+    return TJ_ERR_HIP;
+}
+int num_cus() {
+    static int cus = 0;
+    if (cus == 0) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+    }
+    return cus;
+}
+
+static int screen_call(int prec, int dp, bool self_join, const ScreenArgs& a, int grid, hipStream_t s, int query) {
+    switch (prec) {
+        case P_FP16: return screen_fp16(dp, self_join, a, grid, s, query);
+        case P_FP16X3: return screen_fp16x3(dp, self_join, a, grid, s, query);
+        case P_BF16: return screen_bf16(dp, self_join, a, grid, s, query);
+        case P_BF16X3: return screen_bf16x3(dp, self_join, a, grid, s, query);
+        case P_FP8: return screen_fp8(dp, self_join, a, grid, s, query);
+        case P_FP8X3: return screen_fp8x3(dp, self_join, a, grid, s, query);
+        default: set_error("unknown precision %d", prec); return TJ_ERR_INVALID;
+    }
+}
+int launch_screen(int prec, int dp, bool self_join, const ScreenArgs& a, int grid, hipStream_t s) {
+    return screen_call(prec, dp, self_join, a, grid, s, 0);
+}
+int screen_rows_per_wg(int prec, int dp) {
+    ScreenArgs dummy{};
+    return screen_call(prec, dp, false, dummy, 0, nullptr, 2);
+}
+int screen_max_grid(int prec, int dp, bool self_join) {
+    ScreenArgs dummy{};
+    int occ = screen_call(prec, dp, self_join, dummy, 0, nullptr, 1);
+    if (occ < 1) occ = 1;
+    return occ * num_cus();
+}
+}  // namespace tj
+
+using namespace tj;
+
+struct tj_plan {
+    int64_t nR, nS;
+    bool self_join;
+    int d, dp, method, prec;
+    int64_t out_cap, amb_cap;
+    unsigned flags;
+    int jbits, keybits;
+    int bm, panels, col_splits, cols_per_split, grid;
+    size_t off_maxabs, off_params, off_metaR, off_metaS, off_qRh, off_qRl, off_qSh, off_qSl, off_out, off_amb,
+        off_aux, off_sort, total;
+    hipEvent_t ev[5];
+    bool timing;
+};
+
+static size_t carve(size_t& cur, size_t bytes) {
+    size_t at = (cur + 255) / 256 * 256;
+    cur = at + bytes;
+    return at;
+}
+
+extern "C" {
+
+int tj_version(void) { return TJ_VERSION; }
+
+const char* tj_last_error_string(void) { return g_err; }
+
+int tj_device_info(int* cu_count, int* arch_major, int* arch_minor) {
+    int dev = 0;
+    hipDeviceProp_t prop;
+    TJ_HIP(hipGetDevice(&dev));
+    TJ_HIP(hipGetDeviceProperties(&prop, dev));
+    if (cu_count) *cu_count = prop.multiProcessorCount;
+    if (arch_major) *arch_major = prop.major;
+    if (arch_minor) *arch_minor = prop.minor;
+    return TJ_OK;
+}
+
+int tj_plan_create(tj_plan** out, int64_t nR, int64_t nS, int d, int method, int precision, int64_t out_capacity,
+                   int64_t amb_capacity, unsigned flags) {
+    TJ_CHECK(out != nullptr, TJ_ERR_INVALID, "plan pointer is NULL");
+    *out = nullptr;
+    TJ_CHECK(nR >= 0 && nS >= 0 && d >= 1, TJ_ERR_INVALID, "bad shape nR=%lld nS=%lld d=%d", (long long)nR,
+             (long long)nS, d);
+    TJ_CHECK(method == TJ_METHOD_CERTIFIED || method == TJ_METHOD_PLAIN_FP64, TJ_ERR_INVALID,
+             "method %d is not served by tj_join", method);
+    TJ_CHECK(out_capacity >= 0 && out_capacity < (1ll << 32), TJ_ERR_INVALID, "out_capacity must be in [0, 2^32)");
+    TJ_CHECK(amb_capacity >= 0 && amb_capacity < (1ll << 32), TJ_ERR_INVALID, "amb_capacity must be in [0, 2^32)");
+    TJ_CHECK(nR < (1ll << 31) && nS < (1ll << 31), TJ_ERR_UNSUPPORTED, "point sets beyond 2^31 rows");
+    tj_plan* p = new (std::nothrow) tj_plan();
+    TJ_CHECK(p != nullptr, TJ_ERR_INVALID, "out of host memory");
+    p->nR = nR;
+    p->self_join = (nS == 0);
+    p->nS = p->self_join ? nR : nS;
+    p->d = d;
+    p->method = method;
+    p->prec = method == TJ_METHOD_CERTIFIED ? precision : P_NONE;
+    p->out_cap = out_capacity;
+    p->amb_cap = method == TJ_METHOD_CERTIFIED ? amb_capacity : 0;
+    p->flags = flags;
+    p->jbits = bits_for(p->nS > 1 ? p->nS : 2);
+    p->keybits = bits_for(p->nR > 1 ? p->nR : 2) + p->jbits;
+    if (p->keybits > 63) {
+        const int kb = p->keybits;
+        delete p;
+        set_error("pair keys need %d bits", kb);
+        return TJ_ERR_UNSUPPORTED;
+    }
+    size_t cur = 0;
+    p->off_maxabs = carve(cur, 256);
+    p->off_params = carve(cur, sizeof(JoinParams));
+    if (method == TJ_METHOD_CERTIFIED) {
+        if (p->prec < P_FP16 || p->prec > P_FP8X3) { delete p; set_error("unknown precision %d", precision); return TJ_ERR_INVALID; }
+        p->dp = padded_dim(d, p->prec);
+        const int es = prec_esize(p->prec);
+        if (p->dp < 0 || (prec_terms(p->prec) == 3 && !prec_is_fp8(p->prec) && p->dp > 256)) {
+            delete p;
+            set_error("the screen serves d <= 512 (d <= 256 for a 16-bit hi/lo split); d=%d", d);
+            return TJ_ERR_UNSUPPORTED;
+        }
+        const bool x3 = prec_terms(p->prec) == 3;
+        p->off_metaR = carve(cur, sizeof(RowMeta) * (size_t)nR);
+        p->off_qRh = carve(cur, (size_t)nR * p->dp * es);
+        p->off_qRl = x3 ? carve(cur, (size_t)nR * p->dp * es) : p->off_qRh;
+        if (!p->self_join) {
+            p->off_metaS = carve(cur, sizeof(RowMeta) * (size_t)nS);
+            p->off_qSh = carve(cur, (size_t)nS * p->dp * es);
+            p->off_qSl = x3 ? carve(cur, (size_t)nS * p->dp * es) : p->off_qSh;
+        } else {
+            p->off_metaS = p->off_metaR;
+            p->off_qSh = p->off_qRh;
+            p->off_qSl = p->off_qRl;
+        }
+        p->off_amb = carve(cur, 8 * (size_t)(p->amb_cap > 0 ? p->amb_cap : 1));
+        p->off_aux = carve(cur, 8 * (size_t)(p->amb_cap > 0 ? p->amb_cap : 1));
+        p->bm = screen_rows_per_wg(p->prec, p->dp);
+        if (p->bm <= 0) { delete p; return TJ_ERR_UNSUPPORTED; }
+        p->panels = (int)((nR + p->bm - 1) / p->bm);
+        const int maxg = screen_max_grid(p->prec, p->dp, p->self_join);
+        if (p->self_join) {
+            p->col_splits = 1;
+            p->cols_per_split = (int)std::min<int64_t>(p->nS, 1ll << 30);
+        } else {
+            // enough items to balance the persistent grid, wide enough column
+            // ranges to amortise loading the stationary rows
+            int64_t want = (4ll * maxg + std::max(p->panels, 1) - 1) / std::max(p->panels, 1);
+            int64_t by_width = (p->nS + 4095) / 4096;
+            int64_t cs = std::max<int64_t>(1, std::min(want, by_width));
+            int64_t per = ((p->nS + cs - 1) / cs + 31) / 32 * 32;
+            p->cols_per_split = (int)std::max<int64_t>(per, 32);
+            p->col_splits = (int)((p->nS + p->cols_per_split - 1) / p->cols_per_split);
+        }
+        const int64_t items = (int64_t)p->panels * p->col_splits;
+        p->grid = (int)std::max<int64_t>(1, std::min<int64_t>(items, maxg));
+    }
+    p->off_out = carve(cur, 8 * (size_t)(p->out_cap > 0 ? p->out_cap : 1));
+    p->off_sort = carve(cur, sort_temp_bytes(p->out_cap));
+    p->total = (cur + 255) / 256 * 256;
+    p->timing = false;
+    if (flags & TJ_FLAG_TIMING) {
+        for (int k = 0; k < 5; ++k) {
+            if (hipEventCreate(&p->ev[k]) != hipSuccess) {
+                for (int q = 0; q < k; ++q) (void)hipEventDestroy(p->ev[q]);
+                delete p;
+                set_error("hipEventCreate failed");
+                return TJ_ERR_HIP;
+            }
+        }
+        p->timing = true;
+    }
+    *out = p;
+    return TJ_OK;
+}
+
+int tj_plan_destroy(tj_plan* plan) {
+    if (plan && plan->timing)
+        for (int k = 0; k < 5; ++k) (void)hipEventDestroy(plan->ev[k]);
+    delete plan;
+    return TJ_OK;
+}
+
+int tj_plan_workspace_bytes(const tj_plan* plan, size_t* bytes) {
+    TJ_CHECK(plan && bytes, TJ_ERR_INVALID, "NULL argument");
+    *bytes = plan->total;
+    return TJ_OK;
+}
+
+int tj_plan_describe(const tj_plan* p, char* buf, size_t len) {
+    TJ_CHECK(p && buf, TJ_ERR_INVALID, "NULL argument");
+    snprintf(buf, len,
+             "{\"nR\":%lld,\"nS\":%lld,\"self\":%d,\"d\":%d,\"dp\":%d,\"method\":%d,\"precision\":%d,"
+             "\"jbits\":%d,\"keybits\":%d,\"rows_per_wg\":%d,\"panels\":%d,\"col_splits\":%d,"
+             "\"cols_per_split\":%d,\"grid\":%d,\"workspace\":%zu}",
+             (long long)p->nR, (long long)p->nS, (int)p->self_join, p->d, p->dp, p->method, p->prec, p->jbits,
+             p->keybits, p->bm, p->panels, p->col_splits, p->cols_per_split, p->grid, p->total);
+    return TJ_OK;
+}
+
+int tj_plan_stage_times(tj_plan* p, float* ms, int n) {
+    TJ_CHECK(p && ms, TJ_ERR_INVALID, "NULL argument");
+    TJ_CHECK(p->timing, TJ_ERR_INVALID, "plan was created without TJ_FLAG_TIMING");
+    TJ_HIP(hipEventSynchronize(p->ev[4]));
+    for (int k = 0; k < n && k < 4; ++k) TJ_HIP(hipEventElapsedTime(&ms[k], p->ev[k], p->ev[k + 1]));
+    return TJ_OK;
+}
+
+int tj_join(tj_plan* p, const float* R, const float* S, double T, void* workspace, void* pairs, int out_format,
+            uint64_t* stats, void* stream) {
+    TJ_CHECK(p != nullptr, TJ_ERR_INVALID, "plan is NULL");
+    TJ_CHECK(workspace != nullptr && stats != nullptr, TJ_ERR_INVALID, "workspace and stats are required");
+    TJ_CHECK(R != nullptr || p->nR == 0, TJ_ERR_INVALID, "R is NULL");
+    TJ_CHECK(p->self_join || S != nullptr || p->nS == 0, TJ_ERR_INVALID, "S is NULL for an R x S plan");
+    TJ_CHECK(pairs != nullptr || p->out_cap == 0, TJ_ERR_INVALID, "pairs buffer is NULL");
+    TJ_CHECK(out_format == TJ_OUT_I32 || out_format == TJ_OUT_I64, TJ_ERR_INVALID, "bad out_format %d", out_format);
+    TJ_CHECK(out_format == TJ_OUT_I64 || (p->nR < (1ll << 31) && p->nS < (1ll << 31)), TJ_ERR_INVALID,
+             "int32 output needs fewer than 2^31 rows");
+    TJ_CHECK(T == T && T >= 0.0 && T < INFINITY, TJ_ERR_INVALID, "T must be finite and >= 0");
+    hipStream_t s = (hipStream_t)stream;
+    uint8_t* ws = (uint8_t*)workspace;
+    if (p->self_join) S = R;
+    uint32_t* maxabs = (uint32_t*)(ws + p->off_maxabs);
+    JoinParams* params = (JoinParams*)(ws + p->off_params);
+    uint64_t* out_keys = (uint64_t*)(ws + p->off_out);
+    int rc;
+    if ((rc = launch_fill_u64(stats, TJ_STATS_SLOTS, 0, s))) return rc;
+    if ((rc = launch_fill_u64((uint64_t*)maxabs, 1, 0, s))) return rc;
+    const int64_t pairs_total = p->self_join ? p->nR * (p->nR - 1) / 2 : p->nR * p->nS;
+    if ((rc = launch_fill_u64(stats + TJ_ST_PAIRS, 1, (uint64_t)pairs_total, s))) return rc;
+#define TJ_MARK(k) \
+    if (p->timing) TJ_HIP(hipEventRecord(p->ev[k], s))
+    TJ_MARK(0);
+
+    if (p->method == TJ_METHOD_CERTIFIED) {
+        RowMeta* mR = (RowMeta*)(ws + p->off_metaR);
+        RowMeta* mS = (RowMeta*)(ws + p->off_metaS);
+        if ((rc = launch_scale_scan(R, p->nR, p->d, maxabs, stats, s))) return rc;
+        if (!p->self_join && (rc = launch_scale_scan(S, p->nS, p->d, maxabs, stats, s))) return rc;
+        if ((rc = launch_params(maxabs, T, p->d, p->dp, p->prec, params, stats, s))) return rc;
+        if ((rc = launch_rowmeta(R, p->nR, p->d, p->dp, p->prec, params, ws + p->off_qRh, ws + p->off_qRl, mR, stats, s)))
+            return rc;
+        if (!p->self_join &&
+            (rc = launch_rowmeta(S, p->nS, p->d, p->dp, p->prec, params, ws + p->off_qSh, ws + p->off_qSl, mS, stats, s)))
+            return rc;
+        TJ_MARK(1);
+        if (p->nR > 0 && p->nS > 0) {
+            ScreenArgs a{};
+            a.qa_hi = ws + p->off_qRh; a.qa_lo = ws + p->off_qRl; a.ma = mR; a.na = p->nR;
+            a.qb_hi = ws + p->off_qSh; a.qb_lo = ws + p->off_qSl; a.mb = mS; a.nb = p->nS;
+            a.params = params;
+            a.out_keys = out_keys; a.out_cap = p->out_cap;
+            a.amb_keys = (uint64_t*)(ws + p->off_amb); a.amb_aux = (float*)(ws + p->off_aux); a.amb_cap = p->amb_cap;
+            a.stats = stats;
+            a.jbits = p->jbits;
+            a.panels = p->panels; a.col_splits = p->col_splits; a.cols_per_split = p->cols_per_split;
+            a.audit = (p->flags & TJ_FLAG_AUDIT) ? 1 : 0;
+            if ((rc = launch_screen(p->prec, p->dp, p->self_join, a, p->grid, s))) return rc;
+            TJ_MARK(2);
+            if ((rc = launch_refine(R, S, p->d, p->jbits, a.amb_keys, a.amb_aux, p->amb_cap, params, out_keys,
+                                    p->out_cap, stats, s)))
+                return rc;
+        } else {
+            TJ_MARK(2);
+        }
+        TJ_MARK(3);
+    } else {
+        TJ_MARK(1);
+        if ((rc = launch_plain(R, p->nR, S, p->nS, p->d, p->jbits, T, p->self_join, out_keys, p->out_cap, stats, s)))
+            return rc;
+        TJ_MARK(2);
+        TJ_MARK(3);
+    }
+    if ((rc = launch_sort(ws + p->off_sort, out_keys, stats + TJ_ST_OUT, p->out_cap, p->keybits, s))) return rc;
+    if ((rc = launch_unpack(out_keys, stats + TJ_ST_OUT, p->out_cap, p->jbits, pairs, out_format, s))) return rc;
+    TJ_MARK(4);
+#undef TJ_MARK
+    return TJ_OK;
+}
+
+int tj_scale_scan(const float* x, int64_t n, int d, uint32_t* maxabs_bits, uint64_t* stats, void* stream) {
+    TJ_CHECK(maxabs_bits && stats && (x || n == 0), TJ_ERR_INVALID, "NULL argument");
+    return launch_scale_scan(x, n, d, maxabs_bits, stats, (hipStream_t)stream);
+}
+
+int tj_params(const uint32_t* maxabs_bits, double T, int d, int precision, void* params, uint64_t* stats, void* stream) {
+    TJ_CHECK(maxabs_bits && params && stats, TJ_ERR_INVALID, "NULL argument");
+    int dp = precision == P_NONE ? d : padded_dim(d, precision);
+    TJ_CHECK(dp > 0, TJ_ERR_UNSUPPORTED, "d=%d not served at precision %d", d, precision);
+    return launch_params(maxabs_bits, T, d, dp, precision, (JoinParams*)params, stats, (hipStream_t)stream);
+}
+
+int tj_rowmeta(const float* x, int64_t n, int d, int precision, const void* params, void* q_hi, void* q_lo, float* meta,
+               uint64_t* stats, void* stream) {
+    TJ_CHECK(params && q_hi && meta && (x || n == 0), TJ_ERR_INVALID, "NULL argument");
+    int dp = padded_dim(d, precision);
+    TJ_CHECK(dp > 0, TJ_ERR_UNSUPPORTED, "d=%d not served at precision %d", d, precision);
+    TJ_CHECK(prec_terms(precision) == 1 || q_lo, TJ_ERR_INVALID, "hi/lo precision needs q_lo");
+    return launch_rowmeta(x, n, d, dp, precision, (const JoinParams*)params, q_hi, q_lo ? q_lo : q_hi, (RowMeta*)meta,
+                          stats, (hipStream_t)stream);
+}
+
+int tj_refine_fp64(const float* R, const float* S, int d, int jbits, double T, const uint64_t* amb_keys,
+                   const float* amb_aux, int64_t amb_capacity, const void* params, uint64_t* out_keys,
+                   int64_t out_capacity, uint64_t* stats, void* stream) {
+    (void)T;  // the threshold travels in the parameter block
+    TJ_CHECK(R && S && amb_keys && amb_aux && params && out_keys && stats, TJ_ERR_INVALID, "NULL argument");
+    return launch_refine(R, S, d, jbits, amb_keys, amb_aux, amb_capacity, (const JoinParams*)params, out_keys,
+                         out_capacity, stats, (hipStream_t)stream);
+}
+
+int tj_plain_fp64(const float* R, int64_t nR, const float* S, int64_t nS, int d, int jbits, double T, uint64_t* out_keys,
+                  int64_t out_capacity, uint64_t* stats, void* stream) {
+    TJ_CHECK(R && out_keys && stats, TJ_ERR_INVALID, "NULL argument");
+    const bool self_join = (S == nullptr);
+    return launch_plain(R, nR, self_join ? R : S, self_join ? nR : nS, d, jbits, T, self_join, out_keys, out_capacity,
+                        stats, (hipStream_t)stream);
+}
+
+int tj_sort_keys(void* temp, size_t* temp_bytes, uint64_t* keys, const uint64_t* count, int64_t capacity, int bits,
+                 void* stream) {
+    TJ_CHECK(temp_bytes != nullptr, TJ_ERR_INVALID, "temp_bytes is NULL");
+    TJ_CHECK(capacity >= 0 && capacity < (1ll << 32), TJ_ERR_INVALID, "capacity must be in [0, 2^32)");
+    TJ_CHECK(bits >= 0 && bits <= 64, TJ_ERR_INVALID, "bits must be in [0, 64]");
+    if (temp == nullptr) { *temp_bytes = sort_temp_bytes(capacity); return TJ_OK; }
+    TJ_CHECK(*temp_bytes >= sort_temp_bytes(capacity), TJ_ERR_WORKSPACE, "sort temp too small");
+    TJ_CHECK(keys && count, TJ_ERR_INVALID, "NULL argument");
+    return launch_sort(temp, keys, count, capacity, bits, (hipStream_t)stream);
+}
+
+int tj_unpack_pairs(const uint64_t* keys, const uint64_t* count, int64_t capacity, int jbits, void* pairs, int out_format,
+                    void* stream) {
+    TJ_CHECK(keys && count && pairs, TJ_ERR_INVALID, "NULL argument");
+    TJ_CHECK(out_format == TJ_OUT_I32 || out_format == TJ_OUT_I64, TJ_ERR_INVALID, "bad out_format");
+    return launch_unpack(keys, count, capacity, jbits, pairs, out_format, (hipStream_t)stream);
+}
+
+int tj_canonical_u64(void* temp, size_t* temp_bytes, const int64_t* pairs_i64, const uint64_t* count, int64_t capacity,
+                     int64_t n, uint64_t* ids, void* stream) {
+    TJ_CHECK(temp_bytes != nullptr, TJ_ERR_INVALID, "temp_bytes is NULL");
+    TJ_CHECK(n >= 0 && n < (1ll << 31), TJ_ERR_INVALID, "bad n");
+    const int64_t idcap = 2 * capacity + n;
+    TJ_CHECK(idcap < (1ll << 32), TJ_ERR_INVALID, "capacity too large");
+    const size_t need = sort_temp_bytes(idcap) + 256;
+    if (temp == nullptr) { *temp_bytes = need; return TJ_OK; }
+    TJ_CHECK(*temp_bytes >= need, TJ_ERR_WORKSPACE, "canonical temp too small");
+    TJ_CHECK(pairs_i64 && count && ids, TJ_ERR_INVALID, "NULL argument");
+    hipStream_t s = (hipStream_t)stream;
+    uint64_t* total = (uint64_t*)temp;
+    void* sort_temp = (uint8_t*)temp + 256;
+    int rc;
+    if ((rc = launch_mirror(pairs_i64, count, capacity, n, ids, total, s))) return rc;
+    const int bits = bits_for(n > 1 ? n * n : 2);
+    return launch_sort(sort_temp, ids, total, idcap, bits, s);
+}
+
+}  // extern "C"

@@ -1,0 +1,200 @@
+// sort.hip -- deterministic final pair order.
+//
+// Counterpart of output_mirror + output_sort (OA/src/output.cu:8-31: mirror
+// upper IDs, CUB 64-bit radix sort with a two-phase scratch query) and of
+// the host canonical() sort (TG/src/retained_ops.py:17-19).  The pair keys
+// are (i << jbits) | j, so ascending keys are ascending (i, j); since every
+// key is distinct the sorted output is unique whatever order the screen and
+// refinement appended pairs in.
+//
+// MI355X design: LSD radix sort over only the bits the key range needs
+// (8 bits per pass), written here (no rocPRIM): per-block digit histograms,
+// a one-workgroup exclusive scan, and a stable scatter in which each wave
+// ranks its 64 keys by an 8-ballot multisplit and the 4 waves of a block
+// combine through LDS.  The element count is read from device memory, so
+// the whole sort is capturable in a hipGraph behind a producer whose output
+// size is unknown at launch.
+#include "tj_internal.h"
+
+namespace {
+
+constexpr int SB = 256;     // threads per block = keys per tile
+constexpr int MAXNB = 1024; // histogram blocks
+
+__device__ inline int64_t clamp_count(const uint64_t* count, int64_t cap) {
+    int64_t n = (int64_t)*count;
+    return n < cap ? n : cap;
+}
+
+__global__ void __launch_bounds__(SB) hist_kernel(const uint64_t* __restrict__ keys, const uint64_t* count, int64_t cap,
+                                                  int shift, uint32_t* __restrict__ hist, int nb) {
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t n = clamp_count(count, cap);
+    const int64_t chunk = ((n + nb - 1) / nb + SB - 1) / SB * SB;
+    const int64_t b0 = (int64_t)blockIdx.x * chunk;
+    const int64_t b1 = b0 + chunk < n ? b0 + chunk : n;
+    for (int64_t t = b0 + threadIdx.x; t < b1; t += SB) atomicAdd(&h[(keys[t] >> shift) & 255u], 1u);
+    __syncthreads();
+    hist[(int64_t)threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
+}
+
+// Exclusive scan of 256 * nb counters (digit-major) by one workgroup.
+__global__ void __launch_bounds__(1024) scan_kernel(uint32_t* hist, int total) {
+    __shared__ uint32_t part[1024];
+    const int per = (total + 1023) / 1024;
+    const int a = threadIdx.x * per;
+    const int b = a + per < total ? a + per : total;
+    uint32_t s = 0;
+    for (int t = a; t < b; ++t) s += hist[t];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        uint32_t v = threadIdx.x >= (unsigned)o ? part[threadIdx.x - o] : 0;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = threadIdx.x ? part[threadIdx.x - 1] : 0;
+    for (int t = a; t < b; ++t) {
+        uint32_t v = hist[t];
+        hist[t] = run;
+        run += v;
+    }
+}
+
+__device__ inline int lanes_below(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+}
+
+__global__ void __launch_bounds__(SB) scatter_kernel(const uint64_t* __restrict__ src, uint64_t* __restrict__ dst,
+                                                     const uint64_t* count, int64_t cap, int shift,
+                                                     const uint32_t* __restrict__ offs, int nb) {
+    __shared__ uint32_t run[256];
+    __shared__ uint32_t wcnt[4][256];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t n = clamp_count(count, cap);
+    const int64_t chunk = ((n + nb - 1) / nb + SB - 1) / SB * SB;
+    const int64_t b0 = (int64_t)blockIdx.x * chunk;
+    const int64_t b1 = b0 + chunk < n ? b0 + chunk : n;
+    run[tid] = offs[(int64_t)tid * nb + blockIdx.x];
+    for (int64_t base = b0; base < b1; base += SB) {
+        wcnt[0][tid] = 0; wcnt[1][tid] = 0; wcnt[2][tid] = 0; wcnt[3][tid] = 0;
+        __syncthreads();
+        const int64_t t = base + tid;
+        const bool valid = t < b1;
+        uint64_t key = valid ? src[t] : 0;
+        const uint32_t dg = (uint32_t)(key >> shift) & 255u;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const bool bit = (dg >> b) & 1u;
+            const uint64_t bm = __ballot(bit);
+            peers &= bit ? bm : ~bm;
+        }
+        const int rank = lanes_below(peers);
+        if (valid && rank == 0) wcnt[wave][dg] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        if (valid) {
+            uint32_t pos = run[dg] + (uint32_t)rank;
+            for (int w = 0; w < wave; ++w) pos += wcnt[w][dg];
+            dst[pos] = key;
+        }
+        __syncthreads();
+        run[tid] += wcnt[0][tid] + wcnt[1][tid] + wcnt[2][tid] + wcnt[3][tid];
+        __syncthreads();
+    }
+}
+
+__global__ void copy_kernel(const uint64_t* __restrict__ src, uint64_t* __restrict__ dst, const uint64_t* count, int64_t cap) {
+    const int64_t n = clamp_count(count, cap);
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) dst[t] = src[t];
+}
+
+__global__ void unpack_kernel(const uint64_t* __restrict__ keys, const uint64_t* count, int64_t cap, int jbits,
+                              void* __restrict__ pairs, int fmt) {
+    const int64_t n = clamp_count(count, cap);
+    const uint64_t jmask = (1ull << jbits) - 1;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t k = keys[t];
+        if (fmt == TJ_OUT_I32) ((int2*)pairs)[t] = make_int2((int)(k >> jbits), (int)(k & jmask));
+        else ((longlong2*)pairs)[t] = make_longlong2((long long)(k >> jbits), (long long)(k & jmask));
+    }
+}
+
+// upper pairs (i < j) of an n-point self-join -> i*n+j, j*n+i, plus the n diagonal ids
+__global__ void mirror_kernel(const int64_t* __restrict__ pairs, const uint64_t* count, int64_t cap, int64_t n,
+                              uint64_t* __restrict__ ids, uint64_t* total) {
+    const int64_t m = clamp_count(count, cap);
+    const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gt == 0) *total = (uint64_t)(2 * m + n);
+    for (int64_t t = gt; t < m; t += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t i = (uint64_t)pairs[2 * t], j = (uint64_t)pairs[2 * t + 1];
+        ids[2 * t] = i * (uint64_t)n + j;
+        ids[2 * t + 1] = j * (uint64_t)n + i;
+    }
+    for (int64_t t = gt; t < n; t += (int64_t)gridDim.x * blockDim.x) ids[2 * m + t] = (uint64_t)t * (uint64_t)n + (uint64_t)t;
+}
+
+int nblocks_for(int64_t cap) {
+    int64_t nb = (cap + 8 * SB - 1) / (8 * SB);
+    if (nb < 1) nb = 1;
+    if (nb > MAXNB) nb = MAXNB;
+    return (int)nb;
+}
+
+}  // namespace
+
+namespace tj {
+
+size_t sort_temp_bytes(int64_t capacity) {
+    size_t a = ((size_t)(capacity > 0 ? capacity : 1) * 8 + 255) / 256 * 256;
+    return a + (size_t)256 * nblocks_for(capacity) * 4 + 256;
+}
+
+int launch_sort(void* temp, uint64_t* keys, const uint64_t* count, int64_t capacity, int bits, hipStream_t s) {
+    if (capacity <= 0 || bits <= 0) return TJ_OK;
+    const int nb = nblocks_for(capacity);
+    uint64_t* alt = (uint64_t*)temp;
+    uint32_t* hist = (uint32_t*)((uint8_t*)temp + ((size_t)capacity * 8 + 255) / 256 * 256);
+    const int passes = (bits + 7) / 8;
+    uint64_t* src = keys;
+    uint64_t* dst = alt;
+    for (int p = 0; p < passes; ++p) {
+        hipLaunchKernelGGL(hist_kernel, dim3(nb), dim3(SB), 0, s, src, count, capacity, 8 * p, hist, nb);
+        hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, s, hist, 256 * nb);
+        hipLaunchKernelGGL(scatter_kernel, dim3(nb), dim3(SB), 0, s, src, dst, count, capacity, 8 * p, hist, nb);
+        uint64_t* t = src; src = dst; dst = t;
+    }
+    if (src != keys) {
+        int64_t blocks = (capacity + 255) / 256;
+        if (blocks > 2048) blocks = 2048;
+        hipLaunchKernelGGL(copy_kernel, dim3((unsigned)blocks), dim3(256), 0, s, src, keys, count, capacity);
+    }
+    TJ_HIP(hipGetLastError());
+    return TJ_OK;
+}
+
+int launch_unpack(const uint64_t* keys, const uint64_t* count, int64_t cap, int jbits, void* pairs, int out_format,
+                  hipStream_t s) {
+    if (cap <= 0) return TJ_OK;
+    int64_t blocks = (cap + 255) / 256;
+    if (blocks > 2048) blocks = 2048;
+    hipLaunchKernelGGL(unpack_kernel, dim3((unsigned)blocks), dim3(256), 0, s, keys, count, cap, jbits, pairs, out_format);
+    TJ_HIP(hipGetLastError());
+    return TJ_OK;
+}
+
+int launch_mirror(const int64_t* pairs, const uint64_t* count, int64_t cap, int64_t n, uint64_t* ids, uint64_t* total,
+                  hipStream_t s) {
+    int64_t work = cap > n ? cap : n;
+    int64_t blocks = (work + 255) / 256;
+    if (blocks < 1) blocks = 1;
+    if (blocks > 2048) blocks = 2048;
+    hipLaunchKernelGGL(mirror_kernel, dim3((unsigned)blocks), dim3(256), 0, s, pairs, count, cap, n, ids, total);
+    TJ_HIP(hipGetLastError());
+    return TJ_OK;
+}
+
+}  // namespace tj

@@ -1,0 +1,104 @@
+"""C ABI of libtjoin.so without a GPU: the library loads, exports every entry
+point declared in include/tjoin.h, plans lay out workspaces, and bad
+arguments return error codes instead of aborting."""
+import ctypes as C
+import json
+
+import pytest
+
+from tensorjoin_variants_amd import _lib
+
+
+@pytest.fixture(scope="module")
+def L():
+    return _lib.load()
+
+
+def test_exports_every_declared_symbol(L):
+    declared = _lib.declared_symbols()
+    assert len(declared) >= 15
+    for name in declared:
+        assert hasattr(L, name), name
+    assert L.tj_version() >= 10000
+
+
+def _plan(L, nR, nS, d, method, prec, out=1024, amb=1024, flags=0):
+    p = C.c_void_p()
+    rc = L.tj_plan_create(C.byref(p), nR, nS, d, method, prec, out, amb, flags)
+    return rc, p
+
+
+@pytest.mark.parametrize("prec,d,dp", [("fp16", 3, 16), ("fp16", 128, 128), ("fp16", 130, 256), ("bf16", 32, 32),
+                                       ("fp8", 3, 64), ("fp8", 512, 512), ("fp16x3", 256, 256), ("fp8x3", 100, 128)])
+def test_plan_layout(L, prec, d, dp):
+    rc, p = _plan(L, 1000, 777, d, 0, _lib.PREC[prec])
+    assert rc == 0, L.tj_last_error_string()
+    nbytes = C.c_size_t()
+    assert L.tj_plan_workspace_bytes(p, C.byref(nbytes)) == 0
+    buf = C.create_string_buffer(1024)
+    assert L.tj_plan_describe(p, buf, 1024) == 0
+    desc = json.loads(buf.value.decode())
+    assert desc["dp"] == dp and desc["nR"] == 1000 and desc["nS"] == 777 and desc["self"] == 0
+    assert desc["jbits"] == 10 and desc["keybits"] == 20
+    assert desc["panels"] * desc["rows_per_wg"] >= 1000
+    assert desc["col_splits"] * desc["cols_per_split"] >= 777
+    es = 1 if prec.startswith("fp8") else 2
+    nt = 2 if prec.endswith("x3") else 1
+    assert nbytes.value >= (1000 + 777) * (dp * es * nt + 32)
+    assert L.tj_plan_destroy(p) == 0
+
+
+def test_self_join_plan(L):
+    rc, p = _plan(L, 8192, 0, 32, 0, _lib.PREC["fp16"])
+    assert rc == 0
+    buf = C.create_string_buffer(1024)
+    L.tj_plan_describe(p, buf, 1024)
+    desc = json.loads(buf.value.decode())
+    assert desc["self"] == 1 and desc["nS"] == 8192 and desc["col_splits"] == 1
+    L.tj_plan_destroy(p)
+
+
+@pytest.mark.parametrize("args,code", [
+    ((10, 10, 0, 0, 1), -1),        # d = 0
+    ((10, 10, 8, 7, 1), -1),        # unknown method
+    ((10, 10, 8, 0, 9), -1),        # unknown precision
+    ((10, 10, 600, 0, 1), -3),      # d beyond the widest FP16 screen
+    ((10, 10, 300, 0, 2), -3),      # fp16 hi/lo split beyond d = 256
+    ((-1, 10, 8, 0, 1), -1),        # negative size
+])
+def test_plan_rejects_bad_arguments(L, args, code):
+    rc, p = _plan(L, *args)
+    assert rc == code
+    assert len(L.tj_last_error_string()) > 0
+
+
+def test_capacity_bounds(L):
+    rc, _ = _plan(L, 10, 10, 8, 0, 1, out=1 << 33)
+    assert rc == -1
+
+
+def test_sort_temp_query(L):
+    need = C.c_size_t()
+    assert L.tj_sort_keys(None, C.byref(need), None, None, 1 << 20, 40, None) == 0
+    assert need.value >= 8 << 20
+    small = C.c_size_t(16)
+    assert L.tj_sort_keys(C.c_void_p(1), C.byref(small), None, None, 1 << 20, 40, None) == -4
+
+
+def test_join_rejects_bad_threshold(L):
+    rc, p = _plan(L, 10, 10, 8, 0, 1)
+    assert rc == 0
+    ws = C.c_void_p(1)
+    for T in (float("nan"), -1.0, float("inf")):
+        assert L.tj_join(p, C.c_void_p(1), C.c_void_p(1), T, ws, C.c_void_p(1), 0, C.c_void_p(1), None) == -1
+    L.tj_plan_destroy(p)
+
+
+def test_stats_decoding():
+    import numpy as np
+    raw = np.zeros(16, np.uint64)
+    raw[0] = 5
+    raw[5] = np.array([0.25]).view(np.uint64)[0]
+    raw[12] = np.array([-3]).view(np.uint64)[0]
+    s = _lib.decode_stats(raw)
+    assert s["out"] == 5 and s["max_err"] == 0.25 and s["scale_exp"] == -3

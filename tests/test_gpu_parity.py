@@ -1,0 +1,244 @@
+"""Parity of the HIP join against the CPU FP64 oracle (bit-exact pair sets).
+
+Every case prints its name before touching the GPU, so a failing run names
+its case.  Sizes keep the oracle to seconds.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+PRECS = ["fp16", "bf16", "fp8", "fp16x3", "bf16x3", "fp8x3"]
+
+
+def say(*a):
+    print("[case]", *a, file=sys.stderr, flush=True)
+
+
+@pytest.fixture(scope="module")
+def tj():
+    import torch
+    assert torch.cuda.is_available(), "no GPU visible"
+    import tensorjoin_variants_amd as T
+    T.load_library()
+    return T
+
+
+def oracle_pairs(oracle, R, S, T):
+    return oracle.join(R, S, T)
+
+
+def run_check(tj, oracle, R, S, T, prec, method="certified", audit=True, name=""):
+    say(name, prec, method, R.shape, None if S is None else S.shape, T)
+    got, st = tj.join(R, S, T=T, method=method, precision=prec, out_dtype=np.int64, audit=audit,
+                      return_stats=True)
+    want = oracle_pairs(oracle, R, S, T)
+    assert got.shape == want.shape, (name, prec, got.shape, want.shape, st)
+    assert np.array_equal(got, want), (name, prec)
+    if method == "certified":
+        assert st["violations"] == 0, st
+        if audit:
+            assert st["cert_accept"] <= st["fp64_accept"] + 0
+    return st
+
+
+def sample_T(oracle, R, S, q):
+    M = oracle.d2_matrix(R[:200], (R if S is None else S)[:200])
+    v = M[M > 0] if S is None else M.ravel()
+    return float(np.quantile(v, q)) if v.size else 1.0
+
+
+def test_device(tj):
+    import torch
+    name = torch.cuda.get_device_properties(0).gcnArchName
+    say("device", name)
+    assert "gfx950" in name
+
+
+@pytest.mark.parametrize("d", [1, 3, 32, 100])
+def test_plain_fp64(tj, oracle, d):
+    rng = np.random.default_rng(100 + d)
+    R = rng.standard_normal((1100, d)).astype(np.float32)
+    S = rng.standard_normal((900, d)).astype(np.float32)
+    T = sample_T(oracle, R, S, 0.01)
+    run_check(tj, oracle, R, S, T, "fp16", method="plain_fp64", name=f"plain rxs d{d}")
+    run_check(tj, oracle, R, None, T, "fp16", method="plain_fp64", name=f"plain self d{d}")
+
+
+@pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("d", [3, 16, 32, 100, 128, 256])
+def test_certified_random(tj, oracle, prec, d):
+    rng = np.random.default_rng(1000 + d)
+    R = rng.standard_normal((1000, d)).astype(np.float32)
+    S = rng.standard_normal((777, d)).astype(np.float32)
+    T = sample_T(oracle, R, S, 0.01)
+    run_check(tj, oracle, R, S, T, prec, name=f"rxs d{d}")
+    run_check(tj, oracle, R, None, T, prec, name=f"self d{d}")
+
+
+@pytest.mark.parametrize("prec", ["fp16", "bf16", "fp8", "fp8x3"])
+def test_certified_d512(tj, oracle, prec):
+    rng = np.random.default_rng(512)
+    R = rng.standard_t(3, (700, 512)).astype(np.float32)
+    S = rng.standard_t(3, (600, 512)).astype(np.float32)
+    T = sample_T(oracle, R, S, 0.002)
+    run_check(tj, oracle, R, S, T, prec, name="rxs d512 student-t")
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_clusters_near_threshold(tj, oracle, prec):
+    """Clustered data with T at a sampled distance: many pairs sit near the boundary."""
+    rng = np.random.default_rng(7)
+    c = rng.standard_normal((8, 24)) * 4
+    X = (c[rng.integers(0, 8, 2000)] + rng.standard_normal((2000, 24))).astype(np.float32)
+    M = oracle.d2_matrix(X[:300], X[300:600])
+    T = float(np.sort(M.ravel())[2000])  # exactly a realised distance: an equality pair
+    run_check(tj, oracle, X, None, T, prec, name="clusters T=realised d2")
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_edge_shapes(tj, oracle, prec):
+    rng = np.random.default_rng(5)
+    X = rng.standard_normal((70, 8)).astype(np.float32)
+    for nR, nS in [(1, 1), (1, 70), (70, 1), (33, 65), (257, 31)]:
+        run_check(tj, oracle, X[:nR] if nR <= 70 else rng.standard_normal((nR, 8)).astype(np.float32),
+                  X[:nS] if nS <= 70 else None, 4.0, prec, name=f"shape {nR}x{nS}")
+    Y = rng.standard_normal((257, 8)).astype(np.float32)
+    run_check(tj, oracle, Y, None, 6.0, prec, name="self 257")
+    run_check(tj, oracle, Y[:1], None, 6.0, prec, name="self 1")
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_empty(tj, prec):
+    say("empty", prec)
+    R = np.zeros((0, 16), np.float32)
+    S = np.ones((10, 16), np.float32)
+    assert tj.join(R, S, T=1.0, precision=prec).shape == (0, 2)
+    assert tj.join(S, R, T=1.0, precision=prec).shape == (0, 2)
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_adversarial_fixtures(tj, oracle, prec):
+    """Closed-form fixtures of CEJ/src/run_g3c_b_r1_adversarial.py:365-446."""
+    rng = np.random.default_rng(365)
+    base = rng.standard_normal((300, 32)).astype(np.float32)
+    T = sample_T(oracle, base, None, 0.02)
+    want = oracle.join(base, None, T)
+    say("negate", prec)
+    assert np.array_equal(tj.join(-base, None, T=T, precision=prec), want)
+    for e in (-8, 8):
+        say("scale", e, prec)
+        assert np.array_equal(tj.join(base * np.float32(2.0 ** e), None, T=T * 4.0 ** e, precision=prec), want)
+    perm = rng.permutation(32)
+    signs = rng.choice(np.array([-1, 1], np.float32), 32)
+    run_check(tj, oracle, np.ascontiguousarray(base[:, perm] * signs), None, T, prec, name="dim perm sign flip")
+    # all zero: every pair inside at T = 2^-40
+    Z = np.zeros((200, 16), np.float32)
+    say("all zero", prec)
+    got = tj.join(Z, None, T=2.0 ** -40, precision=prec)
+    assert len(got) == 200 * 199 // 2
+    # signed max accumulator: +-1 rows, block-diagonal answer at T = 0
+    X = np.ones((128, 64), np.float32)
+    X[64:] = -1
+    run_check(tj, oracle, X, None, 0.0, prec, name="signed max")
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_magnitudes(tj, oracle, prec):
+    rng = np.random.default_rng(77)
+    base = rng.standard_normal((400, 20)).astype(np.float32)
+    for scale in (1e-30, 1e-8, 1e12, 1e30):
+        X = (base * np.float32(scale)).astype(np.float32)
+        T = sample_T(oracle, X, None, 0.02)
+        run_check(tj, oracle, X, None, T, prec, name=f"scale {scale}")
+    # rows of wildly different magnitude, sparse rows, exact zeros
+    X = base * rng.choice(np.array([1e-6, 1.0, 1e6], np.float32), (400, 1))
+    X[::7] = 0
+    X[1::5, 3:] = 0
+    X = X.astype(np.float32)
+    T = sample_T(oracle, X, None, 0.05)
+    run_check(tj, oracle, X, None, T, prec, name="mixed magnitudes")
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_known_answers_on_gpu(tj, oracle, prec):
+    import json
+    from pathlib import Path
+    gold = json.loads((Path(__file__).parent / "golden" / "kat.json").read_text())["cases"]
+    for c in gold:
+        x = np.asarray(c["x"], np.float32)[None]
+        y = np.asarray(c["y"], np.float32)[None]
+        say("kat", c["name"], prec)
+        got = tj.join(x, y, T=c["T"], precision=prec, out_dtype=np.int64)
+        assert len(got) == int(c["accept"]), c["name"]
+
+
+def test_non_finite_rejected(tj):
+    X = np.ones((10, 4), np.float32)
+    X[3, 2] = np.nan
+    say("nan input")
+    with pytest.raises(ValueError):
+        tj.join(X, None, T=1.0)
+
+
+def test_capacity_retry(tj, oracle):
+    from tensorjoin_variants_amd.join import Plan, _as_device
+    rng = np.random.default_rng(9)
+    X = rng.standard_normal((500, 8)).astype(np.float32)
+    T = sample_T(oracle, X, None, 0.2)
+    say("capacity retry")
+    plan = Plan(500, None, 8, "certified", "fp16", out_capacity=16, amb_capacity=16)
+    res = plan.run(_as_device(X), None, T)
+    assert plan.out_capacity >= res.stats["out"] > 16
+    assert np.array_equal(res.numpy().astype(np.int64), oracle.join(X, None, T))
+
+
+def test_sort_keys(tj):
+    import ctypes as C
+    import torch
+    from tensorjoin_variants_amd import _lib
+    L = _lib.load()
+    rng = np.random.default_rng(3)
+    for n, bits in [(1, 5), (1000, 12), (300_001, 40), (70_000, 63)]:
+        say("sort", n, bits)
+        keys = rng.integers(0, 2 ** bits, n, dtype=np.uint64)
+        cap = n + 100
+        dev = torch.zeros(cap, dtype=torch.int64, device="cuda")
+        dev[:n] = torch.from_numpy(keys.view(np.int64))
+        cnt = torch.tensor([n], dtype=torch.int64, device="cuda")
+        need = C.c_size_t()
+        assert L.tj_sort_keys(None, C.byref(need), None, None, cap, bits, None) == 0
+        temp = torch.empty(need.value, dtype=torch.uint8, device="cuda")
+        s = torch.cuda.current_stream().cuda_stream
+        assert L.tj_sort_keys(temp.data_ptr(), C.byref(need), dev.data_ptr(), cnt.data_ptr(), cap, bits, s) == 0
+        torch.cuda.synchronize()
+        assert np.array_equal(dev[:n].cpu().numpy().view(np.uint64), np.sort(keys))
+
+
+def test_canonical_and_matrix(tj, oracle):
+    rng = np.random.default_rng(21)
+    X = rng.standard_normal((600, 16)).astype(np.float32)
+    T = sample_T(oracle, X, None, 0.01)
+    want = oracle.canonical_u64(oracle.join(X, None, T), 600)
+    for m in ("F16", "S16", "F8", "S8", "F16X3", "F32"):
+        say("Matrix.run", m)
+        ids, stats = tj.Matrix(T).run(m, X)
+        assert np.array_equal(ids, want), m
+        assert int(np.count_nonzero(ids // 600 == ids % 600)) == 600
+
+
+def test_clusters_selfjoin_full(tj, oracle):
+    """Workload 1 at full size: 8192 points, d = 32, int32 i<j output."""
+    from tensorjoin_variants_amd import workloads
+    w = workloads.clusters_selfjoin()
+    want = oracle.join(w.R, None, w.T)
+    for prec in ("fp16", "fp8", "fp16x3", "bf16"):
+        say("clusters_selfjoin full", prec)
+        got, st = tj.join(w.R, None, T=w.T, precision=prec, out_dtype=np.int32, audit=True, return_stats=True)
+        assert got.dtype == np.int32
+        assert np.array_equal(got.astype(np.int64), want), prec
+        assert st["violations"] == 0
+    say("clusters_selfjoin plain")
+    assert np.array_equal(tj.join(w.R, None, T=w.T, method="plain_fp64").astype(np.int64), want)
