@@ -74,6 +74,9 @@ extern "C" {
 #define TJ_ST_GRID_CAND 10    /* candidate evaluations of the grid join */
 #define TJ_ST_PAIRS 11        /* pair evaluations the call covers */
 #define TJ_ST_SCALE_EXP 12    /* join-unit exponent G (x_join = x * 2^-G), as int64 */
+#define TJ_ST_VIOL_KEY 13      /* diagnostic: key of a refined pair outside its interval (flag bit 63 = certified) */
+#define TJ_ST_VIOL_D2 14       /* diagnostic: its binary64 d2 (join units, binary64 bits) */
+#define TJ_ST_VIOL_EST 15      /* diagnostic: its screen estimate and half-width (two binary32) */
 #define TJ_STATS_SLOTS 16
 
 /* ---------------------------------------------------------------- library */
@@ -161,6 +164,17 @@ int tj_unpack_pairs(const uint64_t* keys, const uint64_t* count, int64_t capacit
 int tj_canonical_u64(void* temp, size_t* temp_bytes, const int64_t* pairs_i64,
                      const uint64_t* count, int64_t capacity, int64_t n, uint64_t* ids,
                      void* stream);
+
+/* ------------------------------------------------------------ diagnostics
+ * MFMA probe: nblocks independent 32x32 tiles, each the product of rows
+ * A[blk*32 + i] and B[blk*32 + j] (row-major, ksteps MFMA steps of 16
+ * (16-bit) or 64 (fp8) elements), accumulated exactly as the screen does;
+ * fp8 rows carry power-of-two scale exponents scale_a / scale_b.
+ * out[blk][i][j] (binary32).  Used to validate the accumulation model of the
+ * certificate and the output layout on the device.  No counterpart in the
+ * reference (its premise was inherited, F16C/NUMERICAL_SCOPE.md:11-43). */
+int tj_mfma_probe(int precision, const void* A, const void* B, const int* scale_a,
+                  const int* scale_b, int ksteps, int nblocks, float* out, void* stream);
 
 #ifdef __cplusplus
 }

@@ -27,9 +27,9 @@ STATS_SLOTS = 16
 STAT_NAMES = {
     0: "out", 1: "ambiguous", 2: "cert_accept", 3: "fp64_accept", 4: "candidates",
     5: "max_err", 6: "max_ratio", 7: "violations", 8: "work", 9: "invalid",
-    10: "grid_candidates", 11: "pairs", 12: "scale_exp",
+    10: "grid_candidates", 11: "pairs", 12: "scale_exp", 13: "viol_key", 14: "viol_d2", 15: "viol_est",
 }
-DOUBLE_SLOTS = {5, 6}
+DOUBLE_SLOTS = {5, 6, 14}
 
 
 class TJError(RuntimeError):
@@ -66,6 +66,7 @@ def load():
         "tj_plain_fp64": (i32, [vp, i64, vp, i64, i32, i32, f64, vp, i64, vp, vp]),
         "tj_sort_keys": (i32, [vp, C.POINTER(sz), vp, vp, i64, i32, vp]),
         "tj_unpack_pairs": (i32, [vp, vp, i64, i32, vp, i32, vp]),
+        "tj_mfma_probe": (i32, [i32, vp, vp, vp, vp, i32, i32, vp, vp]),
         "tj_canonical_u64": (i32, [vp, C.POINTER(sz), vp, vp, i64, i64, vp, vp]),
     }
     for name, (res, args) in sig.items():
@@ -99,6 +100,8 @@ def decode_stats(raw) -> dict:
             out[name] = float(np.array([v], dtype=np.uint64).view(np.float64)[0])
         elif slot == 12:
             out[name] = int(np.array([v], dtype=np.uint64).view(np.int64)[0])
+        elif slot == 15:
+            out[name] = [float(f) for f in np.array([v], dtype=np.uint64).view(np.float32)]
         else:
             out[name] = int(v)
     return out

@@ -69,7 +69,6 @@ __global__ void __launch_bounds__(256) refine_kernel(const float* __restrict__ R
             const uint64_t i = key >> jbits, j = key & jmask;
             const double d2 = d2_seq(R + i * (uint64_t)d, S + j * (uint64_t)d, d);
             inside = d2 <= P.T;
-            if (certified && !inside) viol++;
             const float2 aux = amb_aux[t];
             const double dj = d2 / P.scale_real;  // exact power-of-two scaling
             const double err = fabs(dj - (double)aux.x);
@@ -77,7 +76,15 @@ __global__ void __launch_bounds__(256) refine_kernel(const float* __restrict__ R
             if (aux.y > 0.0f) max_ratio = fmax(max_ratio, err / (double)aux.y);
             // the certified half-width covers the exact d2; allow the binary64
             // gap g and the float rounding of the centre
-            if (err > (double)aux.y + 2.0 * P.g64 * dj + 0x1p-23 * fabs((double)aux.x)) viol++;
+            const bool bad = (certified && !inside) ||
+                             err > (double)aux.y + 2.0 * P.g64 * dj + 0x1p-23 * fabs((double)aux.x);
+            if (bad) {
+                viol++;
+                stats[TJ_ST_VIOL_KEY] = raw;
+                stats[TJ_ST_VIOL_D2] = __builtin_bit_cast(uint64_t, dj);
+                stats[TJ_ST_VIOL_EST] = (uint64_t)__builtin_bit_cast(uint32_t, aux.x) |
+                                        ((uint64_t)__builtin_bit_cast(uint32_t, aux.y) << 32);
+            }
         }
         const uint64_t m = __ballot(inside);
         if (m) {

@@ -1,0 +1,246 @@
+"""Device check of the MFMA accumulation model and output layout the
+certificate relies on (tj_mfma_probe).
+
+For each instruction the measured error |p - sum a b| is compared with
+gamma * sum|a b| + alpha, gamma = m u / (1 - m u), u = 2^-23, m = 2k + 2
+(k products per accumulator chain), alpha = m 2^-126.  The exact sum is the
+binary64 matmul of the decoded operands (its own error, ~k 2^-53, is far
+below the 2^-23 scale being measured).  Inputs are adversarial for
+accumulation: random, cancelling, wide exponent spread, equal products.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[case]", *a, file=sys.stderr, flush=True)
+
+
+# ------------------------------------------------------------ host codecs
+def e4m3_decode(c: np.ndarray) -> np.ndarray:
+    c = c.astype(np.int64)
+    s = np.where(c & 0x80, -1.0, 1.0)
+    e = (c >> 3) & 15
+    m = c & 7
+    v = np.where(e == 0, m / 8.0 * 2.0 ** -6, (1 + m / 8.0) * np.exp2(e - 7.0))
+    return s * v
+
+
+def e4m3_normal_codes(rng, shape, emin=1, emax=15):
+    e = rng.integers(emin, emax + 1, shape)
+    m = rng.integers(0, 8, shape)
+    m = np.where(e == 15, np.minimum(m, 6), m)  # 0x7F / 0xFF are NaN
+    s = rng.integers(0, 2, shape)
+    return ((s << 7) | (e << 3) | m).astype(np.uint8)
+
+
+def bf16_bits(x: np.ndarray) -> np.ndarray:
+    u = x.astype(np.float32).view(np.uint32)
+    return ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+
+
+def bf16_decode(b: np.ndarray) -> np.ndarray:
+    return (b.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+
+
+@pytest.fixture(scope="module")
+def probe():
+    import torch
+    from tensorjoin_variants_amd import _lib
+    L = _lib.load()
+
+    def run(prec, A_bytes, B_bytes, sa, sb, ksteps, nblk):
+        dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+        A, B = dev(A_bytes.view(np.uint8)), dev(B_bytes.view(np.uint8))
+        SA = dev(sa.astype(np.int32)) if sa is not None else None
+        SB = dev(sb.astype(np.int32)) if sb is not None else None
+        out = torch.empty((nblk, 32, 32), dtype=torch.float32, device="cuda")
+        rc = L.tj_mfma_probe(_lib.PREC[prec], A.data_ptr(), B.data_ptr(),
+                             None if SA is None else SA.data_ptr(), None if SB is None else SB.data_ptr(),
+                             ksteps, nblk, out.data_ptr(), torch.cuda.current_stream().cuda_stream)
+        _lib.check(rc, "tj_mfma_probe")
+        torch.cuda.synchronize()
+        return out.cpu().numpy().astype(np.float64)
+    return run
+
+
+def patterns(rng, nblk, K, gen):
+    """Yield (name, A values, B values) float arrays [nblk, 32, K] from a sampler gen(shape)."""
+    a, b = gen((nblk, 32, K)), gen((nblk, 32, K))
+    yield "random", a, b
+    # cancelling: b_k = +-|a_k|-scaled so products alternate sign at similar size
+    c = gen((nblk, 32, K))
+    yield "cancel", c, np.where(np.arange(K) % 2 == 0, 1.0, -1.0) * np.abs(c)
+    # wide spread: one large product, the rest tiny
+    big = gen((nblk, 32, K))
+    yield "spread", big, big
+    yield "equal", np.abs(a), np.abs(a)
+
+
+FP8_WINDOW = 2.0 ** -9  # the premise's extra FP8 term (tj_internal params: gamma_acc += 2^-9)
+
+
+def check(prec, got, A, B, sa, sb, K):
+    scale_a = np.exp2(sa)[..., None] if sa is not None else 1.0
+    scale_b = np.exp2(sb)[..., None] if sb is not None else 1.0
+    Aa, Bb = A * scale_a, B * scale_b
+    exact = np.matmul(Aa, np.swapaxes(Bb, 1, 2))
+    absum = np.matmul(np.abs(Aa), np.swapaxes(np.abs(Bb), 1, 2))
+    m = 2 * K + 2
+    mu = m * 2.0 ** -23
+    gamma = mu / (1 - mu) + (FP8_WINDOW if prec == "fp8" else 0.0)
+    alpha = m * 2.0 ** -126
+    err = np.abs(got - exact)
+    bound = gamma * absum + alpha
+    ratio = err / np.maximum(2.0 ** -23 * absum, 1e-300)
+    return float((err / bound).max()), float(ratio.max()), int((err > bound).sum())
+
+
+@pytest.mark.parametrize("K", [16, 128, 512])
+def test_fp16_model(probe, K):
+    rng = np.random.default_rng(K)
+    nblk = 8
+    gen = lambda shape: (rng.standard_normal(shape) * np.exp2(rng.integers(-12, 12, shape))).astype(np.float16).astype(np.float64)
+    for name, A, B in patterns(rng, nblk, K, gen):
+        A16, B16 = A.astype(np.float16), B.astype(np.float16)
+        A, B = A16.astype(np.float64), B16.astype(np.float64)
+        say("fp16", K, name)
+        got = probe("fp16", A16, B16, None, None, K // 16, nblk)
+        worst, ulps, nbad = check("fp16", got, A, B, None, None, K)
+        print(f"fp16 K={K} {name}: max err/bound {worst:.3g}, max err/(2^-23 sum|ab|) {ulps:.3g}")
+        assert nbad == 0, (name, worst)
+
+
+@pytest.mark.parametrize("K", [16, 128, 512])
+def test_bf16_model(probe, K):
+    rng = np.random.default_rng(100 + K)
+    nblk = 8
+    gen = lambda shape: bf16_decode(bf16_bits(rng.standard_normal(shape) * np.exp2(rng.integers(-30, 30, shape))))
+    for name, A, B in patterns(rng, nblk, K, gen):
+        Ab, Bb = bf16_bits(A), bf16_bits(B)
+        A, B = bf16_decode(Ab), bf16_decode(Bb)
+        say("bf16", K, name)
+        got = probe("bf16", Ab, Bb, None, None, K // 16, nblk)
+        worst, ulps, nbad = check("bf16", got, A, B, None, None, K)
+        print(f"bf16 K={K} {name}: max err/bound {worst:.3g}, max err/(2^-23 sum|ab|) {ulps:.3g}")
+        assert nbad == 0, (name, worst)
+
+
+@pytest.mark.parametrize("K", [64, 128, 512])
+def test_fp8_scaled_model(probe, K):
+    rng = np.random.default_rng(200 + K)
+    nblk = 8
+    for name in ("random", "cancel", "spread", "equal", "small_exp"):
+        if name == "small_exp":
+            Ac, Bc = e4m3_normal_codes(rng, (nblk, 32, K), 1, 4), e4m3_normal_codes(rng, (nblk, 32, K), 1, 4)
+        else:
+            Ac, Bc = e4m3_normal_codes(rng, (nblk, 32, K)), e4m3_normal_codes(rng, (nblk, 32, K))
+        A, B = e4m3_decode(Ac), e4m3_decode(Bc)
+        if name == "cancel":
+            Bc = np.where(np.arange(K) % 2 == 0, Ac & 0x7F, Ac | 0x80).astype(np.uint8)
+            B = e4m3_decode(Bc)
+        if name == "equal":
+            Ac = Ac & 0x7F
+            Bc = Ac.copy()
+            A, B = e4m3_decode(Ac), e4m3_decode(Bc)
+        sa = rng.integers(-20, 20, (nblk, 32))
+        sb = rng.integers(-20, 20, (nblk, 32))
+        say("fp8", K, name)
+        got = probe("fp8", Ac, Bc, sa, sb, K // 64, nblk)
+        worst, ulps, nbad = check("fp8", got, A, B, sa, sb, K)
+        print(f"fp8 K={K} {name}: max err/bound {worst:.3g}, max err/(2^-23 sum|ab|) {ulps:.3g}")
+        assert nbad == 0, (name, worst)
+
+
+def test_output_layout(probe):
+    """One-hot rows: p[i][j] = a_i . b_j identifies the (lane, register) -> (row, col) map."""
+    K = 16
+    A = np.zeros((1, 32, K), np.float16)
+    B = np.zeros((1, 32, K), np.float16)
+    for i in range(32):
+        A[0, i, i % K] = i + 1
+        B[0, i, i % K] = 1.0
+    say("layout")
+    got = probe("fp16", A, B, None, None, 1, 1)
+    want = np.matmul(A.astype(np.float64), np.swapaxes(B.astype(np.float64), 1, 2))
+    assert np.array_equal(got, want)
+
+
+def _fp8_code(v: float) -> int:
+    """e4m3 code of an exactly representable normal value."""
+    s = 0x80 if v < 0 else 0
+    a = abs(v)
+    e = int(np.floor(np.log2(a)))
+    m = int(round((a / 2.0 ** e - 1) * 8))
+    assert (1 + m / 8) * 2.0 ** e == a and -6 <= e <= 8
+    return s | ((e + 7) << 3) | m
+
+
+def test_fp8_alignment_profile(probe):
+    """One dominant product per row and 63 products 2^-k smaller: maps how the
+    scaled FP8 instruction aligns and truncates inside one MFMA."""
+    K = 64
+    ks = list(range(0, 27, 2))
+    nblk = len(ks)
+    Ac = np.zeros((nblk, 32, K), np.uint8)
+    Bc = np.zeros((nblk, 32, K), np.uint8)
+    big = _fp8_code(15.0)
+    for b, k in enumerate(ks):
+        # small products (1.875 * 2^e)^2 ~ 2^(2e + 1.8) with 2e = 8 - k
+        e = (8 - k) // 2
+        small = _fp8_code(1.875 * 2.0 ** e) if -6 <= e <= 8 else 0
+        Ac[b, :, 0] = big
+        Bc[b, :, 0] = big
+        Ac[b, :, 1:] = small
+        Bc[b, :, 1:] = small
+    A, B = e4m3_decode(Ac), e4m3_decode(Bc)
+    zeros = np.zeros((nblk, 32), np.int64)
+    got = probe("fp8", Ac, Bc, zeros, zeros, 1, nblk)
+    got_plain = probe_plain(probe, Ac, Bc)
+    exact = np.einsum("brk,brk->br", A, B)
+    absmax = np.abs(A[:, :, 0] * B[:, :, 0])
+    absum = np.einsum("brk,brk->br", np.abs(A), np.abs(B))
+    for b, k in enumerate(ks):
+        e_s = np.abs(np.diagonal(got[b]) - exact[b]).max() / absmax[b].max()
+        e_p = np.abs(np.diagonal(got_plain[b]) - exact[b]).max() / absmax[b].max()
+        print(f"fp8 align k={k:2d}: scaled err/max = 2^{np.log2(max(e_s, 1e-30)):6.1f}   plain err/max = 2^{np.log2(max(e_p, 1e-30)):6.1f}")
+        # the premise: |p - sum ab| <= (2^-9 + (2K+2) 2^-23) sum |ab|
+        lim = (FP8_WINDOW + (2 * K + 2) * 2.0 ** -23) * absum[b]
+        assert np.all(np.abs(np.diagonal(got[b]) - exact[b]) <= lim)
+        assert np.all(np.abs(np.diagonal(got_plain[b]) - exact[b]) <= lim)
+
+
+def probe_plain(probe, Ac, Bc):
+    import torch
+    from tensorjoin_variants_amd import _lib
+    nblk, _, K = Ac.shape
+    # repack: the non-scaled instruction consumes 16 elements per step with the
+    # same (lane half -> 8 contiguous bytes) convention; any fixed permutation of
+    # k shared by A and B leaves the dot product unchanged
+    A = torch.from_numpy(np.ascontiguousarray(Ac)).to("cuda")
+    B = torch.from_numpy(np.ascontiguousarray(Bc)).to("cuda")
+    out = torch.empty((nblk, 32, 32), dtype=torch.float32, device="cuda")
+    _lib.check(_lib.load().tj_mfma_probe(100, A.data_ptr(), B.data_ptr(), None, None, K // 16, nblk,
+                                         out.data_ptr(), torch.cuda.current_stream().cuda_stream), "probe")
+    torch.cuda.synchronize()
+    return out.cpu().numpy().astype(np.float64)
+
+
+@pytest.mark.parametrize("K", [16, 64, 512])
+def test_fp8_plain_model(probe, K):
+    rng = np.random.default_rng(300 + K)
+    nblk = 8
+    for name in ("random", "cancel"):
+        Ac, Bc = e4m3_normal_codes(rng, (nblk, 32, K)), e4m3_normal_codes(rng, (nblk, 32, K))
+        if name == "cancel":
+            Bc = np.where(np.arange(K) % 2 == 0, Ac & 0x7F, Ac | 0x80).astype(np.uint8)
+        A, B = e4m3_decode(Ac), e4m3_decode(Bc)
+        say("fp8 plain", K, name)
+        got = probe_plain(probe, Ac, Bc)
+        worst, ulps, nbad = check("fp8", got, A, B, None, None, K)
+        print(f"fp8-plain K={K} {name}: max err/bound {worst:.3g}, max err/(2^-23 sum|ab|) {ulps:.3g}")
+        assert nbad == 0, (name, worst)

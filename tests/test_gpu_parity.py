@@ -171,6 +171,10 @@ def test_known_answers_on_gpu(tj, oracle, prec):
         x = np.asarray(c["x"], np.float32)[None]
         y = np.asarray(c["y"], np.float32)[None]
         say("kat", c["name"], prec)
+        if prec in ("fp16x3", "bf16x3") and x.shape[1] > 256:
+            with pytest.raises(tj.TJError):  # the 16-bit hi/lo split serves d <= 256
+                tj.join(x, y, T=c["T"], precision=prec)
+            continue
         got = tj.join(x, y, T=c["T"], precision=prec, out_dtype=np.int64)
         assert len(got) == int(c["accept"]), c["name"]
 
