@@ -108,7 +108,7 @@ def main() -> int:
     wall = time.perf_counter() - t0
     dev_ms = ev0.elapsed_time(ev1) / args.steps
     stats = __import__("tensorjoin_variants_amd._lib", fromlist=["x"]).decode_stats(plan._stats.cpu().numpy().view(np.uint64))
-    assert stats["out"] <= plan.out_capacity and stats["ambiguous"] <= plan.amb_capacity
+    assert stats["out"] <= plan.out_capacity and stats["candidates"] <= plan.amb_capacity
     assert stats["violations"] == 0
     ms_per_step = wall * 1e3 / args.steps
     value = pairs_total / (ms_per_step / 1e3)

@@ -62,10 +62,10 @@ extern "C" {
  * fp64, reject, bitwise-equal] of CEJ/src/run_g3c_b_r1_gpu_cascade.py:108-129,
  * extended with the screen error statistics. */
 #define TJ_ST_OUT 0           /* pairs written to the output (certified + refined) */
-#define TJ_ST_AMB 1           /* ambiguous pairs queued for FP64 */
+#define TJ_ST_AMB 1           /* ambiguous pairs decided by the FP64 refinement */
 #define TJ_ST_CERT_ACCEPT 2   /* pairs certified inside by the screen */
 #define TJ_ST_FP64_ACCEPT 3   /* ambiguous pairs the FP64 refinement accepted */
-#define TJ_ST_CANDIDATES 4    /* pairs that reached the per-pair interval test */
+#define TJ_ST_CANDIDATES 4    /* pairs the screen queued for the per-pair interval test (the amb_capacity queue) */
 #define TJ_ST_MAX_ERR 5       /* max |d2_screen - d2_fp64| over refined pairs (binary64 bits, real units) */
 #define TJ_ST_MAX_RATIO 6     /* max |d2_screen - d2_fp64| / certified half-width (binary64 bits) */
 #define TJ_ST_VIOLATIONS 7    /* refined pairs outside their certified interval (must be 0) */

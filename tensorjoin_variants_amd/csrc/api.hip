@@ -169,7 +169,7 @@ int tj_plan_create(tj_plan** out, int64_t nR, int64_t nS, int d, int method, int
             p->off_qSl = p->off_qRl;
         }
         p->off_amb = carve(cur, 8 * (size_t)(p->amb_cap > 0 ? p->amb_cap : 1));
-        p->off_aux = carve(cur, 8 * (size_t)(p->amb_cap > 0 ? p->amb_cap : 1));
+        p->off_aux = carve(cur, 4 * (size_t)(p->amb_cap > 0 ? p->amb_cap : 1));
         p->bm = screen_rows_per_wg(p->prec, p->dp);
         if (p->bm <= 0) { delete p; return TJ_ERR_UNSUPPORTED; }
         p->panels = (int)((nR + p->bm - 1) / p->bm);
@@ -292,8 +292,8 @@ int tj_join(tj_plan* p, const float* R, const float* S, double T, void* workspac
             a.audit = (p->flags & TJ_FLAG_AUDIT) ? 1 : 0;
             if ((rc = launch_screen(p->prec, p->dp, p->self_join, a, p->grid, s))) return rc;
             TJ_MARK(2);
-            if ((rc = launch_refine(R, S, p->d, p->jbits, a.amb_keys, a.amb_aux, p->amb_cap, params, out_keys,
-                                    p->out_cap, stats, s)))
+            if ((rc = launch_certify(R, S, p->d, p->jbits, a.amb_keys, a.amb_aux, p->amb_cap, mR, mS, params, out_keys,
+                                     p->out_cap, stats, a.audit, s)))
                 return rc;
         } else {
             TJ_MARK(2);

@@ -20,6 +20,10 @@ namespace {
 
 constexpr int TM = 128, KC = 16;
 
+__device__ void plain_tile(const float* __restrict__ R, int64_t nR, const float* __restrict__ S, int64_t nS, int d,
+                           int jbits, double T, int self_join, int64_t bi, int64_t bj, uint64_t* __restrict__ out_keys,
+                           int64_t out_cap, uint64_t* __restrict__ stats, double (*sR)[TM], double (*sS)[TM]);
+
 __global__ void __launch_bounds__(256) plain_kernel(const float* __restrict__ R, int64_t nR, const float* __restrict__ S,
                                                     int64_t nS, int d, int jbits, double T, int self_join,
                                                     int64_t tiles_r, int64_t tiles_s, int64_t per_xcd,
@@ -27,11 +31,20 @@ __global__ void __launch_bounds__(256) plain_kernel(const float* __restrict__ R,
                                                     uint64_t* __restrict__ stats) {
     __shared__ double sR[KC][TM];
     __shared__ double sS[KC][TM];
-    const int64_t bid = blockIdx.x;
-    const int64_t t = (bid % 8) * per_xcd + bid / 8;
-    if (t >= tiles_r * tiles_s) return;
-    const int64_t bi = t / tiles_s, bj = t % tiles_s;
-    if (self_join && bj < bi) return;
+    // persistent: workgroup g walks tiles g, g + grid, ...; the walk order is
+    // XCD-contiguous (dispatch places workgroup g on XCD g % 8)
+    for (int64_t vt = blockIdx.x; vt < per_xcd * 8; vt += gridDim.x) {
+        const int64_t t = (vt % 8) * per_xcd + vt / 8;
+        if (t >= tiles_r * tiles_s) continue;
+        const int64_t bi = t / tiles_s, bj = t % tiles_s;
+        if (self_join && bj < bi) continue;
+        plain_tile(R, nR, S, nS, d, jbits, T, self_join, bi, bj, out_keys, out_cap, stats, sR, sS);
+    }
+}
+
+__device__ void plain_tile(const float* __restrict__ R, int64_t nR, const float* __restrict__ S, int64_t nS, int d,
+                           int jbits, double T, int self_join, int64_t bi, int64_t bj, uint64_t* __restrict__ out_keys,
+                           int64_t out_cap, uint64_t* __restrict__ stats, double (*sR)[TM], double (*sS)[TM]) {
     const int tid = threadIdx.x;
     const int ty = tid >> 4, tx = tid & 15;
     const int64_t r0 = bi * TM, c0 = bj * TM;
@@ -103,8 +116,9 @@ int launch_plain(const float* R, int64_t nR, const float* S, int64_t nS, int d, 
     const int64_t tr = (nR + TM - 1) / TM, ts = (nS + TM - 1) / TM;
     const int64_t tiles = tr * ts;
     const int64_t per_xcd = (tiles + 7) / 8;
-    const int64_t grid = per_xcd * 8;
-    TJ_CHECK(grid < (1ll << 31), TJ_ERR_UNSUPPORTED, "plain: %lld tiles exceed one launch", (long long)grid);
+    int64_t grid = per_xcd * 8;
+    const int64_t maxg = (int64_t)num_cus() * 4;
+    if (grid > maxg) grid = maxg;
     hipLaunchKernelGGL(plain_kernel, dim3((unsigned)grid), dim3(256), 0, s, R, nR, S, nS, d, jbits, T, (int)self_join,
                        tr, ts, per_xcd, out_keys, out_cap, stats);
     TJ_HIP(hipGetLastError());

@@ -18,27 +18,6 @@
 
 namespace {
 
-__device__ inline double d2_seq(const float* __restrict__ x, const float* __restrict__ y, int d) {
-    double s = 0.0;
-    int k = 0;
-    if (((((uintptr_t)x) | ((uintptr_t)y)) & 15) == 0) {
-        for (; k + 4 <= d; k += 4) {
-            float4 a = *(const float4*)(x + k);
-            float4 b = *(const float4*)(y + k);
-            double t;
-            t = (double)a.x - (double)b.x; s = s + t * t;
-            t = (double)a.y - (double)b.y; s = s + t * t;
-            t = (double)a.z - (double)b.z; s = s + t * t;
-            t = (double)a.w - (double)b.w; s = s + t * t;
-        }
-    }
-    for (; k < d; ++k) {
-        double t = (double)x[k] - (double)y[k];
-        s = s + t * t;
-    }
-    return s;
-}
-
 __device__ inline void atomic_max_pos_double(uint64_t* slot, double v) {
     // non-negative binary64 values order like their bit patterns
     if (v > 0.0) atomicMax((unsigned long long*)slot, (unsigned long long)__builtin_bit_cast(uint64_t, v));

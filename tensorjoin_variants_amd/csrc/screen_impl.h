@@ -38,8 +38,6 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef int i32x8 __attribute__((ext_vector_type(8)));
 
-constexpr int ACAP = 256;  // per-wave staged accepted keys
-constexpr int MCAP = 128;  // per-wave staged ambiguous entries
 
 template <int PREC>
 struct Cfg {
@@ -88,13 +86,16 @@ __device__ inline int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 // Exact per-pair interval in binary64 (join units) and its class:
 // 1 = certified inside, 0 = certified outside, 2 = ambiguous.
-template <int TERMS>
+//   B_ij = gamma (H_i H_j [+ H_i L_j + L_i H_j]) + L_i L_j + Z_i E_j + E_i Z_j + E_i E_j + alpha
+//   d2 in [nlo_i + nlo_j - 2p - 2B, nhi_i + nhi_j - 2p + 2B]
+// widened by the binary64 rounding of this evaluation; the oracle's own
+// gap g is charged in Tj_acc / Tj_rej.
 __device__ inline int classify(float p, const RowMeta& mi, const RowMeta& mj, const JoinParams& P,
                                float* est, float* hw) {
     const double pd = (double)p;
     const double Hi = mi.m0.z, Ei = mi.m0.w, Hj = mj.m0.z, Ej = mj.m0.w;
     double B;
-    if (TERMS == 1) {
+    if (P.terms == 1) {
         B = P.gamma_acc * Hi * Hj + Hi * Ej + Ei * Hj + Ei * Ej + P.alpha;
     } else {
         const double Li = mi.m1.x, Lj = mj.m1.x;
@@ -116,65 +117,115 @@ __device__ inline int classify(float p, const RowMeta& mi, const RowMeta& mj, co
     return 2;
 }
 
+// Wait until at most n vector-memory operations of this wave are in flight
+// (n is a small compile-time bound of the prefetch pipeline).
+__device__ inline void wait_vm(int n) {
+    switch (n) {
+        case 0: __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory"); break;
+        case 1: __asm__ volatile("s_waitcnt vmcnt(1)" ::: "memory"); break;
+        case 2: __asm__ volatile("s_waitcnt vmcnt(2)" ::: "memory"); break;
+        case 3: __asm__ volatile("s_waitcnt vmcnt(3)" ::: "memory"); break;
+        case 4: __asm__ volatile("s_waitcnt vmcnt(4)" ::: "memory"); break;
+        case 5: __asm__ volatile("s_waitcnt vmcnt(5)" ::: "memory"); break;
+        case 6: __asm__ volatile("s_waitcnt vmcnt(6)" ::: "memory"); break;
+        case 7: __asm__ volatile("s_waitcnt vmcnt(7)" ::: "memory"); break;
+        case 8: __asm__ volatile("s_waitcnt vmcnt(8)" ::: "memory"); break;
+        case 9: __asm__ volatile("s_waitcnt vmcnt(9)" ::: "memory"); break;
+        case 10: __asm__ volatile("s_waitcnt vmcnt(10)" ::: "memory"); break;
+        case 11: __asm__ volatile("s_waitcnt vmcnt(11)" ::: "memory"); break;
+        case 12: __asm__ volatile("s_waitcnt vmcnt(12)" ::: "memory"); break;
+        case 13: __asm__ volatile("s_waitcnt vmcnt(13)" ::: "memory"); break;
+        case 14: __asm__ volatile("s_waitcnt vmcnt(14)" ::: "memory"); break;
+        default: __asm__ volatile("s_waitcnt vmcnt(15)" ::: "memory"); break;
+    }
+}
+// Workgroup barrier without the implicit vmcnt(0) of __syncthreads (which
+// would drain the prefetch pipeline); LDS stores are drained explicitly.
+__device__ inline void lds_barrier() {
+    __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    __asm__ volatile("" ::: "memory");
+}
+
+// Direct global -> LDS loads (global_load_lds_dword[x4]: lane l of the wave
+// writes LDS[m0 + size * l]).  Issued as inline asm so that the compiler does
+// not treat every later LDS read as aliasing the DMA target and drain the
+// whole prefetch pipeline; completion is tracked by the explicit vmcnt waits
+// of the pipeline (wait_vm).  No VGPR is written by these instructions.
+typedef __attribute__((address_space(3))) uint8_t lds_u8;
+__device__ inline uint32_t lds_addr(const void* p) {
+    return (uint32_t)(uintptr_t)(const lds_u8*)p;
+}
+__device__ inline void dma16(const void* src, uint32_t lds_wave_base) {
+    __asm__ volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, off" ::"s"(lds_wave_base), "v"(src) : "memory", "m0");
+}
+__device__ inline void dma4(const void* src, uint32_t lds_wave_base) {
+    __asm__ volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dword %1, off" ::"s"(lds_wave_base), "v"(src) : "memory", "m0");
+}
+
+template <int PREC, int DP>
+struct Geo {
+    using C = Cfg<PREC>;
+    static constexpr int ROWB = DP * C::ES;                 // bytes of one quantized row
+    static constexpr int R16 = ROWB / 16;                   // 16-byte units per row
+    static constexpr int UNITS = C::NT * 32 * R16;          // units per 32-column block
+    static constexpr int LPB = UNITS / 256;                 // full direct loads per thread per block
+    static constexpr int BLKB = UNITS * 16;                 // bytes per stage
+    static constexpr int NSTAGE = (32768 / BLKB) < 2 ? 2 : ((32768 / BLKB) > 6 ? 6 : (32768 / BLKB));
+    static constexpr int FSH = R16 >= 16 ? 0 : (R16 == 8 ? 1 : (R16 == 4 ? 2 : 3));
+    static constexpr int FMASK = (R16 >= 16 ? 16 : R16) - 1;
+    static constexpr int KS = DP / C::KSTEP;
+    // LDS unit holding 16-byte chunk q of column col (XOR swizzle: the 16
+    // columns read by one 16-lane phase land in 16 distinct bank groups)
+    __device__ static inline int unit(int col, int q) { return col * R16 + (q ^ ((col >> FSH) & FMASK)); }
+};
+
+constexpr int CCAP = 256;  // per-wave staged candidates
+
 template <int PREC, int DP, int RB, bool SELF>
 __global__ void __launch_bounds__(256, 2) screen_kernel(const tj::ScreenArgs A) {
     using C = Cfg<PREC>;
-    constexpr int ROWB = DP * C::ES;             // bytes of one quantized row
-    constexpr int LROW = C::NT * ROWB + 16;      // LDS row stride (padded)
-    constexpr int KS = DP / C::KSTEP;            // MFMA steps per row
-    constexpr int R16 = ROWB / 16;               // 16-byte units per row
-    constexpr int UNITS = C::NT * 32 * R16;      // 16-byte units per S block
-    constexpr int UPT = (UNITS + 255) / 256;     // units per thread
+    using Gm = Geo<PREC, DP>;
+    constexpr int NSTAGE = Gm::NSTAGE;
     constexpr int BM = 4 * RB * 32;
 
-    __shared__ __attribute__((aligned(16))) uint8_t sB[2][32 * LROW];
-    __shared__ uint64_t sAcc[4][ACAP];
-    __shared__ uint64_t sAmbK[4][MCAP];
-    __shared__ float2 sAmbA[4][MCAP];
+    __shared__ __attribute__((aligned(16))) uint8_t sB[NSTAGE][Gm::BLKB];
+    __shared__ __attribute__((aligned(16))) float sNp[NSTAGE][64];
+    __shared__ __attribute__((aligned(16))) uint32_t sSc[NSTAGE][64];
     __shared__ float sP[4][16 * 64];
     __shared__ float sThr[4][RB * 32];
+    __shared__ uint64_t sCK[4][CCAP];
+    __shared__ float sCP[4][CCAP];
     __shared__ int sItem;
 
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int hh = lane >> 5, c32 = lane & 31;
-    const JoinParams P = *A.params;
     const int64_t na = A.na, nb = A.nb;
     const int total_items = A.panels * A.col_splits;
     const uint8_t* qa[2] = {(const uint8_t*)A.qa_hi, (const uint8_t*)A.qa_lo};
     const uint8_t* qb[2] = {(const uint8_t*)A.qb_hi, (const uint8_t*)A.qb_lo};
 
-    int nAcc = 0, nAmb = 0;
-    uint64_t cntCert = 0, cntCand = 0;
+    int nC = 0;
+    uint64_t cntCand = 0;
 
-    auto flush_acc = [&]() {
+    auto flush = [&]() {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         uint64_t base = 0;
-        if (lane == 0) base = atomicAdd((unsigned long long*)&A.stats[TJ_ST_OUT], (unsigned long long)nAcc);
+        if (lane == 0) base = atomicAdd((unsigned long long*)&A.stats[TJ_ST_CANDIDATES], (unsigned long long)nC);
         base = uni64(base);
-        for (int t = lane; t < nAcc; t += 64) {
-            uint64_t idx = base + (uint64_t)t;
-            if (idx < (uint64_t)A.out_cap) A.out_keys[idx] = sAcc[wave][t];
-        }
-        __builtin_amdgcn_wave_barrier();
-        nAcc = 0;
-    };
-    auto flush_amb = [&]() {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        uint64_t base = 0;
-        if (lane == 0) base = atomicAdd((unsigned long long*)&A.stats[TJ_ST_AMB], (unsigned long long)nAmb);
-        base = uni64(base);
-        for (int t = lane; t < nAmb; t += 64) {
-            uint64_t idx = base + (uint64_t)t;
+        for (int t = lane; t < nC; t += 64) {
+            const uint64_t idx = base + (uint64_t)t;
             if (idx < (uint64_t)A.amb_cap) {
-                A.amb_keys[idx] = sAmbK[wave][t];
-                ((float2*)A.amb_aux)[idx] = sAmbA[wave][t];
+                A.amb_keys[idx] = sCK[wave][t];
+                A.amb_aux[idx] = sCP[wave][t];
             }
         }
+        // keep the in-order vmcnt accounting of the prefetch pipeline exact
+        __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
         __builtin_amdgcn_wave_barrier();
-        nAmb = 0;
+        nC = 0;
     };
 
     for (;;) {
@@ -193,20 +244,20 @@ __global__ void __launch_bounds__(256, 2) screen_kernel(const tj::ScreenArgs A) 
         if (cb >= ce) continue;
         const int nblk = (int)((ce - cb + 31) / 32);
 
-        // ---- stationary fragments, thresholds and scales
-        Frag<C::CH> fa[C::NT][RB][KS];
+        // ---- stationary fragments, thresholds and scales (VGPR resident)
+        Frag<C::CH> fa[C::NT][RB][Gm::KS];
         float thr[RB][16];
         int sa[C::NT][RB];
 #pragma unroll
         for (int rb = 0; rb < RB; ++rb) {
-            int64_t row = rw0 + rb * 32 + c32;
-            int64_t rowc = row < na ? row : na - 1;
+            const int64_t row = rw0 + rb * 32 + c32;
+            const int64_t rowc = row < na ? row : na - 1;
 #pragma unroll
             for (int a = 0; a < C::NT; ++a)
 #pragma unroll
-                for (int t = 0; t < KS; ++t) ld_frag(fa[a][rb][t], qa[a] + rowc * ROWB + (2 * t + hh) * C::CH);
+                for (int t = 0; t < Gm::KS; ++t) ld_frag(fa[a][rb][t], qa[a] + rowc * Gm::ROWB + (2 * t + hh) * C::CH);
             if constexpr (C::FP8) {
-                uint32_t pk = __builtin_bit_cast(uint32_t, A.ma[rowc].m1.w);
+                const uint32_t pk = __builtin_bit_cast(uint32_t, A.ma[rowc].m1.w);
                 sa[0][rb] = (int)(pk & 0xFFFFu) - 32768 + 127;
                 if (C::NT == 2) sa[C::NT - 1][rb] = (int)(pk >> 16) - 32768 + 127;
             } else {
@@ -215,154 +266,132 @@ __global__ void __launch_bounds__(256, 2) screen_kernel(const tj::ScreenArgs A) 
             }
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
-                int64_t ri = rw0 + rb * 32 + 8 * (r >> 2) + 4 * hh + (r & 3);
+                const int64_t ri = rw0 + rb * 32 + 8 * (r >> 2) + 4 * hh + (r & 3);
                 thr[rb][r] = ri < na ? A.ma[ri].m1.y : -INFINITY;
             }
-            {
-                int64_t ri = rw0 + rb * 32 + c32;
-                if (hh == 0) sThr[wave][rb * 32 + c32] = ri < na ? A.ma[ri].m1.y : -INFINITY;
-            }
+            if (hh == 0) sThr[wave][rb * 32 + c32] = row < na ? A.ma[row].m1.y : -INFINITY;
         }
 
-        // ---- S block staging (global -> VGPR -> LDS), double buffered
-        uint4 pre[UPT];
-        auto load_block = [&](int blk) {
+        // ---- S blocks: direct global->LDS loads into an NSTAGE ring
+        auto issue = [&](int blk) {
+            const int slot = blk % NSTAGE;
             const int64_t c0 = cb + (int64_t)blk * 32;
+            const uint32_t dst = __builtin_amdgcn_readfirstlane(lds_addr(sB[slot]));
 #pragma unroll
-            for (int u = 0; u < UPT; ++u) {
-                int unit = tid + u * 256;
-                if (UNITS % 256 == 0 || unit < UNITS) {
-                    int a = unit / (32 * R16);
-                    int rem = unit % (32 * R16);
-                    int col = rem / R16, q = rem % R16;
+            for (int u = 0; u * 256 < Gm::UNITS; ++u) {
+                const int pbase = wave * 64 + u * 256;  // this wave's 64 consecutive units
+                if (pbase < Gm::UNITS) {
+                    const int p = pbase + lane;
+                    const int a = p / (32 * Gm::R16);
+                    const int rem = p % (32 * Gm::R16);
+                    const int col = rem / Gm::R16;
+                    const int qs = rem % Gm::R16;
+                    const int q = qs ^ ((col >> Gm::FSH) & Gm::FMASK);
                     int64_t j = c0 + col;
                     if (j >= nb) j = nb - 1;
-                    pre[u] = *(const uint4*)(qb[a] + j * ROWB + q * 16);
+                    dma16(qb[a] + j * Gm::ROWB + q * 16, __builtin_amdgcn_readfirstlane(dst + (uint32_t)pbase * 16u));
                 }
             }
-        };
-        auto store_block = [&](int buf) {
-#pragma unroll
-            for (int u = 0; u < UPT; ++u) {
-                int unit = tid + u * 256;
-                if (UNITS % 256 == 0 || unit < UNITS) {
-                    int a = unit / (32 * R16);
-                    int rem = unit % (32 * R16);
-                    int col = rem / R16, q = rem % R16;
-                    *(uint4*)(&sB[buf][col * LROW + a * ROWB + q * 16]) = pre[u];
-                }
+            if (wave == 0 && lane < 32) {
+                int64_t j = c0 + lane;
+                if (j >= nb) j = nb - 1;
+                dma4(&A.mb[j].m1.z, __builtin_amdgcn_readfirstlane(lds_addr(&sNp[slot][0])));
+                if constexpr (C::FP8) dma4(&A.mb[j].m1.w, __builtin_amdgcn_readfirstlane(lds_addr(&sSc[slot][0])));
             }
         };
-        load_block(0);
-        store_block(0);
-        __syncthreads();
+        const int pre = nblk < NSTAGE - 1 ? nblk : NSTAGE - 1;
+        for (int b = 0; b < pre; ++b) issue(b);
 
         for (int blk = 0; blk < nblk; ++blk) {
-            const int buf = blk & 1;
+            const int younger = (nblk - 1 - blk) < (NSTAGE - 2) ? (nblk - 1 - blk) : (NSTAGE - 2);
+            wait_vm(younger * Gm::LPB);
+            lds_barrier();
+            if (blk + NSTAGE - 1 < nblk) issue(blk + NSTAGE - 1);
+            const int slot = blk % NSTAGE;
             const int64_t c0 = cb + (int64_t)blk * 32;
-            if (blk + 1 < nblk) load_block(blk + 1);
             const int64_t jcol = c0 + c32;
+            const bool skip = SELF && (c0 + 31 <= rw0);
+            if (skip) continue;
             float nj = INFINITY;
             int sbh = 127, sbl = 127;
             if (jcol < nb) {
-                float4 m1 = A.mb[jcol].m1;
-                nj = m1.z;
+                nj = sNp[slot][c32];
                 if constexpr (C::FP8) {
-                    uint32_t pk = __builtin_bit_cast(uint32_t, m1.w);
+                    const uint32_t pk = sSc[slot][c32];
                     sbh = (int)(pk & 0xFFFFu) - 32768 + 127;
                     sbl = (int)(pk >> 16) - 32768 + 127;
                 }
             }
-            const bool skip = SELF && (c0 + 31 <= rw0);
-            if (!skip) {
-                f32x16 acc[RB];
+            f32x16 acc[RB];
 #pragma unroll
-                for (int rb = 0; rb < RB; ++rb) acc[rb] = (f32x16){};
-                const uint8_t* bb = &sB[buf][c32 * LROW + hh * C::CH];
+            for (int rb = 0; rb < RB; ++rb) acc[rb] = (f32x16){};
+            const uint8_t* bbase = sB[slot];
 #pragma unroll
-                for (int t = 0; t < KS; ++t) {
-                    Frag<C::CH> fb;
-                    ld_frag(fb, bb + 2 * t * C::CH);
-                    if constexpr (C::NT == 1) {
-#pragma unroll
-                        for (int rb = 0; rb < RB; ++rb) mma<PREC>(acc[rb], fa[0][rb][t], fb, sa[0][rb], sbh);
-                    } else {
-                        Frag<C::CH> fl;
-                        ld_frag(fl, bb + ROWB + 2 * t * C::CH);
-#pragma unroll
-                        for (int rb = 0; rb < RB; ++rb) {
-                            mma<PREC>(acc[rb], fa[0][rb][t], fb, sa[0][rb], sbh);
-                            mma<PREC>(acc[rb], fa[0][rb][t], fl, sa[0][rb], sbl);
-                            mma<PREC>(acc[rb], fa[1][rb][t], fb, sa[1][rb], sbh);
-                        }
-                    }
+            for (int t = 0; t < Gm::KS; ++t) {
+                Frag<C::CH> fb;
+                if constexpr (C::CH == 16) {
+                    fb.v = *(const uint4*)(bbase + Gm::unit(c32, 2 * t + hh) * 16);
+                } else {
+                    fb.v0 = *(const uint4*)(bbase + Gm::unit(c32, 2 * (2 * t + hh)) * 16);
+                    fb.v1 = *(const uint4*)(bbase + Gm::unit(c32, 2 * (2 * t + hh) + 1) * 16);
                 }
-                // ---- fast reject: w = fma(-2, p, nprime_j) > thr_i rejects rigorously
+                if constexpr (C::NT == 1) {
 #pragma unroll
-                for (int rb = 0; rb < RB; ++rb) {
-                    uint64_t any = 0;
-#pragma unroll
-                    for (int r = 0; r < 16; ++r) any |= __ballot(fmaf(-2.0f, acc[rb][r], nj) <= thr[rb][r]);
-                    if (any == 0) continue;
-                    // ---- per-pair certificate on the candidates: the block goes
-                    // to LDS and is walked by one rolled loop.
-#pragma unroll
-                    for (int r = 0; r < 16; ++r) sP[wave][r * 64 + lane] = acc[rb][r];
-                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                    __builtin_amdgcn_wave_barrier();
-#pragma unroll 1
-                    for (int r = 0; r < 16; ++r) {
-                        const float p = sP[wave][r * 64 + lane];
-                        const int rloc = rb * 32 + 8 * (r >> 2) + 4 * hh + (r & 3);
-                        const bool cand = fmaf(-2.0f, p, nj) <= sThr[wave][rloc];
-                        const uint64_t m = __ballot(cand);
-                        if (m == 0) continue;
-                        cntCand += (uint64_t)__popcll(m);
-                        int cls = 0;
-                        float est = 0.f, hw = 0.f;
-                        uint64_t key = 0;
-                        const int64_t i = rw0 + rloc;
-                        if (cand && i < na && jcol < nb && (!SELF || i < jcol)) {
-                            const RowMeta mi = A.ma[i];
-                            const RowMeta mj = A.mb[jcol];
-                            cls = classify<C::TERMS>(p, mi, mj, P, &est, &hw);
-                            key = key_pack((uint64_t)i, (uint64_t)jcol, A.jbits);
-                        }
-                        if (A.audit && cls == 1) { cls = 3; key |= 1ull << 63; }
-                        const uint64_t mA = __ballot(cls == 1);
-                        const uint64_t mM = __ballot(cls >= 2);
-                        if (mA) {
-                            const int k = uni(__popcll(mA));
-                            if (nAcc + k > ACAP) flush_acc();
-                            if (cls == 1) sAcc[wave][nAcc + lanes_below(mA)] = key;
-                            nAcc += k;
-                            cntCert += (uint64_t)k;
-                        }
-                        if (mM) {
-                            const int k = uni(__popcll(mM));
-                            if (nAmb + k > MCAP) flush_amb();
-                            if (cls >= 2) {
-                                const int pos = nAmb + lanes_below(mM);
-                                sAmbK[wave][pos] = key;
-                                sAmbA[wave][pos] = make_float2(est, hw);
-                            }
-                            nAmb += k;
-                            if (A.audit) cntCert += (uint64_t)__popcll(__ballot(cls == 3));
-                        }
+                    for (int rb = 0; rb < RB; ++rb) mma<PREC>(acc[rb], fa[0][rb][t], fb, sa[0][rb], sbh);
+                } else {
+                    Frag<C::CH> fl;
+                    const uint8_t* lbase = bbase + 32 * Gm::R16 * 16;
+                    if constexpr (C::CH == 16) {
+                        fl.v = *(const uint4*)(lbase + Gm::unit(c32, 2 * t + hh) * 16);
+                    } else {
+                        fl.v0 = *(const uint4*)(lbase + Gm::unit(c32, 2 * (2 * t + hh)) * 16);
+                        fl.v1 = *(const uint4*)(lbase + Gm::unit(c32, 2 * (2 * t + hh) + 1) * 16);
                     }
-                    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+                    for (int rb = 0; rb < RB; ++rb) {
+                        mma<PREC>(acc[rb], fa[0][rb][t], fb, sa[0][rb], sbh);
+                        mma<PREC>(acc[rb], fa[0][rb][t], fl, sa[0][rb], sbl);
+                        mma<PREC>(acc[rb], fa[1][rb][t], fb, sa[1][rb], sbh);
+                    }
                 }
             }
-            if (blk + 1 < nblk) store_block(buf ^ 1);
-            __syncthreads();
+            // ---- fast reject: w = fma(-2, p, nprime_j) > thr_i rejects rigorously
+#pragma unroll
+            for (int rb = 0; rb < RB; ++rb) {
+                uint64_t any = 0;
+#pragma unroll
+                for (int r = 0; r < 16; ++r) any |= __ballot(fmaf(-2.0f, acc[rb][r], nj) <= thr[rb][r]);
+                if (any == 0) continue;
+                // ---- rare: stage the candidates (pair key, p) for the certify pass
+#pragma unroll
+                for (int r = 0; r < 16; ++r) sP[wave][r * 64 + lane] = acc[rb][r];
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+#pragma unroll 1
+                for (int r = 0; r < 16; ++r) {
+                    const float p = sP[wave][r * 64 + lane];
+                    const int rloc = rb * 32 + 8 * (r >> 2) + 4 * hh + (r & 3);
+                    const int64_t i = rw0 + rloc;
+                    const bool cand = fmaf(-2.0f, p, nj) <= sThr[wave][rloc] && i < na && jcol < nb &&
+                                      (!SELF || i < jcol);
+                    const uint64_t m = __ballot(cand);
+                    if (m == 0) continue;
+                    const int k = uni(__popcll(m));
+                    if (nC + k > CCAP) flush();
+                    if (cand) {
+                        const int pos = nC + lanes_below(m);
+                        sCK[wave][pos] = key_pack((uint64_t)i, (uint64_t)jcol, A.jbits);
+                        sCP[wave][pos] = p;
+                    }
+                    nC += k;
+                    cntCand += (uint64_t)k;
+                }
+                __builtin_amdgcn_wave_barrier();
+            }
         }
     }
-    if (nAcc) flush_acc();
-    if (nAmb) flush_amb();
-    if (lane == 0) {
-        if (cntCert) atomicAdd((unsigned long long*)&A.stats[TJ_ST_CERT_ACCEPT], (unsigned long long)cntCert);
-        if (cntCand) atomicAdd((unsigned long long*)&A.stats[TJ_ST_CANDIDATES], (unsigned long long)cntCand);
-    }
+    if (nC) flush();
+    (void)cntCand;
 }
 
 // Rows per wave: as many 32-row blocks as keep the stationary fragments

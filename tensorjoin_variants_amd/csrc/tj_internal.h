@@ -160,6 +160,30 @@ __device__ inline int fp8_row_exp(double maxabs) {
 //                             (e_hi in bits 0-15, e_lo in 16-31, biased by 32768).
 struct RowMeta { float4 m0, m1; };
 
+// Sequential binary64 difference form, k ascending, no FMA (every file is
+// compiled with -ffp-contract=off): the predicate of
+// CEJ/src/audit_g2b_accepted_fp64.py:51-57.
+__device__ inline double d2_seq(const float* __restrict__ x, const float* __restrict__ y, int d) {
+    double s = 0.0;
+    int k = 0;
+    if (((((uintptr_t)x) | ((uintptr_t)y)) & 15) == 0) {
+        for (; k + 4 <= d; k += 4) {
+            const float4 a = *(const float4*)(x + k);
+            const float4 b = *(const float4*)(y + k);
+            double t;
+            t = (double)a.x - (double)b.x; s = s + t * t;
+            t = (double)a.y - (double)b.y; s = s + t * t;
+            t = (double)a.z - (double)b.z; s = s + t * t;
+            t = (double)a.w - (double)b.w; s = s + t * t;
+        }
+    }
+    for (; k < d; ++k) {
+        const double t = (double)x[k] - (double)y[k];
+        s = s + t * t;
+    }
+    return s;
+}
+
 __host__ __device__ inline uint64_t key_pack(uint64_t i, uint64_t j, int jbits) { return (i << jbits) | j; }
 inline int bits_for(int64_t n) { int b = 0; while (((int64_t)1 << b) < n) ++b; return b; }
 
@@ -187,6 +211,10 @@ int launch_rowmeta(const float* x, int64_t n, int d, int dp, int prec, const Joi
 int launch_refine(const float* R, const float* S, int d, int jbits, const uint64_t* amb_keys,
                   const float* amb_aux, int64_t amb_cap, const JoinParams* p, uint64_t* out_keys,
                   int64_t out_cap, uint64_t* stats, hipStream_t s);
+int launch_certify(const float* R, const float* S, int d, int jbits, const uint64_t* cand_keys,
+                   const float* cand_p, int64_t cand_cap, const RowMeta* ma, const RowMeta* mb,
+                   const JoinParams* p, uint64_t* out_keys, int64_t out_cap, uint64_t* stats, int audit,
+                   hipStream_t s);
 int launch_plain(const float* R, int64_t nR, const float* S, int64_t nS, int d, int jbits, double T,
                  bool self_join, uint64_t* out_keys, int64_t out_cap, uint64_t* stats, hipStream_t s);
 size_t sort_temp_bytes(int64_t capacity);

@@ -145,7 +145,8 @@ class Plan:
             stats = _lib.decode_stats(self._stats.cpu().numpy().view(np.uint64))
             if stats["invalid"]:
                 raise ValueError(f"{stats['invalid']} non-finite input coordinates")
-            need_out, need_amb = stats["out"], stats["ambiguous"]
+            # the screen's queue holds every candidate (certified inside + ambiguous band)
+            need_out, need_amb = stats["out"], stats["candidates"]
             if need_out <= self.out_capacity and need_amb <= self.amb_capacity:
                 stats["plan"] = self.describe()
                 return JoinResult(pairs[:need_out], stats)

@@ -39,8 +39,7 @@ def run_check(tj, oracle, R, S, T, prec, method="certified", audit=True, name=""
     assert np.array_equal(got, want), (name, prec)
     if method == "certified":
         assert st["violations"] == 0, st
-        if audit:
-            assert st["cert_accept"] <= st["fp64_accept"] + 0
+        assert st["cert_accept"] + st["fp64_accept"] == st["out"], st
     return st
 
 
