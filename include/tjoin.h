@@ -169,12 +169,14 @@ int tj_canonical_u64(void* temp, size_t* temp_bytes, const int64_t* pairs_i64,
  * MFMA probe: nblocks independent 32x32 tiles, each the product of rows
  * A[blk*32 + i] and B[blk*32 + j] (row-major, ksteps MFMA steps of 16
  * (16-bit) or 64 (fp8) elements), accumulated exactly as the screen does;
- * fp8 rows carry power-of-two scale exponents scale_a / scale_b.
- * out[blk][i][j] (binary32).  Used to validate the accumulation model of the
+ * fp8 rows carry power-of-two scale exponents scale_a / scale_b; seed (or
+ * NULL) gives the initial accumulator of row blk*32 + i, as the screen seeds
+ * its chains.  out[blk][i][j] (binary32).  Used to validate the accumulation model of the
  * certificate and the output layout on the device.  No counterpart in the
  * reference (its premise was inherited, F16C/NUMERICAL_SCOPE.md:11-43). */
 int tj_mfma_probe(int precision, const void* A, const void* B, const int* scale_a,
-                  const int* scale_b, int ksteps, int nblocks, float* out, void* stream);
+                  const int* scale_b, const float* seed, int ksteps, int nblocks, float* out,
+                  void* stream);
 
 #ifdef __cplusplus
 }

@@ -66,7 +66,7 @@ def load():
         "tj_plain_fp64": (i32, [vp, i64, vp, i64, i32, i32, f64, vp, i64, vp, vp]),
         "tj_sort_keys": (i32, [vp, C.POINTER(sz), vp, vp, i64, i32, vp]),
         "tj_unpack_pairs": (i32, [vp, vp, i64, i32, vp, i32, vp]),
-        "tj_mfma_probe": (i32, [i32, vp, vp, vp, vp, i32, i32, vp, vp]),
+        "tj_mfma_probe": (i32, [i32, vp, vp, vp, vp, vp, i32, i32, vp, vp]),
         "tj_canonical_u64": (i32, [vp, C.POINTER(sz), vp, vp, i64, i64, vp, vp]),
     }
     for name, (res, args) in sig.items():

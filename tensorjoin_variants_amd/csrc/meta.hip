@@ -71,7 +71,9 @@ __global__ void params_kernel(const uint32_t* maxabs, double T, int d, int dp, i
     // Same form as the reference's FP16 premise (F16C/NUMERICAL_SCOPE.md:11-43,
     // gamma = 1026/(2^23-1026) at k = 512), k counted over every term of the
     // hi/lo split here.
-    double k = (double)prec_terms(prec) * dp;
+    // (k counts the products of every term of the hi/lo split; +1 for the
+    // accumulator seed t_i of the fast path)
+    double k = (double)prec_terms(prec) * dp + 1.0;
     double mu = (2.0 * k + 2.0) * 0x1p-23;
     q.gamma_acc = mu / (1.0 - mu) * (1.0 + 0x1p-40);
     // The FP8 MFMA (scaled f8f6f4 and plain fp8 alike) sums the products of
@@ -172,31 +174,33 @@ __global__ void __launch_bounds__(256) rowmeta_kernel(const float* __restrict__ 
     const float L = TERMS == 3 ? f32_up(sqrt(nl * (1.0 + rel)) * root_pad) : 0.0f;
     const float E = f32_up(sqrt(nr * (1.0 + rel)) * root_pad);
 
-    // Fast-path terms.  With Z = H + L, the pair bound
-    //   B_ij = gamma (H_i H_j [+ H_i L_j + L_i H_j]) + L_i L_j + Z_i E_j + E_i Z_j + E_i E_j + alpha
-    // is split by 2ab <= rho a^2 + b^2/rho into beta_i + beta_j + alpha with
-    //   beta = ((gamma + rho) Z^2 + L^2 + (1/rho + 1) E^2) / 2.
-    // The fused w = fma(-2, p, nprime_j) rounds once: |err| <= 2^-24 (|nprime| + 2|p|),
-    // |p| <= (1+gamma) Z_i Z_j + alpha, charged as omega_i + omega_j.
+    // Fast-path terms.  The screen seeds the MFMA accumulator of row i with
+    // t_i, so one chain yields s_ij = t_i + p_ij (p the dot of the quantized
+    // rows) with |s - (t_i + sum a b)| <= gamma (|t_i| + sum |a b|) + alpha.
+    // With Z = H + L the residual part of the exact dot is bounded by
+    //   L_i L_j + Z_i E_j + E_i Z_j + E_i E_j,  sum |a b| <= H_i H_j [+ H_i L_j + L_i H_j],
+    // and 2ab <= rho a^2 + b^2/rho splits everything except gamma |t_i| into
+    //   beta_i + beta_j,  beta = ((gamma + rho) Z^2 + L^2 + (1/rho + 1) E^2) / 2.
+    // The exact squared distance then satisfies
+    //   d2 >= nlo_i + nlo_j - 2 s + 2 t_i - 2 gamma |t_i| - 2 beta_i - 2 beta_j - 2 alpha.
+    // Choosing 2 t_i - 2 gamma |t_i| >= D_i = Tj_rej + 2 beta_i + 2 alpha - nlo_i
+    // leaves  d2 > Tj_rej  whenever  s < v_j = (nlo_j - 2 beta_j) / 2:
+    // the whole fast reject is one binary32 compare of the MFMA output.
     const double Z = (double)H + (double)L;
     const double rho = P.rho;
     const double pad = 1.0 + 0x1p-44;
     const double beta = ((P.gamma_acc + rho) * Z * Z + (double)L * L + (1.0 / rho + 1.0) * (double)E * E) * 0.5 * pad;
-    const double omega_row = 0x1p-23 * Z * Z * pad;
-    const double omega_col = 0x1p-23 * ((double)nhi + 2.0 * beta + Z * Z) * pad;
-    // stationary side: reject iff w > thr, thr >= Tj_rej - (nlo - 2 beta - omega_row - 4 alpha)
-    double c = (double)nlo - 2.0 * beta - omega_row - 4.0 * P.alpha;
-    double t = P.Tj_rej - c;
-    t = t + fabs(t) * 0x1p-50 + 0x1p-1000;
-    const float thr = f32_up(t);
-    // streamed side: nprime <= nlo - 2 beta - omega_col
-    double np = (double)nlo - 2.0 * beta - omega_col;
-    np = np - fabs(np) * 0x1p-50 - 0x1p-1000;
-    const float nprime = f32_down(np);
+    double D = P.Tj_rej + 2.0 * beta + 2.0 * P.alpha - (double)nlo;
+    D += fabs(D) * 0x1p-50 + 0x1p-1000;  // binary64 rounding of D, outward
+    const double tseed = D >= 0.0 ? D / (2.0 * (1.0 - P.gamma_acc)) : D / (2.0 * (1.0 + P.gamma_acc));
+    const float t32 = f32_up(tseed + fabs(tseed) * 0x1p-50 + 0x1p-1000);  // rounded upward
+    double v = 0.5 * ((double)nlo - 2.0 * beta);
+    v -= fabs(v) * 0x1p-50 + 0x1p-1000;
+    const float v32 = f32_down(v);
     RowMeta m;
     m.m0 = make_float4(nlo, nhi, H, E);
     uint32_t packed = (uint32_t)(e_hi + 32768) | ((uint32_t)(e_lo + 32768) << 16);
-    m.m1 = make_float4(L, thr, nprime, __builtin_bit_cast(float, packed));
+    m.m1 = make_float4(L, t32, v32, __builtin_bit_cast(float, packed));
     meta[row] = m;
 }
 

@@ -16,7 +16,8 @@ namespace {
 template <int PREC>
 __global__ void __launch_bounds__(64) mfma_probe_kernel(const uint8_t* __restrict__ A, const uint8_t* __restrict__ B,
                                                         const int* __restrict__ sa, const int* __restrict__ sb,
-                                                        int ksteps, float* __restrict__ out) {
+                                                        const float* __restrict__ seed, int ksteps,
+                                                        float* __restrict__ out) {
     using C = tjscreen::Cfg<PREC>;
     const int lane = threadIdx.x, hh = lane >> 5, c32 = lane & 31;
     const int64_t rowb = (int64_t)ksteps * C::KSTEP * C::ES;
@@ -25,6 +26,10 @@ __global__ void __launch_bounds__(64) mfma_probe_kernel(const uint8_t* __restric
     const int s_a = C::FP8 ? sa[blockIdx.x * 32 + c32] + 127 : 127;
     const int s_b = C::FP8 ? sb[blockIdx.x * 32 + c32] + 127 : 127;
     tjscreen::f32x16 acc = {};
+    if (seed) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[r] = seed[blockIdx.x * 32 + 8 * (r >> 2) + 4 * hh + (r & 3)];
+    }
     for (int t = 0; t < ksteps; ++t) {
         tjscreen::Frag<C::CH> fa, fb;
         tjscreen::ld_frag(fa, a + 2 * t * C::CH);
@@ -58,17 +63,17 @@ __global__ void __launch_bounds__(64) mfma_probe_fp8_plain(const uint8_t* __rest
 }  // namespace
 
 extern "C" int tj_mfma_probe(int precision, const void* A, const void* B, const int* scale_a, const int* scale_b,
-                             int ksteps, int nblocks, float* out, void* stream) {
+                             const float* seed, int ksteps, int nblocks, float* out, void* stream) {
     TJ_CHECK(A && B && out && ksteps > 0 && nblocks > 0, TJ_ERR_INVALID, "bad probe arguments");
     hipStream_t s = (hipStream_t)stream;
     const uint8_t* a = (const uint8_t*)A;
     const uint8_t* b = (const uint8_t*)B;
     switch (precision) {
-        case P_FP16: hipLaunchKernelGGL(mfma_probe_kernel<P_FP16>, dim3(nblocks), dim3(64), 0, s, a, b, scale_a, scale_b, ksteps, out); break;
-        case P_BF16: hipLaunchKernelGGL(mfma_probe_kernel<P_BF16>, dim3(nblocks), dim3(64), 0, s, a, b, scale_a, scale_b, ksteps, out); break;
+        case P_FP16: hipLaunchKernelGGL(mfma_probe_kernel<P_FP16>, dim3(nblocks), dim3(64), 0, s, a, b, scale_a, scale_b, seed, ksteps, out); break;
+        case P_BF16: hipLaunchKernelGGL(mfma_probe_kernel<P_BF16>, dim3(nblocks), dim3(64), 0, s, a, b, scale_a, scale_b, seed, ksteps, out); break;
         case P_FP8:
             TJ_CHECK(scale_a && scale_b, TJ_ERR_INVALID, "fp8 probe needs scales");
-            hipLaunchKernelGGL(mfma_probe_kernel<P_FP8>, dim3(nblocks), dim3(64), 0, s, a, b, scale_a, scale_b, ksteps, out);
+            hipLaunchKernelGGL(mfma_probe_kernel<P_FP8>, dim3(nblocks), dim3(64), 0, s, a, b, scale_a, scale_b, seed, ksteps, out);
             break;
         case 100:  // non-scaled fp8 instruction, ksteps of 16
             hipLaunchKernelGGL(mfma_probe_fp8_plain, dim3(nblocks), dim3(64), 0, s, a, b, ksteps, out);

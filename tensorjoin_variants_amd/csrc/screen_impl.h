@@ -74,6 +74,13 @@ __device__ inline void mma(f32x16& acc, const Frag<Cfg<PREC>::CH>& a, const Frag
     }
 }
 
+template <int PREC>
+__device__ inline f32x16 mma_c(const Frag<Cfg<PREC>::CH>& a, const Frag<Cfg<PREC>::CH>& b, const f32x16& c, int sa, int sb) {
+    f32x16 d = c;
+    mma<PREC>(d, a, b, sa, sb);
+    return d;
+}
+
 __device__ inline int lanes_below(uint64_t m) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
 }
@@ -86,28 +93,31 @@ __device__ inline int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 // Exact per-pair interval in binary64 (join units) and its class:
 // 1 = certified inside, 0 = certified outside, 2 = ambiguous.
-//   B_ij = gamma (H_i H_j [+ H_i L_j + L_i H_j]) + L_i L_j + Z_i E_j + E_i Z_j + E_i E_j + alpha
+// s is the screen's MFMA output t_i + p (t_i the accumulator seed of row i):
+//   p = s - t_i,  |p - x_i.x_j| <= B_ij,
+//   B_ij = gamma (|t_i| + H_i H_j [+ H_i L_j + L_i H_j]) + L_i L_j + Z_i E_j + E_i Z_j + E_i E_j + alpha
 //   d2 in [nlo_i + nlo_j - 2p - 2B, nhi_i + nhi_j - 2p + 2B]
 // widened by the binary64 rounding of this evaluation; the oracle's own
 // gap g is charged in Tj_acc / Tj_rej.
-__device__ inline int classify(float p, const RowMeta& mi, const RowMeta& mj, const JoinParams& P,
+__device__ inline int classify(float s, const RowMeta& mi, const RowMeta& mj, const JoinParams& P,
                                float* est, float* hw) {
-    const double pd = (double)p;
+    const double ti = (double)mi.m1.y;
+    const double pd = (double)s - ti;
     const double Hi = mi.m0.z, Ei = mi.m0.w, Hj = mj.m0.z, Ej = mj.m0.w;
     double B;
     if (P.terms == 1) {
-        B = P.gamma_acc * Hi * Hj + Hi * Ej + Ei * Hj + Ei * Ej + P.alpha;
+        B = P.gamma_acc * (fabs(ti) + Hi * Hj) + Hi * Ej + Ei * Hj + Ei * Ej + P.alpha;
     } else {
         const double Li = mi.m1.x, Lj = mj.m1.x;
         const double Zi = Hi + Li, Zj = Hj + Lj;
-        B = P.gamma_acc * (Hi * Hj + Hi * Lj + Li * Hj) + Li * Lj + Zi * Ej + Ei * Zj + Ei * Ej + P.alpha;
+        B = P.gamma_acc * (fabs(ti) + Hi * Hj + Hi * Lj + Li * Hj) + Li * Lj + Zi * Ej + Ei * Zj + Ei * Ej + P.alpha;
     }
     B *= 1.0 + 0x1p-44;  // a dozen binary64 roundings, each <= 2^-53 relative
     const double slo = (double)mi.m0.x + (double)mj.m0.x;  // exact
     const double shi = (double)mi.m0.y + (double)mj.m0.y;  // exact
     double lo = slo - 2.0 * pd - 2.0 * B;
     double hi = shi - 2.0 * pd + 2.0 * B;
-    const double sl = (shi + 2.0 * fabs(pd) + 2.0 * B) * 0x1p-48;
+    const double sl = (shi + 2.0 * fabs(pd) + 2.0 * fabs(ti) + 2.0 * B) * 0x1p-48;
     lo -= sl;
     hi += sl;
     *est = (float)(0.5 * (slo + shi) - 2.0 * pd);
@@ -193,7 +203,6 @@ __global__ void __launch_bounds__(256, 2) screen_kernel(const tj::ScreenArgs A) 
     __shared__ __attribute__((aligned(16))) float sNp[NSTAGE][64];
     __shared__ __attribute__((aligned(16))) uint32_t sSc[NSTAGE][64];
     __shared__ float sP[4][16 * 64];
-    __shared__ float sThr[4][RB * 32];
     __shared__ uint64_t sCK[4][CCAP];
     __shared__ float sCP[4][CCAP];
     __shared__ int sItem;
@@ -246,7 +255,7 @@ __global__ void __launch_bounds__(256, 2) screen_kernel(const tj::ScreenArgs A) 
 
         // ---- stationary fragments, thresholds and scales (VGPR resident)
         Frag<C::CH> fa[C::NT][RB][Gm::KS];
-        float thr[RB][16];
+        f32x16 seed[RB];
         int sa[C::NT][RB];
 #pragma unroll
         for (int rb = 0; rb < RB; ++rb) {
@@ -267,9 +276,30 @@ __global__ void __launch_bounds__(256, 2) screen_kernel(const tj::ScreenArgs A) 
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
                 const int64_t ri = rw0 + rb * 32 + 8 * (r >> 2) + 4 * hh + (r & 3);
-                thr[rb][r] = ri < na ? A.ma[ri].m1.y : -INFINITY;
+                seed[rb][r] = ri < na ? A.ma[ri].m1.y : -INFINITY;
             }
-            if (hh == 0) sThr[wave][rb * 32 + c32] = row < na ? A.ma[row].m1.y : -INFINITY;
+        }
+
+        // Consume the stationary registers here, so that the compiler's wait
+        // counts (blind to the asm DMA below) settle before the block loop
+        // instead of inserting vmcnt(0) inside it, which would drain the ring.
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb) {
+#pragma unroll
+            for (int a = 0; a < C::NT; ++a)
+#pragma unroll
+                for (int t = 0; t < Gm::KS; ++t) {
+                    if constexpr (C::CH == 16) {
+                        const uint4 q = fa[a][rb][t].v;
+                        __asm__ volatile("" ::"v"(q.x), "v"(q.y), "v"(q.z), "v"(q.w));
+                    } else {
+                        const uint4 q0 = fa[a][rb][t].v0, q1 = fa[a][rb][t].v1;
+                        __asm__ volatile("" ::"v"(q0.x), "v"(q0.y), "v"(q0.z), "v"(q0.w));
+                        __asm__ volatile("" ::"v"(q1.x), "v"(q1.y), "v"(q1.z), "v"(q1.w));
+                    }
+                }
+#pragma unroll
+            for (int r = 0; r < 16; ++r) __asm__ volatile("" ::"v"(seed[rb][r]));
         }
 
         // ---- S blocks: direct global->LDS loads into an NSTAGE ring
@@ -323,8 +353,6 @@ __global__ void __launch_bounds__(256, 2) screen_kernel(const tj::ScreenArgs A) 
                 }
             }
             f32x16 acc[RB];
-#pragma unroll
-            for (int rb = 0; rb < RB; ++rb) acc[rb] = (f32x16){};
             const uint8_t* bbase = sB[slot];
 #pragma unroll
             for (int t = 0; t < Gm::KS; ++t) {
@@ -337,7 +365,10 @@ __global__ void __launch_bounds__(256, 2) screen_kernel(const tj::ScreenArgs A) 
                 }
                 if constexpr (C::NT == 1) {
 #pragma unroll
-                    for (int rb = 0; rb < RB; ++rb) mma<PREC>(acc[rb], fa[0][rb][t], fb, sa[0][rb], sbh);
+                    for (int rb = 0; rb < RB; ++rb) {
+                        if (t == 0) acc[rb] = mma_c<PREC>(fa[0][rb][t], fb, seed[rb], sa[0][rb], sbh);
+                        else mma<PREC>(acc[rb], fa[0][rb][t], fb, sa[0][rb], sbh);
+                    }
                 } else {
                     Frag<C::CH> fl;
                     const uint8_t* lbase = bbase + 32 * Gm::R16 * 16;
@@ -349,31 +380,39 @@ __global__ void __launch_bounds__(256, 2) screen_kernel(const tj::ScreenArgs A) 
                     }
 #pragma unroll
                     for (int rb = 0; rb < RB; ++rb) {
-                        mma<PREC>(acc[rb], fa[0][rb][t], fb, sa[0][rb], sbh);
+                        if (t == 0) acc[rb] = mma_c<PREC>(fa[0][rb][t], fb, seed[rb], sa[0][rb], sbh);
+                        else mma<PREC>(acc[rb], fa[0][rb][t], fb, sa[0][rb], sbh);
                         mma<PREC>(acc[rb], fa[0][rb][t], fl, sa[0][rb], sbl);
                         mma<PREC>(acc[rb], fa[1][rb][t], fb, sa[1][rb], sbh);
                     }
                 }
             }
-            // ---- fast reject: w = fma(-2, p, nprime_j) > thr_i rejects rigorously
+            // ---- fast reject: s = t_i + p_ij < v_j rejects rigorously; one
+            // max over the 16 rows of a lane, one compare, one ballot per block
 #pragma unroll
             for (int rb = 0; rb < RB; ++rb) {
-                uint64_t any = 0;
+                float mx = acc[rb][0];
 #pragma unroll
-                for (int r = 0; r < 16; ++r) any |= __ballot(fmaf(-2.0f, acc[rb][r], nj) <= thr[rb][r]);
-                if (any == 0) continue;
-                // ---- rare: stage the candidates (pair key, p) for the certify pass
+                for (int r = 1; r < 16; ++r) mx = fmaxf(mx, acc[rb][r]);
+                if (__ballot(mx >= nj) == 0) continue;
+                // ---- rare: stage the candidates (pair key, s) for the certify
+                // pass.  A wave-uniform 16-bit mask names the registers that
+                // hold one; the block goes to LDS and only those are visited.
+                uint32_t rm = 0;
+#pragma unroll
+                for (int r = 0; r < 16; ++r) rm |= (__ballot(acc[rb][r] >= nj) != 0 ? 1u : 0u) << r;
 #pragma unroll
                 for (int r = 0; r < 16; ++r) sP[wave][r * 64 + lane] = acc[rb][r];
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
-#pragma unroll 1
-                for (int r = 0; r < 16; ++r) {
-                    const float p = sP[wave][r * 64 + lane];
+                rm = uni((int)rm);
+                while (rm) {
+                    const int r = __builtin_ctz(rm);
+                    rm &= rm - 1;
+                    const float sv = sP[wave][r * 64 + lane];
                     const int rloc = rb * 32 + 8 * (r >> 2) + 4 * hh + (r & 3);
                     const int64_t i = rw0 + rloc;
-                    const bool cand = fmaf(-2.0f, p, nj) <= sThr[wave][rloc] && i < na && jcol < nb &&
-                                      (!SELF || i < jcol);
+                    const bool cand = sv >= nj && i < na && jcol < nb && (!SELF || i < jcol);
                     const uint64_t m = __ballot(cand);
                     if (m == 0) continue;
                     const int k = uni(__popcll(m));
@@ -381,7 +420,7 @@ __global__ void __launch_bounds__(256, 2) screen_kernel(const tj::ScreenArgs A) 
                     if (cand) {
                         const int pos = nC + lanes_below(m);
                         sCK[wave][pos] = key_pack((uint64_t)i, (uint64_t)jcol, A.jbits);
-                        sCP[wave][pos] = p;
+                        sCP[wave][pos] = sv;
                     }
                     nC += k;
                     cntCand += (uint64_t)k;

@@ -154,9 +154,10 @@ __device__ inline int fp8_row_exp(double maxabs) {
 // ------------------------------------------------------------- row meta
 // Two float4 per row:
 //   m0 = {nlo, nhi, H, E}     ||x_join||^2 bracket, ||hi||, ||residual|| upper bounds
-//   m1 = {L, thr, nprime, e}  ||lo|| bound (hi/lo split), fast-path reject threshold
-//                             (stationary side), fast-path column term (streamed
-//                             side), FP8 scale exponents packed as int bits
+//   m1 = {L, t, v, e}         ||lo|| bound (hi/lo split), accumulator seed t of the
+//                             row as the stationary side, reject level v of the row
+//                             as the streamed side (s = t_i + p_ij < v_j rejects),
+//                             FP8 scale exponents packed as int bits
 //                             (e_hi in bits 0-15, e_lo in 16-31, biased by 32768).
 struct RowMeta { float4 m0, m1; };
 

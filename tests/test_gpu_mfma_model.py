@@ -53,14 +53,16 @@ def probe():
     from tensorjoin_variants_amd import _lib
     L = _lib.load()
 
-    def run(prec, A_bytes, B_bytes, sa, sb, ksteps, nblk):
+    def run(prec, A_bytes, B_bytes, sa, sb, ksteps, nblk, seed=None):
         dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
         A, B = dev(A_bytes.view(np.uint8)), dev(B_bytes.view(np.uint8))
         SA = dev(sa.astype(np.int32)) if sa is not None else None
         SB = dev(sb.astype(np.int32)) if sb is not None else None
+        SD = dev(seed.astype(np.float32)) if seed is not None else None
         out = torch.empty((nblk, 32, 32), dtype=torch.float32, device="cuda")
         rc = L.tj_mfma_probe(_lib.PREC[prec], A.data_ptr(), B.data_ptr(),
                              None if SA is None else SA.data_ptr(), None if SB is None else SB.data_ptr(),
+                             None if SD is None else SD.data_ptr(),
                              ksteps, nblk, out.data_ptr(), torch.cuda.current_stream().cuda_stream)
         _lib.check(rc, "tj_mfma_probe")
         torch.cuda.synchronize()
@@ -84,13 +86,16 @@ def patterns(rng, nblk, K, gen):
 FP8_WINDOW = 2.0 ** -9  # the premise's extra FP8 term (tj_internal params: gamma_acc += 2^-9)
 
 
-def check(prec, got, A, B, sa, sb, K):
+def check(prec, got, A, B, sa, sb, K, seed=None):
     scale_a = np.exp2(sa)[..., None] if sa is not None else 1.0
     scale_b = np.exp2(sb)[..., None] if sb is not None else 1.0
     Aa, Bb = A * scale_a, B * scale_b
     exact = np.matmul(Aa, np.swapaxes(Bb, 1, 2))
     absum = np.matmul(np.abs(Aa), np.swapaxes(np.abs(Bb), 1, 2))
-    m = 2 * K + 2
+    if seed is not None:  # the accumulator seed is one more term of the chain
+        exact = exact + seed.astype(np.float64)[:, :, None]
+        absum = absum + np.abs(seed.astype(np.float64))[:, :, None]
+    m = 2 * (K + 1) + 2
     mu = m * 2.0 ** -23
     gamma = mu / (1 - mu) + (FP8_WINDOW if prec == "fp8" else 0.0)
     alpha = m * 2.0 ** -126
@@ -224,7 +229,7 @@ def probe_plain(probe, Ac, Bc):
     A = torch.from_numpy(np.ascontiguousarray(Ac)).to("cuda")
     B = torch.from_numpy(np.ascontiguousarray(Bc)).to("cuda")
     out = torch.empty((nblk, 32, 32), dtype=torch.float32, device="cuda")
-    _lib.check(_lib.load().tj_mfma_probe(100, A.data_ptr(), B.data_ptr(), None, None, K // 16, nblk,
+    _lib.check(_lib.load().tj_mfma_probe(100, A.data_ptr(), B.data_ptr(), None, None, None, K // 16, nblk,
                                          out.data_ptr(), torch.cuda.current_stream().cuda_stream), "probe")
     torch.cuda.synchronize()
     return out.cpu().numpy().astype(np.float64)
@@ -244,3 +249,39 @@ def test_fp8_plain_model(probe, K):
         worst, ulps, nbad = check("fp8", got, A, B, None, None, K)
         print(f"fp8-plain K={K} {name}: max err/bound {worst:.3g}, max err/(2^-23 sum|ab|) {ulps:.3g}")
         assert nbad == 0, (name, worst)
+
+
+@pytest.mark.parametrize("prec", ["fp16", "bf16", "fp8"])
+def test_seeded_chain_model(probe, prec):
+    """The screen seeds each chain with t_i = O(|d2|): the model must hold with
+    the seed counted as one more accumulated term (seeds of both signs, small
+    and large against the products)."""
+    rng = np.random.default_rng(400)
+    nblk = 8
+    K = 128
+    for mag in (1e-3, 1.0, 1e3):
+        if prec == "fp8":
+            Ac, Bc = e4m3_normal_codes(rng, (nblk, 32, K)), e4m3_normal_codes(rng, (nblk, 32, K))
+            A, B = e4m3_decode(Ac), e4m3_decode(Bc)
+            sa = rng.integers(-10, 10, (nblk, 32))
+            sb = rng.integers(-10, 10, (nblk, 32))
+            scale = np.exp2(sa + 0.0).mean() * np.exp2(sb + 0.0).mean()
+            ref = np.abs(A).sum(-1).mean() * np.abs(B).mean() * scale
+            seed = (rng.standard_normal((nblk, 32)) * mag * ref).astype(np.float32)
+            got = probe("fp8", Ac, Bc, sa, sb, K // 64, nblk, seed)
+            worst, ulps, nbad = check("fp8", got, A, B, sa, sb, K, seed)
+        else:
+            A = rng.standard_normal((nblk, 32, K))
+            B = rng.standard_normal((nblk, 32, K))
+            if prec == "fp16":
+                Ab, Bb = A.astype(np.float16), B.astype(np.float16)
+                A, B = Ab.astype(np.float64), Bb.astype(np.float64)
+            else:
+                Ab, Bb = bf16_bits(A), bf16_bits(B)
+                A, B = bf16_decode(Ab), bf16_decode(Bb)
+            seed = (rng.standard_normal((nblk, 32)) * mag * K).astype(np.float32)
+            got = probe(prec, Ab, Bb, None, None, K // 16, nblk, seed)
+            worst, ulps, nbad = check(prec, got, A, B, None, None, K, seed)
+        say(prec, "seeded", mag)
+        print(f"{prec} seeded mag={mag}: max err/bound {worst:.3g}, max err/(2^-23 sum) {ulps:.3g}")
+        assert nbad == 0, (prec, mag, worst)
