@@ -33,12 +33,29 @@ __global__ void __launch_bounds__(256) certify_kernel(const float* __restrict__ 
     const JoinParams P = *pp;
     int64_t count = (int64_t)stats[TJ_ST_CANDIDATES];
     if (count > ccap) count = ccap;
-    const int lane = threadIdx.x & 63;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint64_t jmask = (1ull << jbits) - 1;
     double max_err = 0.0, max_ratio = 0.0;
     uint64_t viol = 0, n_cert = 0, n_amb = 0, n_amb_in = 0;
+    // accepted keys are staged per wave and appended with one atomic per
+    // OCAP keys (most candidates are accepted: one atomic per iteration would
+    // serialise on the output cursor)
+    constexpr int OCAP = 512;
+    __shared__ uint64_t sOut[4][OCAP];
+    int nO = 0;
+    auto flush = [&]() {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        uint64_t pos = 0;
+        if (lane == 0) pos = atomicAdd((unsigned long long*)&stats[TJ_ST_OUT], (unsigned long long)nO);
+        pos = __shfl(pos, 0);
+        for (int t = lane; t < nO; t += 64)
+            if (pos + (uint64_t)t < (uint64_t)out_cap) out_keys[pos + t] = sOut[wave][t];
+        __builtin_amdgcn_wave_barrier();
+        nO = 0;
+    };
     const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
-    const int64_t w = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int64_t w = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave;
     for (int64_t base = w * 64; base < count; base += nwaves * 64) {
         const int64_t t = base + lane;
         bool inside = false;
@@ -78,15 +95,12 @@ __global__ void __launch_bounds__(256) certify_kernel(const float* __restrict__ 
         const uint64_t m = __ballot(inside);
         if (m) {
             const int k = __popcll(m);
-            uint64_t pos = 0;
-            if (lane == 0) pos = atomicAdd((unsigned long long*)&stats[TJ_ST_OUT], (unsigned long long)k);
-            pos = __shfl(pos, 0);
-            if (inside) {
-                const uint64_t at = pos + (uint64_t)tjscreen::lanes_below(m);
-                if (at < (uint64_t)out_cap) out_keys[at] = key;
-            }
+            if (nO + k > OCAP) flush();
+            if (inside) sOut[wave][nO + tjscreen::lanes_below(m)] = key;
+            nO += k;
         }
     }
+    if (nO) flush();
     for (int o = 32; o > 0; o >>= 1) {
         max_err = fmax(max_err, __shfl_xor(max_err, o));
         max_ratio = fmax(max_ratio, __shfl_xor(max_ratio, o));

@@ -9,7 +9,8 @@
 //
 // MI355X design: LSD radix sort over only the bits the key range needs
 // (8 bits per pass), written here (no rocPRIM): per-block digit histograms,
-// a one-workgroup exclusive scan, and a stable scatter in which each wave
+// a two-step exclusive scan (one workgroup per digit row, then the 256 digit
+// totals), and a stable scatter in which each wave
 // ranks its 64 keys by an 8-ballot multisplit and the 4 waves of a block
 // combine through LDS.  The element count is read from device memory, so
 // the whole sort is capturable in a hipGraph behind a producer whose output
@@ -40,28 +41,46 @@ __global__ void __launch_bounds__(SB) hist_kernel(const uint64_t* __restrict__ k
     hist[(int64_t)threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
 }
 
-// Exclusive scan of 256 * nb counters (digit-major) by one workgroup.
-__global__ void __launch_bounds__(1024) scan_kernel(uint32_t* hist, int total) {
-    __shared__ uint32_t part[1024];
-    const int per = (total + 1023) / 1024;
+// Exclusive scan of the 256 x nb digit-major counters in two small steps:
+// one workgroup per digit scans its row (block offsets within the digit) and
+// records the digit total; one workgroup turns the totals into digit bases,
+// added to every row entry by the scatter.
+__global__ void __launch_bounds__(256) scan_rows_kernel(uint32_t* hist, int nb, uint32_t* tot) {
+    __shared__ uint32_t part[256];
+    uint32_t* row = hist + (int64_t)blockIdx.x * nb;
+    const int per = (nb + 255) / 256;
     const int a = threadIdx.x * per;
-    const int b = a + per < total ? a + per : total;
+    const int b = a + per < nb ? a + per : nb;
     uint32_t s = 0;
-    for (int t = a; t < b; ++t) s += hist[t];
+    for (int t = a; t < b; ++t) s += row[t];
     part[threadIdx.x] = s;
     __syncthreads();
-    for (int o = 1; o < 1024; o <<= 1) {
-        uint32_t v = threadIdx.x >= (unsigned)o ? part[threadIdx.x - o] : 0;
+    for (int o = 1; o < 256; o <<= 1) {
+        const uint32_t v = threadIdx.x >= (unsigned)o ? part[threadIdx.x - o] : 0;
         __syncthreads();
         part[threadIdx.x] += v;
         __syncthreads();
     }
     uint32_t run = threadIdx.x ? part[threadIdx.x - 1] : 0;
     for (int t = a; t < b; ++t) {
-        uint32_t v = hist[t];
-        hist[t] = run;
+        const uint32_t v = row[t];
+        row[t] = run;
         run += v;
     }
+    if (threadIdx.x == 255) tot[blockIdx.x] = part[255];
+}
+
+__global__ void __launch_bounds__(256) scan_totals_kernel(uint32_t* tot) {
+    __shared__ uint32_t part[256];
+    part[threadIdx.x] = tot[threadIdx.x];
+    __syncthreads();
+    for (int o = 1; o < 256; o <<= 1) {
+        const uint32_t v = threadIdx.x >= (unsigned)o ? part[threadIdx.x - o] : 0;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    tot[256 + threadIdx.x] = threadIdx.x ? part[threadIdx.x - 1] : 0;  // digit bases
 }
 
 __device__ inline int lanes_below(uint64_t m) {
@@ -70,15 +89,16 @@ __device__ inline int lanes_below(uint64_t m) {
 
 __global__ void __launch_bounds__(SB) scatter_kernel(const uint64_t* __restrict__ src, uint64_t* __restrict__ dst,
                                                      const uint64_t* count, int64_t cap, int shift,
-                                                     const uint32_t* __restrict__ offs, int nb) {
+                                                     const uint32_t* __restrict__ offs, const uint32_t* __restrict__ bases,
+                                                     int nb) {
     __shared__ uint32_t run[256];
     __shared__ uint32_t wcnt[4][256];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, wave = tid >> 6;
     const int64_t n = clamp_count(count, cap);
     const int64_t chunk = ((n + nb - 1) / nb + SB - 1) / SB * SB;
     const int64_t b0 = (int64_t)blockIdx.x * chunk;
     const int64_t b1 = b0 + chunk < n ? b0 + chunk : n;
-    run[tid] = offs[(int64_t)tid * nb + blockIdx.x];
+    run[tid] = bases[tid] + offs[(int64_t)tid * nb + blockIdx.x];
     for (int64_t base = b0; base < b1; base += SB) {
         wcnt[0][tid] = 0; wcnt[1][tid] = 0; wcnt[2][tid] = 0; wcnt[3][tid] = 0;
         __syncthreads();
@@ -150,7 +170,7 @@ namespace tj {
 
 size_t sort_temp_bytes(int64_t capacity) {
     size_t a = ((size_t)(capacity > 0 ? capacity : 1) * 8 + 255) / 256 * 256;
-    return a + (size_t)256 * nblocks_for(capacity) * 4 + 256;
+    return a + (size_t)256 * nblocks_for(capacity) * 4 + 512 * 4 + 256;
 }
 
 int launch_sort(void* temp, uint64_t* keys, const uint64_t* count, int64_t capacity, int bits, hipStream_t s) {
@@ -158,13 +178,15 @@ int launch_sort(void* temp, uint64_t* keys, const uint64_t* count, int64_t capac
     const int nb = nblocks_for(capacity);
     uint64_t* alt = (uint64_t*)temp;
     uint32_t* hist = (uint32_t*)((uint8_t*)temp + ((size_t)capacity * 8 + 255) / 256 * 256);
+    uint32_t* tot = hist + (size_t)256 * nb;  // [0, 256): digit totals, [256, 512): digit bases
     const int passes = (bits + 7) / 8;
     uint64_t* src = keys;
     uint64_t* dst = alt;
     for (int p = 0; p < passes; ++p) {
         hipLaunchKernelGGL(hist_kernel, dim3(nb), dim3(SB), 0, s, src, count, capacity, 8 * p, hist, nb);
-        hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, s, hist, 256 * nb);
-        hipLaunchKernelGGL(scatter_kernel, dim3(nb), dim3(SB), 0, s, src, dst, count, capacity, 8 * p, hist, nb);
+        hipLaunchKernelGGL(scan_rows_kernel, dim3(256), dim3(256), 0, s, hist, nb, tot);
+        hipLaunchKernelGGL(scan_totals_kernel, dim3(1), dim3(256), 0, s, tot);
+        hipLaunchKernelGGL(scatter_kernel, dim3(nb), dim3(SB), 0, s, src, dst, count, capacity, 8 * p, hist, tot + 256, nb);
         uint64_t* t = src; src = dst; dst = t;
     }
     if (src != keys) {
