@@ -24,6 +24,15 @@ int screen_fp8(int dp, bool self_join, const ScreenArgs& a, int grid, hipStream_
 int screen_fp8x3(int dp, bool self_join, const ScreenArgs& a, int grid, hipStream_t s, int query);
 int launch_mirror(const int64_t* pairs, const uint64_t* count, int64_t cap, int64_t n, uint64_t* ids, uint64_t* total,
                   hipStream_t s);
+struct GridLayout {
+    size_t bounds, params, rkeys, skeys, sort_tmp, srow, hkey, hval, cnt, off, bsum, outj, count_nr, count_ns, total;
+    uint64_t hslots;
+    int idxbits;
+};
+GridLayout grid_layout(int64_t nR, int64_t nS, int64_t out_cap);
+int launch_grid_join(const float* R, int64_t nR, const float* S, int64_t nS, int d, double T, bool self_join,
+                     uint8_t* ws, const GridLayout& L, int64_t out_cap, void* pairs, int out_format, uint64_t* stats,
+                     hipStream_t s);
 
 static thread_local char g_err[512] = "";
 
@@ -84,6 +93,8 @@ struct tj_plan {
     unsigned flags;
     int jbits, keybits;
     int bm, panels, col_splits, cols_per_split, grid;
+    size_t off_grid;
+    GridLayout glay;
     size_t off_maxabs, off_params, off_metaR, off_metaS, off_qRh, off_qRl, off_qSh, off_qSl, off_out, off_amb,
         off_aux, off_sort, total;
     hipEvent_t ev[5];
@@ -119,8 +130,9 @@ int tj_plan_create(tj_plan** out, int64_t nR, int64_t nS, int d, int method, int
     *out = nullptr;
     TJ_CHECK(nR >= 0 && nS >= 0 && d >= 1, TJ_ERR_INVALID, "bad shape nR=%lld nS=%lld d=%d", (long long)nR,
              (long long)nS, d);
-    TJ_CHECK(method == TJ_METHOD_CERTIFIED || method == TJ_METHOD_PLAIN_FP64, TJ_ERR_INVALID,
-             "method %d is not served by tj_join", method);
+    TJ_CHECK(method == TJ_METHOD_CERTIFIED || method == TJ_METHOD_PLAIN_FP64 || method == TJ_METHOD_GRID,
+             TJ_ERR_INVALID, "method %d is not served by tj_join", method);
+    TJ_CHECK(method != TJ_METHOD_GRID || d <= 3, TJ_ERR_UNSUPPORTED, "the grid method serves d <= 3 (d=%d)", d);
     TJ_CHECK(out_capacity >= 0 && out_capacity < (1ll << 32), TJ_ERR_INVALID, "out_capacity must be in [0, 2^32)");
     TJ_CHECK(amb_capacity >= 0 && amb_capacity < (1ll << 32), TJ_ERR_INVALID, "amb_capacity must be in [0, 2^32)");
     TJ_CHECK(nR < (1ll << 31) && nS < (1ll << 31), TJ_ERR_UNSUPPORTED, "point sets beyond 2^31 rows");
@@ -190,8 +202,14 @@ int tj_plan_create(tj_plan** out, int64_t nR, int64_t nS, int d, int method, int
         const int64_t items = (int64_t)p->panels * p->col_splits;
         p->grid = (int)std::max<int64_t>(1, std::min<int64_t>(items, maxg));
     }
-    p->off_out = carve(cur, 8 * (size_t)(p->out_cap > 0 ? p->out_cap : 1));
-    p->off_sort = carve(cur, sort_temp_bytes(p->out_cap));
+    if (method == TJ_METHOD_GRID) {
+        p->glay = grid_layout(p->nR, p->nS, p->out_cap);
+        p->off_grid = carve(cur, p->glay.total);
+        p->off_out = p->off_sort = p->off_grid;
+    } else {
+        p->off_out = carve(cur, 8 * (size_t)(p->out_cap > 0 ? p->out_cap : 1));
+        p->off_sort = carve(cur, sort_temp_bytes(p->out_cap));
+    }
     p->total = (cur + 255) / 256 * 256;
     p->timing = false;
     if (flags & TJ_FLAG_TIMING) {
@@ -299,6 +317,17 @@ int tj_join(tj_plan* p, const float* R, const float* S, double T, void* workspac
             TJ_MARK(2);
         }
         TJ_MARK(3);
+    } else if (p->method == TJ_METHOD_GRID) {
+        if ((rc = launch_scale_scan(R, p->nR, p->d, maxabs, stats, s))) return rc;
+        if (!p->self_join && (rc = launch_scale_scan(S, p->nS, p->d, maxabs, stats, s))) return rc;
+        TJ_MARK(1);
+        if ((rc = launch_grid_join(R, p->nR, S, p->nS, p->d, T, p->self_join, ws + p->off_grid, p->glay, p->out_cap,
+                                   pairs, out_format, stats, s)))
+            return rc;
+        TJ_MARK(2);
+        TJ_MARK(3);
+        TJ_MARK(4);
+        return TJ_OK;
     } else {
         TJ_MARK(1);
         if ((rc = launch_plain(R, p->nR, S, p->nS, p->d, p->jbits, T, p->self_join, out_keys, p->out_cap, stats, s)))

@@ -1,0 +1,421 @@
+// grid.hip -- uniform-grid epsilon-join for low-dimensional points (d <= 3).
+//
+// Counterpart of GDS-Join (GDS/main.cu:690-950 index build on the CPU,
+// GDS/kernel.cu:132-486 one query per thread over 3^k neighbour cells,
+// GDS/GPU.cu:775-872 batches with thrust::sort_by_key of the results).
+// Same index idea -- cells of side >= eps, a query only visits its own and
+// the adjacent cells -- rebuilt for MI355X:
+//  * the index is built on the device: bounds (order-preserving integer
+//    atomics), cell keys, my radix sort by cell, S gathered into cell order
+//    as 16-byte rows, and an open-addressing hash of the non-empty cells
+//    (memory bounded by 2|S| slots however sparse the grid is);
+//  * queries run in cell order (R is sorted by cell too), so the 64 lanes of
+//    a wave visit the same neighbour cells and the same S rows;
+//  * output without a global sort: a count pass, an exclusive scan, a write
+//    pass at exact per-row offsets, and a per-row sort of the j's, so the
+//    (i, j) rows come out in order;
+//  * the predicate is the sequential binary64 difference form of the CPU
+//    oracle (d2 <= T; GDS used sqrt(d2) <= eps, GDS/kernel.cu:368-371).
+// The cell side is sqrt(T) * (1 + 2^-10) (or larger when the cell ids would
+// not fit a 64-bit key): every pair with d2_seq <= T has all coordinate
+// differences below one cell side, so it lies in adjacent cells.
+#include "tj_internal.h"
+
+namespace {
+
+struct GridParams {
+    double lo[3];     // grid origin per dimension (one cell below the minimum)
+    double inv_side;  // 1 / side
+    double side;
+    uint64_t ncell[3];
+    uint64_t mul[3];  // linear id = sum c_k * mul[k]
+    int keybits, idxbits, d, pad;
+};
+
+__device__ inline uint32_t f2ord(float f) {
+    const uint32_t u = __builtin_bit_cast(uint32_t, f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ inline float ord2f(uint32_t o) {
+    const uint32_t u = (o & 0x80000000u) ? (o & 0x7FFFFFFFu) : ~o;
+    return __builtin_bit_cast(float, u);
+}
+
+// bounds[0..2] = ordered min, bounds[3..5] = ordered max
+__global__ void __launch_bounds__(256) bounds_kernel(const float* __restrict__ x, int64_t n, int d, uint32_t* bounds) {
+    uint32_t mn[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, mx[3] = {0, 0, 0};
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
+        for (int k = 0; k < d; ++k) {
+            const uint32_t o = f2ord(x[t * d + k]);
+            mn[k] = o < mn[k] ? o : mn[k];
+            mx[k] = o > mx[k] ? o : mx[k];
+        }
+    for (int k = 0; k < d; ++k) {
+        for (int s = 32; s > 0; s >>= 1) {
+            const uint32_t a = __shfl_xor(mn[k], s), b = __shfl_xor(mx[k], s);
+            mn[k] = a < mn[k] ? a : mn[k];
+            mx[k] = b > mx[k] ? b : mx[k];
+        }
+        if ((threadIdx.x & 63) == 0) {
+            atomicMin(&bounds[k], mn[k]);
+            atomicMax(&bounds[3 + k], mx[k]);
+        }
+    }
+}
+
+__global__ void init_bounds_kernel(uint32_t* bounds) {
+    if (threadIdx.x < 3) { bounds[threadIdx.x] = 0xFFFFFFFFu; bounds[3 + threadIdx.x] = 0; }
+}
+
+__global__ void grid_params_kernel(const uint32_t* bounds, double T, int d, int idxbits, GridParams* gp) {
+    if (threadIdx.x != 0) return;
+    GridParams g;
+    double side = sqrt(T) * (1.0 + 0x1p-10);
+    if (!(side > 0.0)) side = 0x1p-100;  // T = 0: only identical points pair up
+    double lo[3] = {0, 0, 0}, ext[3] = {0, 0, 0};
+    for (int k = 0; k < d; ++k) {
+        const double a = (double)ord2f(bounds[k]), b = (double)ord2f(bounds[3 + k]);
+        lo[k] = a;
+        ext[k] = b > a ? b - a : 0.0;
+    }
+    // widen the cell until the linear id of every (padded) cell fits beside the index bits
+    for (;;) {
+        int bits = 0;
+        for (int k = 0; k < d; ++k) {
+            const double nc = floor(ext[k] / side) + 4.0;
+            int b = 0;
+            while (b < 62 && ldexp(1.0, b) < nc) ++b;
+            bits += b;
+        }
+        if (bits + idxbits <= 63) break;
+        side *= 2.0;
+    }
+    uint64_t mul = 1;
+    int bits = 0;
+    for (int k = 0; k < 3; ++k) {
+        if (k < d) {
+            g.lo[k] = lo[k] - side;  // cell 0 stays empty: neighbours c-1 >= 0
+            g.ncell[k] = (uint64_t)(floor(ext[k] / side) + 4.0);
+        } else {
+            g.lo[k] = 0.0;
+            g.ncell[k] = 1;
+        }
+        g.mul[k] = mul;
+        mul *= g.ncell[k];
+    }
+    while (bits < 63 && (1ull << bits) < mul) ++bits;
+    g.side = side;
+    g.inv_side = 1.0 / side;
+    g.keybits = bits;
+    g.idxbits = idxbits;
+    g.d = d;
+    g.pad = 0;
+    *gp = g;
+}
+
+__device__ inline void cell_of(const float* x, const GridParams& g, int64_t c[3]) {
+    for (int k = 0; k < 3; ++k) c[k] = 0;
+    for (int k = 0; k < g.d; ++k) {
+        int64_t v = (int64_t)floor(((double)x[k] - g.lo[k]) * g.inv_side);
+        if (v < 1) v = 1;
+        if (v > (int64_t)g.ncell[k] - 2) v = (int64_t)g.ncell[k] - 2;
+        c[k] = v;
+    }
+}
+
+__global__ void __launch_bounds__(256) keys_kernel(const float* __restrict__ x, int64_t n, const GridParams* __restrict__ gp,
+                                                   uint64_t* __restrict__ keys) {
+    const GridParams g = *gp;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        int64_t c[3];
+        cell_of(x + t * g.d, g, c);
+        const uint64_t id = (uint64_t)c[0] * g.mul[0] + (uint64_t)c[1] * g.mul[1] + (uint64_t)c[2] * g.mul[2];
+        keys[t] = (id << g.idxbits) | (uint64_t)t;
+    }
+}
+
+__device__ inline uint64_t hmix(uint64_t k) {
+    k ^= k >> 33; k *= 0xff51afd7ed558ccdull; k ^= k >> 33; k *= 0xc4ceb9fe1a85ec53ull; k ^= k >> 33;
+    return k;
+}
+constexpr uint64_t EMPTY = ~0ull;
+
+// S in cell order: 16-byte rows (x, y, z, original index as bits); run starts
+// of each non-empty cell inserted into the hash (slot: key, start, end).
+__global__ void __launch_bounds__(256) gather_kernel(const float* __restrict__ S, int64_t n, const GridParams* __restrict__ gp,
+                                                     const uint64_t* __restrict__ keys, float4* __restrict__ srow,
+                                                     uint64_t* __restrict__ hkey, uint2* __restrict__ hval, uint64_t hmask) {
+    const GridParams g = *gp;
+    const uint64_t imask = (1ull << g.idxbits) - 1;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t kk = keys[p];
+        const uint64_t j = kk & imask, cell = kk >> g.idxbits;
+        float4 r = make_float4(0.f, 0.f, 0.f, __builtin_bit_cast(float, (uint32_t)j));
+        const float* y = S + j * g.d;
+        r.x = y[0];
+        if (g.d > 1) r.y = y[1];
+        if (g.d > 2) r.z = y[2];
+        srow[p] = r;
+        if (p == 0 || (keys[p - 1] >> g.idxbits) != cell) {
+            int64_t e = p + 1;
+            while (e < n && (keys[e] >> g.idxbits) == cell) ++e;
+            uint64_t h = hmix(cell) & hmask;
+            for (;;) {
+                const unsigned long long prev = atomicCAS((unsigned long long*)&hkey[h], (unsigned long long)EMPTY,
+                                                          (unsigned long long)cell);
+                if (prev == EMPTY || prev == cell) break;
+                h = (h + 1) & hmask;
+            }
+            hval[h] = make_uint2((uint32_t)p, (uint32_t)e);
+        }
+    }
+}
+
+__device__ inline uint2 lookup(const uint64_t* __restrict__ hkey, const uint2* __restrict__ hval, uint64_t hmask, uint64_t cell) {
+    uint64_t h = hmix(cell) & hmask;
+    for (;;) {
+        const uint64_t k = hkey[h];
+        if (k == cell) return hval[h];
+        if (k == EMPTY) return make_uint2(0, 0);
+        h = (h + 1) & hmask;
+    }
+}
+
+__device__ inline double d2_row(const float* q, const float4& y, int d) {
+    double s = 0.0, t;
+    t = (double)q[0] - (double)y.x; s = s + t * t;
+    if (d > 1) { t = (double)q[1] - (double)y.y; s = s + t * t; }
+    if (d > 2) { t = (double)q[2] - (double)y.z; s = s + t * t; }
+    return s;
+}
+
+// One query per lane, queries in cell order.  mode 0: count into cnt[i];
+// mode 1: write the j's at out_j[off[i] + k].
+template <int MODE>
+__global__ void __launch_bounds__(256) probe_kernel(const float* __restrict__ R, int64_t nR, const GridParams* __restrict__ gp,
+                                                    const uint64_t* __restrict__ rkeys, const float4* __restrict__ srow,
+                                                    const uint64_t* __restrict__ hkey, const uint2* __restrict__ hval,
+                                                    uint64_t hmask, double T, int self_join, uint32_t* __restrict__ cnt,
+                                                    const uint64_t* __restrict__ off, uint32_t* __restrict__ out_j,
+                                                    int64_t out_cap, uint64_t* __restrict__ stats) {
+    const GridParams g = *gp;
+    const uint64_t imask = (1ull << g.idxbits) - 1;
+    uint64_t cand = 0;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nR; t += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t i = rkeys[t] & imask;
+        float q[3] = {0.f, 0.f, 0.f};
+        for (int k = 0; k < g.d; ++k) q[k] = R[i * g.d + k];
+        int64_t c[3];
+        cell_of(q, g, c);
+        uint32_t n = 0;
+        uint64_t base = 0;
+        if (MODE == 1) base = off[i];
+        const int r1 = g.d > 1 ? 1 : 0, r2 = g.d > 2 ? 1 : 0;
+        for (int dz = -r2; dz <= r2; ++dz)
+            for (int dy = -r1; dy <= r1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx) {
+                    const uint64_t cell = (uint64_t)(c[0] + dx) * g.mul[0] + (uint64_t)(c[1] + dy) * g.mul[1] +
+                                          (uint64_t)(c[2] + dz) * g.mul[2];
+                    const uint2 run = lookup(hkey, hval, hmask, cell);
+                    for (uint32_t p = run.x; p < run.y; ++p) {
+                        const float4 y = srow[p];
+                        const uint32_t j = __builtin_bit_cast(uint32_t, y.w);
+                        if (self_join && j <= i) continue;
+                        ++cand;
+                        if (d2_row(q, y, g.d) <= T) {
+                            if (MODE == 1 && base + n < (uint64_t)out_cap) out_j[base + n] = j;
+                            ++n;
+                        }
+                    }
+                }
+        if (MODE == 0) cnt[i] = n;
+    }
+    if (MODE == 0) {
+        for (int s = 32; s > 0; s >>= 1) cand += __shfl_xor(cand, s);
+        if ((threadIdx.x & 63) == 0 && cand) atomicAdd((unsigned long long*)&stats[TJ_ST_GRID_CAND], (unsigned long long)cand);
+    }
+}
+
+// ---- exclusive scan of n uint32 counts into uint64 offsets (total -> stats[OUT])
+constexpr int SCAN_B = 1024;
+__global__ void __launch_bounds__(256) scan_block_sums(const uint32_t* __restrict__ in, int64_t n, uint64_t* __restrict__ bsum) {
+    __shared__ uint64_t part[256];
+    const int64_t b0 = (int64_t)blockIdx.x * SCAN_B;
+    uint64_t s = 0;
+    for (int k = threadIdx.x; k < SCAN_B; k += 256) if (b0 + k < n) s += in[b0 + k];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) part[threadIdx.x] += part[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) bsum[blockIdx.x] = part[0];
+}
+__global__ void __launch_bounds__(1024) scan_sums(uint64_t* bsum, int64_t nb, uint64_t* stats) {
+    __shared__ uint64_t part[1024];
+    const int64_t per = (nb + 1023) / 1024;
+    const int64_t a = threadIdx.x * per, b = a + per < nb ? a + per : nb;
+    uint64_t s = 0;
+    for (int64_t t = a; t < b; ++t) s += bsum[t];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        const uint64_t v = threadIdx.x >= (unsigned)o ? part[threadIdx.x - o] : 0;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint64_t run = threadIdx.x ? part[threadIdx.x - 1] : 0;
+    for (int64_t t = a; t < b; ++t) {
+        const uint64_t v = bsum[t];
+        bsum[t] = run;
+        run += v;
+    }
+    if (threadIdx.x == 1023) stats[TJ_ST_OUT] = part[1023];
+}
+__global__ void __launch_bounds__(256) scan_apply(const uint32_t* __restrict__ in, int64_t n, const uint64_t* __restrict__ bsum,
+                                                  uint64_t* __restrict__ out) {
+    __shared__ uint64_t part[256];
+    const int64_t b0 = (int64_t)blockIdx.x * SCAN_B;
+    // each thread owns 4 consecutive entries
+    uint64_t v[4], s = 0;
+    for (int u = 0; u < 4; ++u) {
+        const int64_t t = b0 + threadIdx.x * 4 + u;
+        v[u] = t < n ? in[t] : 0;
+        s += v[u];
+    }
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 1; o < 256; o <<= 1) {
+        const uint64_t w = threadIdx.x >= (unsigned)o ? part[threadIdx.x - o] : 0;
+        __syncthreads();
+        part[threadIdx.x] += w;
+        __syncthreads();
+    }
+    uint64_t run = bsum[blockIdx.x] + (threadIdx.x ? part[threadIdx.x - 1] : 0);
+    for (int u = 0; u < 4; ++u) {
+        const int64_t t = b0 + threadIdx.x * 4 + u;
+        if (t < n) out[t] = run;
+        run += v[u];
+    }
+}
+
+// Per row: sort its j's (insertion sort; rows are short) and write (i, j).
+__global__ void __launch_bounds__(256) finish_kernel(int64_t nR, const uint32_t* __restrict__ cnt, const uint64_t* __restrict__ off,
+                                                     uint32_t* __restrict__ out_j, int64_t out_cap, void* __restrict__ pairs,
+                                                     int fmt) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nR; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t b = off[i];
+        const uint32_t n = cnt[i];
+        if (b + n > (uint64_t)out_cap) continue;
+        uint32_t* seg = out_j + b;
+        for (uint32_t a = 1; a < n; ++a) {
+            const uint32_t v = seg[a];
+            uint32_t k = a;
+            while (k > 0 && seg[k - 1] > v) { seg[k] = seg[k - 1]; --k; }
+            seg[k] = v;
+        }
+        for (uint32_t a = 0; a < n; ++a) {
+            if (fmt == TJ_OUT_I32) ((int2*)pairs)[b + a] = make_int2((int)i, (int)seg[a]);
+            else ((longlong2*)pairs)[b + a] = make_longlong2((long long)i, (long long)seg[a]);
+        }
+    }
+}
+
+}  // namespace
+
+namespace tj {
+
+struct GridLayout {
+    size_t bounds, params, rkeys, skeys, sort_tmp, srow, hkey, hval, cnt, off, bsum, outj, count_nr, count_ns, total;
+    uint64_t hslots;
+    int idxbits;
+};
+
+GridLayout grid_layout(int64_t nR, int64_t nS, int64_t out_cap) {
+    GridLayout L{};
+    size_t cur = 0;
+    auto carve = [&](size_t bytes) { size_t at = (cur + 255) / 256 * 256; cur = at + bytes; return at; };
+    const int64_t nmax = nR > nS ? nR : nS;
+    uint64_t hs = 1;
+    while (hs < (uint64_t)(2 * (nS > 0 ? nS : 1))) hs <<= 1;
+    L.hslots = hs;
+    L.idxbits = bits_for(nmax > 1 ? nmax : 2);
+    L.bounds = carve(64);
+    L.params = carve(sizeof(GridParams));
+    L.count_nr = carve(8);
+    L.count_ns = carve(8);
+    L.rkeys = carve(8 * (size_t)(nR > 0 ? nR : 1));
+    L.skeys = carve(8 * (size_t)(nS > 0 ? nS : 1));
+    L.sort_tmp = carve(sort_temp_bytes(nmax > 0 ? nmax : 1));
+    L.srow = carve(16 * (size_t)(nS > 0 ? nS : 1));
+    L.hkey = carve(8 * hs);
+    L.hval = carve(8 * hs);
+    L.cnt = carve(4 * (size_t)(nR > 0 ? nR : 1));
+    L.off = carve(8 * (size_t)(nR > 0 ? nR : 1));
+    L.bsum = carve(8 * (size_t)((nR + SCAN_B - 1) / SCAN_B + 1));
+    L.outj = carve(4 * (size_t)(out_cap > 0 ? out_cap : 1));
+    L.total = (cur + 255) / 256 * 256;
+    return L;
+}
+
+static unsigned blocks_for(int64_t n, int64_t cap) {
+    int64_t b = (n + 255) / 256;
+    if (b < 1) b = 1;
+    if (b > cap) b = cap;
+    return (unsigned)b;
+}
+
+int launch_grid_join(const float* R, int64_t nR, const float* S, int64_t nS, int d, double T, bool self_join,
+                     uint8_t* ws, const GridLayout& L, int64_t out_cap, void* pairs, int out_format, uint64_t* stats,
+                     hipStream_t s) {
+    TJ_CHECK(d >= 1 && d <= 3, TJ_ERR_UNSUPPORTED, "grid join serves d <= 3 (d=%d)", d);
+    TJ_CHECK(nR < (1ll << 32) && nS < (1ll << 32), TJ_ERR_UNSUPPORTED, "grid join: point sets beyond 2^32");
+    if (self_join) { S = R; nS = nR; }
+    if (nR == 0 || nS == 0) return TJ_OK;
+    uint32_t* bounds = (uint32_t*)(ws + L.bounds);
+    GridParams* gp = (GridParams*)(ws + L.params);
+    uint64_t* rkeys = (uint64_t*)(ws + L.rkeys);
+    uint64_t* skeys = (uint64_t*)(ws + L.skeys);
+    float4* srow = (float4*)(ws + L.srow);
+    uint64_t* hkey = (uint64_t*)(ws + L.hkey);
+    uint2* hval = (uint2*)(ws + L.hval);
+    uint32_t* cnt = (uint32_t*)(ws + L.cnt);
+    uint64_t* off = (uint64_t*)(ws + L.off);
+    uint64_t* bsum = (uint64_t*)(ws + L.bsum);
+    uint32_t* outj = (uint32_t*)(ws + L.outj);
+    uint64_t* cR = (uint64_t*)(ws + L.count_nr);
+    uint64_t* cS = (uint64_t*)(ws + L.count_ns);
+    int rc;
+    const unsigned gb = (unsigned)(num_cus() * 4);
+    hipLaunchKernelGGL(init_bounds_kernel, dim3(1), dim3(64), 0, s, bounds);
+    hipLaunchKernelGGL(bounds_kernel, dim3(blocks_for(nR, gb)), dim3(256), 0, s, R, nR, d, bounds);
+    if (!self_join) hipLaunchKernelGGL(bounds_kernel, dim3(blocks_for(nS, gb)), dim3(256), 0, s, S, nS, d, bounds);
+    hipLaunchKernelGGL(grid_params_kernel, dim3(1), dim3(64), 0, s, bounds, T, d, L.idxbits, gp);
+    if ((rc = launch_fill_u64(cR, 1, (uint64_t)nR, s))) return rc;
+    if ((rc = launch_fill_u64(cS, 1, (uint64_t)nS, s))) return rc;
+    if ((rc = launch_fill_u64(hkey, (int64_t)L.hslots, EMPTY, s))) return rc;
+    // R and S sorted by cell (key = cell id << idxbits | index)
+    hipLaunchKernelGGL(keys_kernel, dim3(blocks_for(nR, gb)), dim3(256), 0, s, R, nR, gp, rkeys);
+    if ((rc = launch_sort(ws + L.sort_tmp, rkeys, cR, nR, 63, s))) return rc;
+    hipLaunchKernelGGL(keys_kernel, dim3(blocks_for(nS, gb)), dim3(256), 0, s, S, nS, gp, skeys);
+    if ((rc = launch_sort(ws + L.sort_tmp, skeys, cS, nS, 63, s))) return rc;
+    hipLaunchKernelGGL(gather_kernel, dim3(blocks_for(nS, gb)), dim3(256), 0, s, S, nS, gp, skeys, srow, hkey, hval,
+                       (uint64_t)(L.hslots - 1));
+    // count -> scan -> write -> per-row sort
+    const unsigned pb = blocks_for(nR, (int64_t)num_cus() * 16);
+    hipLaunchKernelGGL(probe_kernel<0>, dim3(pb), dim3(256), 0, s, R, nR, gp, rkeys, srow, hkey, hval,
+                       (uint64_t)(L.hslots - 1), T, (int)self_join, cnt, off, outj, out_cap, stats);
+    const int64_t nb = (nR + SCAN_B - 1) / SCAN_B;
+    hipLaunchKernelGGL(scan_block_sums, dim3((unsigned)nb), dim3(256), 0, s, cnt, nR, bsum);
+    hipLaunchKernelGGL(scan_sums, dim3(1), dim3(1024), 0, s, bsum, nb, stats);
+    hipLaunchKernelGGL(scan_apply, dim3((unsigned)nb), dim3(256), 0, s, cnt, nR, bsum, off);
+    hipLaunchKernelGGL(probe_kernel<1>, dim3(pb), dim3(256), 0, s, R, nR, gp, rkeys, srow, hkey, hval,
+                       (uint64_t)(L.hslots - 1), T, (int)self_join, cnt, off, outj, out_cap, stats);
+    hipLaunchKernelGGL(finish_kernel, dim3(blocks_for(nR, gb)), dim3(256), 0, s, nR, cnt, off, outj, out_cap, pairs,
+                       out_format);
+    TJ_HIP(hipGetLastError());
+    return TJ_OK;
+}
+
+}  // namespace tj

@@ -157,17 +157,20 @@ class Plan:
 
 def _initial_capacities(nR: int, nS: int | None, self_join: bool) -> tuple[int, int]:
     pairs = nR * (nR - 1) // 2 if self_join else nR * (nS or 0)
-    out = int(min(max(pairs, 1), max(1 << 20, min(pairs // 512, 1 << 26))))
+    out = int(min(max(pairs, 1), max(1 << 20, min(pairs // 512, 1 << 26), 16 * max(nR, nS or 0))))
     amb = int(min(max(pairs, 1), 1 << 22))
     return out, amb
 
 
-def join(R, S=None, eps: float | None = None, *, T: float | None = None, method: str = "certified",
+def join(R, S=None, eps: float | None = None, *, T: float | None = None, method: str = "auto",
          precision: str = "auto", out_dtype=None, audit: bool = False, return_stats: bool = False):
     """Exact epsilon-join.  Returns sorted (i, j) pairs [P, 2] (numpy).
 
     A self-join (S is None) returns i < j.  The predicate is
     d2_seq(R_i, S_j) <= T with T = fl64(eps^2) unless T is given.
+    method: 'auto' (grid for d <= 3, else certified), 'certified' (MFMA
+    screen + FP64 refinement), 'grid' (uniform grid, d <= 3), 'plain_fp64'
+    (FP64 brute force).
     """
     if (eps is None) == (T is None):
         raise ValueError("give exactly one of eps and T")
@@ -183,9 +186,11 @@ def join(R, S=None, eps: float | None = None, *, T: float | None = None, method:
     nS = None if Sd is None else Sd.shape[0]
     if Sd is not None and Sd.shape[1] != d:
         raise ValueError("R and S differ in dimension")
+    if method == "auto":
+        method = "grid" if d <= 3 else "certified"
     if precision == "auto":
         from .autotune import choose_precision
-        precision = choose_precision(d)
+        precision = choose_precision(d) if method == "certified" else "none"
     if out_dtype is None:
         out_dtype = np.int32 if max(nR, nS or 0) < 2**31 else np.int64
     out_cap, amb_cap = _initial_capacities(nR, nS, S is None)

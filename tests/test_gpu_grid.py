@@ -1,0 +1,90 @@
+"""Uniform-grid join (d <= 3) against the CPU FP64 oracle, bit-exact pair sets."""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[case]", *a, file=sys.stderr, flush=True)
+
+
+@pytest.fixture(scope="module")
+def tj():
+    import torch
+    assert torch.cuda.is_available()
+    import tensorjoin_variants_amd as T
+    T.load_library()
+    return T
+
+
+def check(tj, oracle, R, S, T, name):
+    say(name, R.shape, None if S is None else S.shape, T)
+    got, st = tj.join(R, S, T=T, method="grid", out_dtype=np.int64, return_stats=True)
+    want = oracle.join(R, S, T) if R.shape[0] * (R.shape[0] if S is None else S.shape[0]) < 4e8 else oracle.grid_join(R, S, T)
+    assert got.shape == want.shape, (name, got.shape, want.shape)
+    assert np.array_equal(got, want), name
+    return st
+
+
+@pytest.mark.parametrize("d", [1, 2, 3])
+def test_grid_random(tj, oracle, d):
+    rng = np.random.default_rng(30 + d)
+    R = rng.random((5000, d)).astype(np.float32)
+    S = rng.random((4000, d)).astype(np.float32)
+    eps = {1: 0.0005, 2: 0.01, 3: 0.03}[d]
+    st = check(tj, oracle, R, S, eps * eps, f"rxs d{d}")
+    assert st["grid_candidates"] < R.shape[0] * S.shape[0] / 10
+    check(tj, oracle, R, None, eps * eps, f"self d{d}")
+
+
+def test_grid_boundaries(tj, oracle):
+    # points on an exact lattice of step eps: many pairs at exactly d2 = T
+    g = np.arange(30, dtype=np.float32) * np.float32(0.125)
+    X = np.stack(np.meshgrid(g, g, g[:5], indexing="ij"), -1).reshape(-1, 3).astype(np.float32)
+    check(tj, oracle, X, None, 0.125 ** 2, "lattice self T=eps^2")
+    check(tj, oracle, X, X[::3].copy(), 0.125 ** 2, "lattice rxs")
+    # duplicates and T = 0
+    rng = np.random.default_rng(8)
+    Y = rng.random((500, 3)).astype(np.float32)
+    Y[250:] = Y[:250]
+    check(tj, oracle, Y, None, 0.0, "duplicates T=0")
+    # negative / mixed-sign coordinates, large offsets
+    Z = (rng.standard_normal((3000, 2)) * 100 + 1e4).astype(np.float32)
+    check(tj, oracle, Z, None, 4.0, "offset 1e4")
+    # one dimension degenerate (all equal)
+    W = rng.random((2000, 3)).astype(np.float32)
+    W[:, 1] = 0.5
+    check(tj, oracle, W, None, 0.02 ** 2, "flat dim")
+
+
+def test_grid_edge_sizes(tj, oracle):
+    rng = np.random.default_rng(9)
+    X = rng.random((70, 3)).astype(np.float32)
+    for nR, nS in [(1, 1), (1, 70), (70, 1)]:
+        check(tj, oracle, X[:nR], X[:nS], 0.3 ** 2, f"{nR}x{nS}")
+    check(tj, oracle, X[:1], None, 0.3 ** 2, "self 1")
+    say("grid empty")
+    assert tj.join(X[:0], X, T=1.0, method="grid").shape == (0, 2)
+
+
+def test_grid_blobs_medium(tj, oracle):
+    """The grid workload's distribution (90% uniform, 10% in tight blobs) at 1/16 size."""
+    from tensorjoin_variants_amd import workloads
+    w = workloads.grid_rxs(n=262_144)
+    st = check(tj, oracle, w.R, w.S, w.T, "grid_rxs 262144")
+    say("grid stats", st["out"], st["grid_candidates"])
+
+
+def test_grid_full_size(tj, oracle):
+    """The full grid workload (4,194,304 x 4,194,304, d = 3) against the CPU grid oracle."""
+    from tensorjoin_variants_amd import workloads
+    w = workloads.grid_rxs()
+    say("grid full: GPU")
+    got, st = tj.join(w.R, w.S, T=w.T, method="grid", out_dtype=np.int64, return_stats=True)
+    say("grid full: CPU oracle", st["out"], st["grid_candidates"])
+    want = oracle.grid_join(w.R, w.S, w.T)
+    assert got.shape == want.shape
+    assert np.array_equal(got, want)
