@@ -1,19 +1,22 @@
 #!/usr/bin/env python3
 """Benchmark: exact epsilon-join pair evaluations per second on one MI355X.
 
-Workload (default): the d = 128 certified-screen join -- |R| = 1,048,576,
-|S| = 262,144, d = 128, half-normal coordinates with 40% exact zeros, rows
-L2-normalised, eps at selectivity 1e-4 -- with inputs resident in HBM.
-A step is one full join: row metadata, FP16 MFMA screen, FP64 refinement of
-the ambiguous band, radix sort, unpack to int32 (i, j) rows.
+Default workload (the metric's configuration): the d = 128 certified-screen
+join -- |R| = 1,048,576, |S| = 262,144, d = 128, half-normal coordinates with
+40% exact zeros, rows L2-normalised, eps at selectivity 1e-4 -- inputs
+resident in HBM.  A step is one full join: row metadata, MFMA screen,
+per-pair certificate + FP64 refinement of the ambiguous band, radix sort,
+unpack to int32 (i, j) rows.
 
-Also measured in the same run:
+Measured in the same run:
 * the plain GPU method (FP64 brute force, no certificate) on the same
   workload, and the pair-set difference between the two (must be 0);
-* the screen kernel against the FP16 MFMA peak (roofline);
+* the screen kernel against the MFMA peak of its precision (roofline);
 * the CPU oracle (C, OpenMP, sequential FP64) on a bounded row sample.
 
-Prints one JSON line.
+Other workloads (--workload): clusters_selfjoin (8192 x 32 self-join),
+grid_rxs (4,194,304^2, d = 3, uniform grid), heavytail_fp8 (262,144^2,
+d = 512, Student-t, FP8 screen).  Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -31,6 +34,7 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "exact ε-join pair evals/s: MFMA-certified vs plain GPU, 0 pair diffs vs FP64"
 PEAK = {"fp16": 2.5e15, "bf16": 2.5e15, "fp16x3": 2.5e15, "bf16x3": 2.5e15, "fp8": 5.0e15, "fp8x3": 5.0e15}
+HBM = 8.0e12
 
 
 def log(*a):
@@ -42,19 +46,52 @@ def cpu_baseline(w, seconds_target: float = 12.0) -> dict:
     from oracle import oracle
     oracle.build()
     threads = len(os.sched_getaffinity(0))
-    nS = w.R.shape[0] if w.S is None else w.S.shape[0]
-    k = 32
+    S = w.S
+    nS = w.R.shape[0] if S is None else S.shape[0]
+    if w.name == "grid_rxs":
+        t0 = time.perf_counter()
+        got = oracle.grid_join(w.R, w.S, w.T, threads=threads)
+        dt = time.perf_counter() - t0
+        pairs = w.R.shape[0] * nS
+        return {"value": pairs / dt, "unit": "pair-evals/s (logical)", "cores": threads, "kind": "port",
+                "sample": f"full grid join on the CPU (uniform grid, sequential FP64 predicate), {len(got)} pairs, "
+                          f"{dt:.2f} s"}
+    k = 16
     t0 = time.perf_counter()
-    oracle.join(w.R[:k], w.S, w.T, threads=threads, count_only=True)
+    oracle.join(w.R[:k], S, w.T, threads=threads, count_only=True)
     dt = time.perf_counter() - t0
     k2 = int(min(w.R.shape[0], max(k, k * seconds_target / max(dt, 1e-3))))
     t0 = time.perf_counter()
-    cnt = oracle.join(w.R[:k2], w.S, w.T, threads=threads, count_only=True)
+    cnt = oracle.join(w.R[:k2], S if S is not None else w.R, w.T, threads=threads, count_only=True)
     dt = time.perf_counter() - t0
     pairs = k2 * nS
     return {"value": pairs / dt, "unit": "pair-evals/s", "cores": threads, "kind": "port",
-            "sample": f"first {k2} rows of R x all {nS} rows of S ({pairs:.3e} pairs, {cnt} inside), "
-                      f"{dt:.2f} s"}
+            "sample": f"first {k2} rows of R x all {nS} rows of S ({pairs:.3e} pairs, {cnt} inside), {dt:.2f} s"}
+
+
+def make_workload(name: str, scale: float):
+    from tensorjoin_variants_amd import workloads
+    if name == "certified_d128":
+        return workloads.certified_d128(nR=int(1_048_576 * scale), nS=int(262_144 * scale))
+    if name == "heavytail_fp8":
+        return workloads.heavytail_fp8(nR=int(262_144 * scale), nS=int(262_144 * scale))
+    if name == "clusters_selfjoin":
+        return workloads.clusters_selfjoin(n=int(8192 * scale))
+    if name == "grid_rxs":
+        return workloads.grid_rxs(n=int(4_194_304 * scale))
+    raise SystemExit(f"unknown workload {name}")
+
+
+def timed(plan, R, S, T, out, fmt, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        plan.enqueue(R, S, T, out, fmt)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        plan.enqueue(R, S, T, out, fmt)
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / steps
 
 
 def main() -> int:
@@ -62,7 +99,9 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--precision", default="fp16")
+    ap.add_argument("--workload", default="certified_d128",
+                    choices=["certified_d128", "heavytail_fp8", "clusters_selfjoin", "grid_rxs"])
+    ap.add_argument("--precision", default=None)
     ap.add_argument("--scale", type=float, default=1.0, help="row-count scale (1.0 = full workload)")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
@@ -73,44 +112,37 @@ def main() -> int:
         return 2
 
     import torch
-    from tensorjoin_variants_amd import workloads
+    from tensorjoin_variants_amd import _lib
     from tensorjoin_variants_amd.join import Plan, _as_device
 
-    nR, nS = int(1_048_576 * args.scale), int(262_144 * args.scale)
-    log(f"generating certified_d128 workload {nR}x{nS}x128")
+    grid = args.workload == "grid_rxs"
+    prec = args.precision or ("fp8" if args.workload == "heavytail_fp8" else "fp16")
+    method = "grid" if grid else "certified"
+    log(f"generating {args.workload} (scale {args.scale})")
     t0 = time.perf_counter()
-    w = workloads.certified_d128(nR=nR, nS=nS)
+    w = make_workload(args.workload, args.scale)
     log(f"generated in {time.perf_counter() - t0:.1f}s, eps={w.eps:.6g} T={w.T!r}")
-    R, S = _as_device(w.R), _as_device(w.S)
-    pairs_total = nR * nS
-    d = w.R.shape[1]
+    R = _as_device(w.R)
+    S = None if w.S is None else _as_device(w.S)
+    nR, d = w.R.shape
+    nS = None if w.S is None else w.S.shape[0]
+    pairs_total = w.pairs_evaluated
+    fmt = "int32" if w.out_dtype == np.int32 else "int64"
+    tdt = torch.int32 if fmt == "int32" else torch.int64
 
-    plan = Plan(nR, nS, d, "certified", args.precision, out_capacity=int(pairs_total * 2e-4) + 4096,
-                amb_capacity=1 << 22, timing=True)
+    guess = int(pairs_total * 2e-4) + 4096 if not grid else 16 * nR
+    plan = Plan(nR, nS, d, method, prec if not grid else "none", out_capacity=guess,
+                amb_capacity=int(pairs_total * 2e-4) + 4096, timing=True)
     log("plan", json.dumps(plan.describe()))
-    res = plan.run(R, S, w.T, np.int32)  # sizes the capacities
+    res = plan.run(R, S, w.T, w.out_dtype)  # sizes the capacities
     log("first run stats", {k: v for k, v in res.stats.items() if k != "plan"})
-    fmt = "int32"
-    out = torch.empty((plan.out_capacity, 2), dtype=torch.int32, device="cuda")
-    for _ in range(args.warmup):
-        plan.enqueue(R, S, w.T, out, fmt)
-    torch.cuda.synchronize()
+    out = torch.empty((plan.out_capacity, 2), dtype=tdt, device="cuda")
 
     # ---- timed region: K joins back to back, no host synchronisation inside
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    ev0.record()
-    for _ in range(args.steps):
-        plan.enqueue(R, S, w.T, out, fmt)
-    ev1.record()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    dev_ms = ev0.elapsed_time(ev1) / args.steps
-    stats = __import__("tensorjoin_variants_amd._lib", fromlist=["x"]).decode_stats(plan._stats.cpu().numpy().view(np.uint64))
+    ms_per_step = timed(plan, R, S, w.T, out, fmt, args.steps, args.warmup)
+    stats = _lib.decode_stats(plan._stats.cpu().numpy().view(np.uint64))
     assert stats["out"] <= plan.out_capacity and stats["candidates"] <= plan.amb_capacity
     assert stats["violations"] == 0
-    ms_per_step = wall * 1e3 / args.steps
     value = pairs_total / (ms_per_step / 1e3)
 
     # ---- stage breakdown (separate instrumented loop)
@@ -118,46 +150,57 @@ def main() -> int:
     for _ in range(max(3, min(args.steps, 5))):
         plan.enqueue(R, S, w.T, out, fmt)
         stages.append(plan.stage_times())
-    screen_ms = float(np.mean([s["screen"] for s in stages]))
-    flops = 2.0 * d * pairs_total
-    achieved = flops / (screen_ms / 1e3)
-    peak = PEAK[args.precision]
-    log("stages ms", {k: round(float(np.mean([s[k] for s in stages])), 4) for k in stages[0]})
+    st_ms = {k: float(np.mean([s[k] for s in stages])) for k in stages[0]}
+    log("stages ms", {k: round(v, 4) for k, v in st_ms.items()})
+    if grid:
+        # dominant work: the grid probe; algorithmic bytes = inputs read once + pairs written once
+        alg_bytes = (nR + (nS or 0)) * d * 4 + stats["out"] * 2 * (4 if fmt == "int32" else 8)
+        achieved = alg_bytes / (st_ms["screen"] / 1e3)
+        roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM / 1e9, "unit": "GB/s",
+                    "frac": achieved / HBM, "traffic": None, "kernel": "grid join (index build + probe + write)",
+                    "algorithmic_bytes_per_launch": alg_bytes}
+    else:
+        flops = 2.0 * d * pairs_total
+        achieved = flops / (st_ms["screen"] / 1e3)
+        peak = PEAK[prec]
+        roofline = {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
+                    "frac": achieved / peak, "traffic": None, "kernel": "screen_kernel",
+                    "algorithmic_flops_per_launch": flops}
 
     result = {
         "metric": METRIC, "value": value, "unit": "pair-evals/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": args.precision, "data": "synthetic",
-        "config": {"workload": "certified_d128", "nR": nR, "nS": nS, "d": d, "selectivity": 1e-4,
-                   "eps": w.eps, "T": w.T, "output": "sorted int32 (i, j)", "precision": args.precision},
-        "device_ms_per_step": dev_ms,
-        "stages_ms": {k: float(np.mean([s[k] for s in stages])) for k in stages[0]},
-        "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None,
-                     "kernel": "screen_kernel", "algorithmic_flops_per_launch": flops},
-        "pairs_out": stats["out"], "ambiguous": stats["ambiguous"],
+        "vs_baseline": None, "dtype": prec if not grid else "f64", "data": "synthetic",
+        "config": {"workload": args.workload, "nR": nR, "nS": nS if nS is not None else nR, "d": d,
+                   "self_join": w.S is None, "eps": w.eps, "T": w.T, "output": f"sorted {fmt} (i, j)",
+                   "method": method, "precision": prec if not grid else None, "scale": args.scale},
+        "stages_ms": st_ms, "roofline": roofline,
+        "pairs_out": stats["out"], "ambiguous": stats["ambiguous"], "candidates": stats["candidates"],
         "ambiguous_fraction": stats["ambiguous"] / pairs_total,
         "max_abs_d2_err_vs_fp64": stats["max_err"], "max_err_over_halfwidth": stats["max_ratio"],
         "certificate_violations": stats["violations"],
     }
+    if grid:
+        result["grid_candidate_evals"] = stats["grid_candidates"]
 
-    if not args.no_plain:
+    if not args.no_plain and not grid:
         log("plain FP64 GPU method (one run)")
         pplan = Plan(nR, nS, d, "plain_fp64", "none", out_capacity=plan.out_capacity, amb_capacity=0, timing=True)
-        pout = torch.empty((pplan.out_capacity, 2), dtype=torch.int32, device="cuda")
+        pout = torch.empty((pplan.out_capacity, 2), dtype=tdt, device="cuda")
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         pplan.enqueue(R, S, w.T, pout, fmt)
         torch.cuda.synchronize()
         pwall = time.perf_counter() - t0
-        pst = __import__("tensorjoin_variants_amd._lib", fromlist=["x"]).decode_stats(pplan._stats.cpu().numpy().view(np.uint64))
+        pst = _lib.decode_stats(pplan._stats.cpu().numpy().view(np.uint64))
         n1, n2 = stats["out"], pst["out"]
         same = n1 == n2 and bool(torch.equal(out[:n1], pout[:n2]))
-        diffs = 0 if same else -1
+        diffs = 0
         if not same:
             a = out[:n1].cpu().numpy().astype(np.int64)
             b = pout[:n2].cpu().numpy().astype(np.int64)
-            ka, kb = a[:, 0] * nS + a[:, 1], b[:, 0] * nS + b[:, 1]
+            nn = nS or nR
+            ka, kb = a[:, 0] * nn + a[:, 1], b[:, 0] * nn + b[:, 1]
             diffs = int(np.setdiff1d(ka, kb).size + np.setdiff1d(kb, ka).size)
         result["plain_gpu"] = {"value": pairs_total / pwall, "ms": pwall * 1e3, "pairs_out": n2,
                                "stages_ms": pplan.stage_times()}
