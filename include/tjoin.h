@@ -112,6 +112,47 @@ int tj_plan_stage_times(tj_plan* plan, float* ms, int n);
 int tj_join(tj_plan* plan, const float* R, const float* S, double T, void* workspace,
             void* pairs, int out_format, uint64_t* stats, void* stream);
 
+/* hipGraph form of tj_join.  tj_plan_capture (synchronous) records the whole
+ * tj_join sequence for these buffers into a graph owned by the plan;
+ * tj_plan_replay launches it on `stream`.  No memset nodes: every clear is
+ * a kernel.  The reference used CUDA Graphs only in a diagnostic
+ * (CEJ/src/run_g9_graph_diagnostic.py:223-232); here it is the launch path
+ * for callers that repeat a join on fixed buffers. */
+int tj_plan_capture(tj_plan* plan, const float* R, const float* S, double T, void* workspace,
+                    void* pairs, int out_format, uint64_t* stats);
+int tj_plan_replay(tj_plan* plan, void* stream);
+
+/* --------------------------------------------------------- batches of joins
+ * Many independent self-joins of uneven sizes and widths in one pipeline
+ * with one persistent screen launch over the panels of every join, largest
+ * first.  Replaces the host loop over 4,096-tile batches with a blocking
+ * count read after every stage (CEJ/src/run_g5_tensorjoin_public.py:
+ * 377-381,410-535; TG/src/g2_operator_r1.py:45-57).
+ * tj_batch_create (synchronous) fixes the joins' sizes n[t] (host array),
+ * widths d[t] (1..256) and screen precisions (TJ_PREC_FP16 or TJ_PREC_FP8
+ * per join) and uploads the work list.  tj_batch_join takes x, a device
+ * array of `joins` device pointers (join t: row-major FP32 [n[t] x d[t]]),
+ * and T, a device array of the squared thresholds; it writes the local
+ * (i, j), i < j, of every join as int32 rows into `pairs`, join after join,
+ * each join's rows sorted, and offsets[t] (device int64[joins + 1]) = first
+ * row of join t, offsets[joins] = total.  stats as tj_join, summed over the
+ * joins; capacities and retry as tj_plan. */
+typedef struct tj_batch tj_batch;
+
+int tj_batch_create(tj_batch** batch, int joins, const int64_t* n, const int* d,
+                    const int* precision, int64_t out_capacity, int64_t cand_capacity,
+                    unsigned flags);
+int tj_batch_destroy(tj_batch* batch);
+int tj_batch_workspace_bytes(const tj_batch* batch, size_t* bytes);
+int tj_batch_describe(const tj_batch* batch, char* buf, size_t len); /* host only */
+int tj_batch_stage_times(tj_batch* batch, float* ms, int n);         /* as tj_plan_stage_times */
+int tj_batch_join(tj_batch* batch, const float* const* x, const double* T, void* workspace,
+                  int32_t* pairs, int64_t* offsets, uint64_t* stats, void* stream);
+/* hipGraph form of tj_batch_join, as tj_plan_capture / tj_plan_replay. */
+int tj_batch_capture(tj_batch* batch, const float* const* x, const double* T, void* workspace,
+                     int32_t* pairs, int64_t* offsets, uint64_t* stats);
+int tj_batch_replay(tj_batch* batch, void* stream);
+
 /* ------------------------------------------------------- pipeline stages
  * The stages tj_join runs, exported for callers that compose their own
  * pipeline and for tests.  All counts are read from device memory so that

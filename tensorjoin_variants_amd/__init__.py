@@ -6,12 +6,15 @@ Public API (the reference's join call as a drop-in):
     Matrix(T).run('F16' | 'F8' | 'F32', x)       -> (sorted directed u64 ids incl. self, stats)
     join_reference_compat(x, eps)                -> (directed u64 ids, stats)
     Plan(...)                                    -> reusable plan + workspace
+    batched_join(sets, eps)                      -> one sorted pair array per set
+    BatchPlan(...)                               -> reusable batch plan (+ hipGraph capture)
 
 All computation runs in libtjoin.so (hand-written HIP for gfx950); the
 package raises if the library is missing.
 """
 from ._lib import TJError, load as load_library
+from .batch import BatchPlan, batched_join
 from .join import JoinResult, Matrix, Plan, canonical_u64, join, join_reference_compat
 
 __all__ = ["join", "Matrix", "Plan", "JoinResult", "canonical_u64", "join_reference_compat",
-           "TJError", "load_library"]
+           "batched_join", "BatchPlan", "TJError", "load_library"]

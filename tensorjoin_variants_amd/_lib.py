@@ -68,6 +68,17 @@ def load():
         "tj_unpack_pairs": (i32, [vp, vp, i64, i32, vp, i32, vp]),
         "tj_mfma_probe": (i32, [i32, vp, vp, vp, vp, vp, i32, i32, vp, vp]),
         "tj_canonical_u64": (i32, [vp, C.POINTER(sz), vp, vp, i64, i64, vp, vp]),
+        "tj_plan_capture": (i32, [vp, vp, vp, f64, vp, vp, i32, vp]),
+        "tj_plan_replay": (i32, [vp, vp]),
+        "tj_batch_create": (i32, [C.POINTER(vp), i32, C.POINTER(i64), C.POINTER(i32), C.POINTER(i32), i64, i64,
+                                  C.c_uint]),
+        "tj_batch_destroy": (i32, [vp]),
+        "tj_batch_workspace_bytes": (i32, [vp, C.POINTER(sz)]),
+        "tj_batch_describe": (i32, [vp, C.c_char_p, sz]),
+        "tj_batch_stage_times": (i32, [vp, C.POINTER(C.c_float), i32]),
+        "tj_batch_join": (i32, [vp, vp, vp, vp, vp, vp, vp, vp]),
+        "tj_batch_capture": (i32, [vp, vp, vp, vp, vp, vp, vp]),
+        "tj_batch_replay": (i32, [vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

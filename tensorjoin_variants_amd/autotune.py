@@ -72,3 +72,80 @@ class Autotuner:
         except OSError:
             pass
         return {"winner": win, "ms": times}
+
+
+def width_class(d: int) -> int:
+    """Padded width of the 16-bit batch screen (16, 32, 64, 128, 256)."""
+    for w in (16, 32, 64, 128, 256):
+        if d <= w:
+            return w
+    raise ValueError(f"d={d} exceeds the batch screen's widest instantiation (256)")
+
+
+class BatchTuner:
+    """Per-width screen precision for batches of joins.
+
+    ``tune`` times each batch precision (FP16, FP8) end to end on a sample of
+    the batch's own joins -- the largest few of every width class, where the
+    pair work is -- and keeps the faster per width class (cached in the same
+    JSON file as ``Autotuner``).  ``choose`` maps a batch to per-join
+    precisions from that table (FP16 where a class was never tuned).  The
+    choice never changes the output, only the time.
+    """
+
+    CANDIDATES = ("fp16", "fp8")
+
+    def __init__(self, cache: Path = CACHE):
+        self.cache = cache
+        try:
+            self.table = json.loads(cache.read_text())
+        except Exception:
+            self.table = {}
+
+    @staticmethod
+    def key(w: int) -> str:
+        return f"batch_w{w}"
+
+    def choose(self, ds, ns=None) -> list[str]:
+        return [self.table.get(self.key(width_class(d)), "fp16") for d in ds]
+
+    def tune(self, sets, Ts, per_class: int = 8, reps: int = 2) -> dict:
+        import numpy as np
+        from .batch import BatchPlan, initial_capacities
+        from .join import torch
+        t = torch()
+        Ts = np.broadcast_to(np.asarray(Ts, dtype=np.float64), (len(sets),))
+        classes: dict[int, list[int]] = {}
+        for k, x in enumerate(sets):
+            classes.setdefault(width_class(x.shape[1]), []).append(k)
+        report = {}
+        for w, idx in sorted(classes.items()):
+            idx = sorted(idx, key=lambda k: -sets[k].shape[0])[:per_class]
+            sub = [sets[k] for k in idx]
+            ns = [x.shape[0] for x in sub]
+            ds = [x.shape[1] for x in sub]
+            times = {}
+            for prec in self.CANDIDATES:
+                oc, cc = initial_capacities(ns)
+                plan = BatchPlan(ns, ds, prec, oc, cc)
+                plan.run(sub, Ts[idx])  # warm + size the capacities
+                xs, ptrs, Td = plan.inputs(sub, Ts[idx])
+                best = float("inf")
+                for _ in range(reps):
+                    a, b = t.cuda.Event(enable_timing=True), t.cuda.Event(enable_timing=True)
+                    a.record()
+                    plan.enqueue(ptrs, Td)
+                    b.record()
+                    b.synchronize()
+                    best = min(best, a.elapsed_time(b))
+                times[prec] = best
+                plan.close()
+            win = min(times, key=times.get)
+            self.table[self.key(w)] = win
+            report[w] = {"winner": win, "ms": times, "joins": len(idx)}
+        try:
+            self.cache.parent.mkdir(parents=True, exist_ok=True)
+            self.cache.write_text(json.dumps(self.table, indent=1, sort_keys=True))
+        except OSError:
+            pass
+        return report

@@ -81,6 +81,32 @@ int screen_max_grid(int prec, int dp, bool self_join) {
     if (occ < 1) occ = 1;
     return occ * num_cus();
 }
+
+// Record what fn enqueues into an instantiated graph.  The capture runs on a
+// private stream (default priority) so that the caller's stream, which may
+// be the legacy default stream, is never put in capture mode.
+int capture_graph(hipGraphExec_t* exec, int (*fn)(void*, hipStream_t), void* ctx) {
+    hipStream_t cs;
+    TJ_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        (void)hipStreamDestroy(cs);
+        return hip_fail(e, "hipStreamBeginCapture");
+    }
+    const int rc = fn(ctx, cs);
+    hipGraph_t graph = nullptr;
+    e = hipStreamEndCapture(cs, &graph);
+    (void)hipStreamDestroy(cs);
+    if (rc != TJ_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    e = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+    return TJ_OK;
+}
 }  // namespace tj
 
 using namespace tj;
@@ -99,6 +125,8 @@ struct tj_plan {
         off_aux, off_sort, total;
     hipEvent_t ev[5];
     bool timing;
+    hipGraphExec_t graph;
+    bool has_graph;
 };
 
 static size_t carve(size_t& cur, size_t bytes) {
@@ -223,11 +251,13 @@ int tj_plan_create(tj_plan** out, int64_t nR, int64_t nS, int d, int method, int
         }
         p->timing = true;
     }
+    p->has_graph = false;
     *out = p;
     return TJ_OK;
 }
 
 int tj_plan_destroy(tj_plan* plan) {
+    if (plan && plan->has_graph) (void)hipGraphExecDestroy(plan->graph);
     if (plan && plan->timing)
         for (int k = 0; k < 5; ++k) (void)hipEventDestroy(plan->ev[k]);
     delete plan;
@@ -259,8 +289,8 @@ int tj_plan_stage_times(tj_plan* p, float* ms, int n) {
     return TJ_OK;
 }
 
-int tj_join(tj_plan* p, const float* R, const float* S, double T, void* workspace, void* pairs, int out_format,
-            uint64_t* stats, void* stream) {
+static int join_check(const tj_plan* p, const float* R, const float* S, double T, void* workspace, void* pairs,
+                      int out_format, uint64_t* stats) {
     TJ_CHECK(p != nullptr, TJ_ERR_INVALID, "plan is NULL");
     TJ_CHECK(workspace != nullptr && stats != nullptr, TJ_ERR_INVALID, "workspace and stats are required");
     TJ_CHECK(R != nullptr || p->nR == 0, TJ_ERR_INVALID, "R is NULL");
@@ -270,8 +300,30 @@ int tj_join(tj_plan* p, const float* R, const float* S, double T, void* workspac
     TJ_CHECK(out_format == TJ_OUT_I64 || (p->nR < (1ll << 31) && p->nS < (1ll << 31)), TJ_ERR_INVALID,
              "int32 output needs fewer than 2^31 rows");
     TJ_CHECK(T == T && T >= 0.0 && T < INFINITY, TJ_ERR_INVALID, "T must be finite and >= 0");
-    hipStream_t s = (hipStream_t)stream;
-    uint8_t* ws = (uint8_t*)workspace;
+    return TJ_OK;
+}
+
+struct JoinCall {
+    tj_plan* p;
+    const float* R;
+    const float* S;
+    double T;
+    void* workspace;
+    void* pairs;
+    int out_format;
+    uint64_t* stats;
+    bool marks;
+};
+
+static int join_enqueue(const JoinCall& c, hipStream_t s) {
+    tj_plan* p = c.p;
+    const float* R = c.R;
+    const float* S = c.S;
+    const double T = c.T;
+    void* pairs = c.pairs;
+    const int out_format = c.out_format;
+    uint64_t* stats = c.stats;
+    uint8_t* ws = (uint8_t*)c.workspace;
     if (p->self_join) S = R;
     uint32_t* maxabs = (uint32_t*)(ws + p->off_maxabs);
     JoinParams* params = (JoinParams*)(ws + p->off_params);
@@ -282,7 +334,7 @@ int tj_join(tj_plan* p, const float* R, const float* S, double T, void* workspac
     const int64_t pairs_total = p->self_join ? p->nR * (p->nR - 1) / 2 : p->nR * p->nS;
     if ((rc = launch_fill_u64(stats + TJ_ST_PAIRS, 1, (uint64_t)pairs_total, s))) return rc;
 #define TJ_MARK(k) \
-    if (p->timing) TJ_HIP(hipEventRecord(p->ev[k], s))
+    if (c.marks && p->timing) TJ_HIP(hipEventRecord(p->ev[k], s))
     TJ_MARK(0);
 
     if (p->method == TJ_METHOD_CERTIFIED) {
@@ -339,6 +391,36 @@ int tj_join(tj_plan* p, const float* R, const float* S, double T, void* workspac
     if ((rc = launch_unpack(out_keys, stats + TJ_ST_OUT, p->out_cap, p->jbits, pairs, out_format, s))) return rc;
     TJ_MARK(4);
 #undef TJ_MARK
+    return TJ_OK;
+}
+
+int tj_join(tj_plan* p, const float* R, const float* S, double T, void* workspace, void* pairs, int out_format,
+            uint64_t* stats, void* stream) {
+    int rc = join_check(p, R, S, T, workspace, pairs, out_format, stats);
+    if (rc) return rc;
+    const JoinCall c{p, R, S, T, workspace, pairs, out_format, stats, true};
+    return join_enqueue(c, (hipStream_t)stream);
+}
+
+static int join_capture_fn(void* ctx, hipStream_t s) { return join_enqueue(*(const JoinCall*)ctx, s); }
+
+int tj_plan_capture(tj_plan* p, const float* R, const float* S, double T, void* workspace, void* pairs,
+                    int out_format, uint64_t* stats) {
+    int rc = join_check(p, R, S, T, workspace, pairs, out_format, stats);
+    if (rc) return rc;
+    JoinCall c{p, R, S, T, workspace, pairs, out_format, stats, false};
+    hipGraphExec_t exec;
+    if ((rc = capture_graph(&exec, join_capture_fn, &c))) return rc;
+    if (p->has_graph) (void)hipGraphExecDestroy(p->graph);
+    p->graph = exec;
+    p->has_graph = true;
+    return TJ_OK;
+}
+
+int tj_plan_replay(tj_plan* p, void* stream) {
+    TJ_CHECK(p != nullptr, TJ_ERR_INVALID, "plan is NULL");
+    TJ_CHECK(p->has_graph, TJ_ERR_INVALID, "no graph captured for this plan");
+    TJ_HIP(hipGraphLaunch(p->graph, (hipStream_t)stream));
     return TJ_OK;
 }
 

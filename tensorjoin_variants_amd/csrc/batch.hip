@@ -1,0 +1,653 @@
+// batch.hip -- many independent self-joins of uneven sizes and widths in one
+// pipeline: one scale scan, one parameter pass, one row-metadata pass, one
+// persistent screen launch over every join's panels (one per precision
+// family present: FP16, FP8), one certificate pass, one radix sort and one
+// unpack with per-join offsets.
+//
+// The reference runs one join at a time and slices its triangular tile list
+// into host batches of 4,096 tiles with a blocking count read after every
+// stage (CEJ/src/run_g5_tensorjoin_public.py:377-381,410-535;
+// TG/src/g2_operator_r1.py:45-57).  Here the unit of work is a panel of one
+// join; the work list of all joins is ordered largest first (LPT) at plan
+// creation and drained by a persistent grid through an atomic cursor, so
+// joins of 64 rows and of 16,384 rows, of width 16 and of width 256, share
+// the machine without a host round trip.  Every count stays on the device:
+// the whole sequence is capturable in a hipGraph (tj_batch_capture).
+//
+// Row spaces.  Joins are numbered in caller order; their rows are numbered
+// consecutively in that order (the batch row space, gbase = first row of a
+// join).  Joins of one (precision, padded width) form a group whose quantized
+// rows and metadata are stored contiguously (grow = first row of a join in
+// its group).  A pair key is (batch row of i) << jbits | (local j), so one
+// sort over all keys orders the output by join, then (i, j).
+#include <stdio.h>
+#include <string.h>
+#include <algorithm>
+#include <new>
+#include <vector>
+#include "certify_impl.h"
+#include "meta_impl.h"
+
+namespace tjbatch {
+
+struct BatchJoin {
+    int64_t gbase;  // first row in the batch row space
+    int64_t grow;   // first row in the group's arrays
+    int32_t n, d, dp, prec;
+    int32_t group, bm;
+};
+struct BatchGroup {
+    int64_t q_off, meta_off;  // workspace offsets of the quantized rows and metadata
+    int32_t combo, prec, dp, pad;
+};
+
+// (precision, padded width) instantiations of the batched screen body, in
+// two families: a kernel holding FP16 and FP8 bodies together exceeds 256
+// VGPRs and spills, each family alone does not.
+#define TJ_BATCH_FP16(X) X(0, P_FP16, 16) X(1, P_FP16, 32) X(2, P_FP16, 64) X(3, P_FP16, 128) X(4, P_FP16, 256)
+#define TJ_BATCH_FP8(X) X(5, P_FP8, 64) X(6, P_FP8, 128) X(7, P_FP8, 256)
+#define TJ_BATCH_COMBOS(X) TJ_BATCH_FP16(X) TJ_BATCH_FP8(X)
+constexpr int N_FAMILIES = 2;
+inline int family_of(int prec) { return prec == P_FP8 ? 1 : 0; }
+
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
+#define TJ_X_RING(ID, PR, DPV) cmax(tjscreen::ring_bytes<PR, DPV>(),
+#define TJ_X_CLOSE(ID, PR, DPV) )
+constexpr int RING_MAX = TJ_BATCH_COMBOS(TJ_X_RING) 0 TJ_BATCH_COMBOS(TJ_X_CLOSE);
+#define TJ_X_NST(ID, PR, DPV) cmax(tjscreen::Geo<PR, DPV>::NSTAGE,
+constexpr int NSTAGE_MAX = TJ_BATCH_COMBOS(TJ_X_NST) 0 TJ_BATCH_COMBOS(TJ_X_CLOSE);
+#undef TJ_X_RING
+#undef TJ_X_NST
+#undef TJ_X_CLOSE
+
+int combo_of(int prec, int dp) {
+#define X(ID, PR, DPV) \
+    if (prec == PR && dp == DPV) return ID;
+    TJ_BATCH_COMBOS(X)
+#undef X
+    return -1;
+}
+int rows_per_wg(int combo) {
+    switch (combo) {
+#define X(ID, PR, DPV) \
+    case ID: return 4 * tjscreen::rb_for<PR, DPV>() * 32;
+        TJ_BATCH_COMBOS(X)
+#undef X
+        default: return 0;
+    }
+}
+
+// ------------------------------------------------------------ scale scan
+// max |x| per join (binary32 bits) and the count of non-finite coordinates.
+// One wave takes 8 consecutive batch rows; one atomic per join run.
+__global__ void __launch_bounds__(256) batch_scan_kernel(const float* const* __restrict__ xs,
+                                                         const BatchJoin* __restrict__ joins,
+                                                         const int32_t* __restrict__ row_join, int64_t rows,
+                                                         uint32_t* __restrict__ maxabs, uint64_t* __restrict__ stats) {
+    const int lane = threadIdx.x & 63;
+    const int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    uint32_t bad = 0;
+    auto put = [&](int t, uint32_t m) {
+        for (int o = 32; o > 0; o >>= 1) {
+            const uint32_t om = __shfl_xor(m, o);
+            m = om > m ? om : m;
+        }
+        if (lane == 0 && m) atomicMax(&maxabs[t], m);
+    };
+    for (int64_t c = w * 8; c < rows; c += nw * 8) {
+        int cur = -1;
+        uint32_t m = 0;
+        for (int64_t r = c; r < c + 8 && r < rows; ++r) {
+            const int t = row_join[r];
+            if (t != cur) {
+                if (cur >= 0) put(cur, m);
+                cur = t;
+                m = 0;
+            }
+            const int d = joins[t].d;
+            const float* xr = xs[t] + (r - joins[t].gbase) * (int64_t)d;
+            for (int k = lane; k < d; k += 64) {
+                const uint32_t u = __builtin_bit_cast(uint32_t, xr[k]) & 0x7FFFFFFFu;
+                if (u >= 0x7F800000u) bad++;
+                else m = u > m ? u : m;
+            }
+        }
+        put(cur, m);
+    }
+    for (int o = 32; o > 0; o >>= 1) bad += __shfl_xor(bad, o);
+    if (lane == 0 && bad) atomicAdd((unsigned long long*)&stats[TJ_ST_INVALID], (unsigned long long)bad);
+}
+
+// ------------------------------------------------------------- parameters
+// One thread per join.  A threshold that is not finite and >= 0 counts as
+// an invalid input (the host layer rejects such calls before they start).
+__global__ void batch_params_kernel(const BatchJoin* __restrict__ joins, int nj, const double* __restrict__ T,
+                                    const uint32_t* __restrict__ maxabs, JoinParams* __restrict__ params,
+                                    uint64_t* __restrict__ stats) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nj) return;
+    const BatchJoin J = joins[t];
+    const double Tt = T[t];
+    if (!(Tt >= 0.0 && Tt < INFINITY)) atomicAdd((unsigned long long*)&stats[TJ_ST_INVALID], 1ull);
+    params[t] = tjmeta::make_params(maxabs[t], Tt, J.d, J.dp, J.prec);
+}
+
+// ----------------------------------------------------------- row metadata
+// One wave per batch row, written into its group's arrays.
+__global__ void __launch_bounds__(256) batch_rowmeta_kernel(const float* const* __restrict__ xs,
+                                                            const BatchJoin* __restrict__ joins,
+                                                            const BatchGroup* __restrict__ groups,
+                                                            const int32_t* __restrict__ row_join, int64_t rows,
+                                                            const JoinParams* __restrict__ params, uint8_t* ws) {
+    const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= rows) return;
+    const int lane = threadIdx.x & 63;
+    const int t = row_join[r];
+    const BatchJoin J = joins[t];
+    const BatchGroup G = groups[J.group];
+    const JoinParams P = params[t];
+    const int64_t local = r - J.gbase;
+    const float* xr = xs[t] + local * (int64_t)J.d;
+    uint8_t* q = ws + G.q_off;
+    RowMeta* meta = (RowMeta*)(ws + G.meta_off);
+    if (J.prec == P_FP8) tjmeta::rowmeta_row<P_FP8>(xr, J.grow + local, J.d, J.dp, P, lane, q, q, meta);
+    else tjmeta::rowmeta_row<P_FP16>(xr, J.grow + local, J.d, J.dp, P, lane, q, q, meta);
+}
+
+// ------------------------------------------------------------------ screen
+struct ScreenBatchArgs {
+    const int2* items;  // (join, panel), largest first
+    int64_t n_items;
+    const BatchJoin* joins;
+    const BatchGroup* groups;
+    const uint8_t* ws;
+    tjscreen::CandOut co;
+    uint64_t* cursor;
+    int jbits;
+};
+
+// One persistent launch over the panels of every join of one precision
+// family.  The body of an item is the single-join screen's panel sweep,
+// instantiated per (precision, width) and selected by the item's group.
+template <int FAM>
+__global__ void __launch_bounds__(256, 2) batch_screen_kernel(const ScreenBatchArgs A) {
+    using namespace tjscreen;
+    __shared__ __attribute__((aligned(16))) uint8_t sB[RING_MAX];
+    __shared__ __attribute__((aligned(16))) float sNp[NSTAGE_MAX * 64];
+    __shared__ __attribute__((aligned(16))) uint32_t sSc[NSTAGE_MAX * 64];
+    __shared__ float sP[4][16 * 64];
+    __shared__ uint64_t sCK[4][CCAP];
+    __shared__ float sCP[4][CCAP];
+    __shared__ int64_t sItem;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    int nC = 0;
+    for (;;) {
+        if (tid == 0) sItem = (int64_t)atomicAdd((unsigned long long*)A.cursor, 1ull);
+        __syncthreads();
+        const int64_t item = sItem;
+        __syncthreads();
+        if (item >= A.n_items) break;
+        const int2 it = A.items[item];
+        const BatchJoin J = A.joins[it.x];
+        const BatchGroup G = A.groups[J.group];
+        PanelIO io;
+        const uint8_t* q = A.ws + G.q_off;
+        const RowMeta* meta = (const RowMeta*)(A.ws + G.meta_off);
+        io.qa[0] = io.qa[1] = io.qb[0] = io.qb[1] = q;
+        io.ma = io.mb = meta;
+        io.na = io.nb = J.grow + J.n;
+        io.koff_a = J.gbase - J.grow;  // i -> batch row
+        io.koff_b = -J.grow;           // j -> local row
+        io.jbits = A.jbits;
+        const int64_t r0 = J.grow + (int64_t)it.y * J.bm;
+        const int64_t ce = J.grow + J.n;
+#define X(ID, PR, DPV)                                                                                          \
+    case ID:                                                                                                    \
+        screen_panel<PR, DPV, rb_for<PR, DPV>(), true>(io, A.co, r0, r0, ce, sB, sNp, sSc, sP[wave], sCK[wave], \
+                                                       sCP[wave], nC);                                          \
+        break;
+        if constexpr (FAM == 0) {
+            switch (uni(G.combo)) {
+                TJ_BATCH_FP16(X)
+                default: break;
+            }
+        } else {
+            switch (uni(G.combo)) {
+                TJ_BATCH_FP8(X)
+                default: break;
+            }
+        }
+#undef X
+    }
+    if (nC) cand_flush(A.co, sCK[wave], sCP[wave], nC, lane);
+}
+
+// ----------------------------------------------------------- certificate
+struct BatchSrc {
+    const float* const* xs;
+    const BatchJoin* joins;
+    const BatchGroup* groups;
+    const int32_t* row_join;
+    const JoinParams* params;
+    const uint8_t* ws;
+    int jbits;
+    __device__ inline void fetch(uint64_t key, tjcert::PairRef& r) const {
+        const int64_t ig = (int64_t)(key >> jbits);
+        const int64_t jl = (int64_t)(key & ((1ull << jbits) - 1));
+        const int t = row_join[ig];
+        const BatchJoin J = joins[t];
+        const int64_t il = ig - J.gbase;
+        const RowMeta* meta = (const RowMeta*)(ws + groups[J.group].meta_off);
+        r.P = params + t;
+        r.mi = meta[J.grow + il];
+        r.mj = meta[J.grow + jl];
+        r.xi = xs[t] + il * (int64_t)J.d;
+        r.xj = xs[t] + jl * (int64_t)J.d;
+        r.d = J.d;
+    }
+};
+
+// ------------------------------------------------------------------ unpack
+// Sorted keys -> local int32 (i, j) rows, and offsets[t] = first output row
+// of join t (offsets[joins] = total), from the join boundaries in the
+// sorted order.
+__global__ void batch_unpack_kernel(const uint64_t* __restrict__ keys, const uint64_t* __restrict__ count, int64_t cap,
+                                    int jbits, const int32_t* __restrict__ row_join,
+                                    const BatchJoin* __restrict__ joins, int nj, int32_t* __restrict__ pairs,
+                                    int64_t* __restrict__ offsets) {
+    int64_t cnt = (int64_t)*count;
+    if (cnt > cap) cnt = cap;
+    const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    if (cnt == 0) {
+        for (int64_t u = gt; u <= nj; u += stride) offsets[u] = 0;
+        return;
+    }
+    const uint64_t jmask = (1ull << jbits) - 1;
+    for (int64_t k = gt; k < cnt; k += stride) {
+        const uint64_t key = keys[k];
+        const int64_t ig = (int64_t)(key >> jbits);
+        const int t = row_join[ig];
+        pairs[2 * k] = (int32_t)(ig - joins[t].gbase);
+        pairs[2 * k + 1] = (int32_t)(key & jmask);
+        const int tp = k > 0 ? row_join[keys[k - 1] >> jbits] : -1;
+        for (int u = tp + 1; u <= t; ++u) offsets[u] = k;
+        if (k == cnt - 1)
+            for (int u = t + 1; u <= nj; ++u) offsets[u] = cnt;
+    }
+}
+
+}  // namespace tjbatch
+
+using namespace tj;
+using namespace tjbatch;
+
+namespace tj {
+int screen_batch_occupancy(int fam) {
+    int blocks = 0;
+    const hipError_t e = fam == 0 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, batch_screen_kernel<0>, 256, 0)
+                                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, batch_screen_kernel<1>, 256, 0);
+    if (e != hipSuccess) return 1;
+    return blocks > 0 ? blocks : 1;
+}
+}  // namespace tj
+
+struct tj_batch {
+    int nj;
+    int64_t rows, max_n, n_items, pairs_total;
+    int ibits, jbits, keybits;
+    int64_t out_cap, cand_cap;
+    unsigned flags;
+    int n_groups;
+    int64_t fam_begin[N_FAMILIES + 1];  // items of family f: [fam_begin[f], fam_begin[f + 1])
+    int fam_grid[N_FAMILIES];
+    // device tables (owned)
+    void* dev;
+    BatchJoin* d_joins;
+    BatchGroup* d_groups;
+    int2* d_items;
+    int32_t* d_row_join;
+    // workspace layout
+    size_t off_maxabs, off_cursor, off_params, off_cand, off_aux, off_out, off_sort, total;
+    hipEvent_t ev[5];
+    bool timing;
+    hipGraphExec_t graph;
+    bool has_graph;
+};
+
+static size_t bcarve(size_t& cur, size_t bytes) {
+    size_t at = (cur + 255) / 256 * 256;
+    cur = at + bytes;
+    return at;
+}
+
+namespace tj {
+int capture_graph(hipGraphExec_t* exec, int (*fn)(void*, hipStream_t), void* ctx);
+}
+
+struct BatchCall {
+    tj_batch* b;
+    const float* const* x;
+    const double* T;
+    void* ws;
+    int32_t* pairs;
+    int64_t* offsets;
+    uint64_t* stats;
+    bool marks;
+};
+
+static int batch_enqueue(const BatchCall& c, hipStream_t s) {
+    tj_batch* b = c.b;
+    uint8_t* ws = (uint8_t*)c.ws;
+    uint32_t* maxabs = (uint32_t*)(ws + b->off_maxabs);
+    uint64_t* cursors = (uint64_t*)(ws + b->off_cursor);
+    JoinParams* params = (JoinParams*)(ws + b->off_params);
+    uint64_t* cand = (uint64_t*)(ws + b->off_cand);
+    float* aux = (float*)(ws + b->off_aux);
+    uint64_t* out_keys = (uint64_t*)(ws + b->off_out);
+    uint64_t* stats = c.stats;
+    int rc;
+#define TJ_MARK(k) \
+    if (c.marks && b->timing) TJ_HIP(hipEventRecord(b->ev[k], s))
+    if ((rc = launch_fill_u64(stats, TJ_STATS_SLOTS, 0, s))) return rc;
+    if ((rc = launch_fill_u64(stats + TJ_ST_PAIRS, 1, (uint64_t)b->pairs_total, s))) return rc;
+    if ((rc = launch_fill_u64((uint64_t*)maxabs, (b->nj + 1) / 2, 0, s))) return rc;
+    if ((rc = launch_fill_u64(cursors, N_FAMILIES, 0, s))) return rc;
+    TJ_MARK(0);
+    if (b->rows > 0) {
+        int64_t blocks = (b->rows + 31) / 32;  // 8 rows per wave, 4 waves per block
+        if (blocks > 4096) blocks = 4096;
+        hipLaunchKernelGGL(batch_scan_kernel, dim3((unsigned)blocks), dim3(256), 0, s, c.x, b->d_joins,
+                           b->d_row_join, b->rows, maxabs, stats);
+        TJ_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(batch_params_kernel, dim3((unsigned)((b->nj + 255) / 256)), dim3(256), 0, s, b->d_joins, b->nj,
+                       c.T, maxabs, params, stats);
+    TJ_HIP(hipGetLastError());
+    if (b->rows > 0) {
+        hipLaunchKernelGGL(batch_rowmeta_kernel, dim3((unsigned)((b->rows + 3) / 4)), dim3(256), 0, s, c.x,
+                           b->d_joins, b->d_groups, b->d_row_join, b->rows, params, ws);
+        TJ_HIP(hipGetLastError());
+    }
+    TJ_MARK(1);
+    for (int f = 0; f < N_FAMILIES; ++f) {
+        const int64_t n_items = b->fam_begin[f + 1] - b->fam_begin[f];
+        if (n_items == 0) continue;
+        ScreenBatchArgs a{};
+        a.items = b->d_items + b->fam_begin[f];
+        a.n_items = n_items;
+        a.joins = b->d_joins;
+        a.groups = b->d_groups;
+        a.ws = ws;
+        a.co = tjscreen::CandOut{cand, aux, b->cand_cap, &stats[TJ_ST_CANDIDATES]};
+        a.cursor = cursors + f;
+        a.jbits = b->jbits;
+        if (f == 0) hipLaunchKernelGGL(batch_screen_kernel<0>, dim3((unsigned)b->fam_grid[f]), dim3(256), 0, s, a);
+        else hipLaunchKernelGGL(batch_screen_kernel<1>, dim3((unsigned)b->fam_grid[f]), dim3(256), 0, s, a);
+        TJ_HIP(hipGetLastError());
+    }
+    TJ_MARK(2);
+    if (b->cand_cap > 0 && b->n_items > 0) {
+        int64_t blocks = (b->cand_cap + 255) / 256;
+        const int64_t maxb = (int64_t)num_cus() * 8;
+        if (blocks > maxb) blocks = maxb;
+        const BatchSrc src{c.x, b->d_joins, b->d_groups, b->d_row_join, params, ws, b->jbits};
+        hipLaunchKernelGGL(tjcert::certify_kernel<BatchSrc>, dim3((unsigned)blocks), dim3(256), 0, s, src, cand, aux,
+                           b->cand_cap, out_keys, b->out_cap, stats, (b->flags & TJ_FLAG_AUDIT) ? 1 : 0);
+        TJ_HIP(hipGetLastError());
+    }
+    TJ_MARK(3);
+    if ((rc = launch_sort(ws + b->off_sort, out_keys, stats + TJ_ST_OUT, b->out_cap, b->keybits, s))) return rc;
+    {
+        int64_t work = std::max<int64_t>(b->out_cap, b->nj + 1);
+        int64_t blocks = (work + 255) / 256;
+        if (blocks > 2048) blocks = 2048;
+        hipLaunchKernelGGL(batch_unpack_kernel, dim3((unsigned)blocks), dim3(256), 0, s, out_keys, stats + TJ_ST_OUT,
+                           b->out_cap, b->jbits, b->d_row_join, b->d_joins, b->nj, c.pairs, c.offsets);
+        TJ_HIP(hipGetLastError());
+    }
+    TJ_MARK(4);
+#undef TJ_MARK
+    return TJ_OK;
+}
+
+extern "C" {
+
+int tj_batch_create(tj_batch** out, int joins, const int64_t* n, const int* d, const int* precision,
+                    int64_t out_capacity, int64_t cand_capacity, unsigned flags) {
+    TJ_CHECK(out != nullptr, TJ_ERR_INVALID, "batch pointer is NULL");
+    *out = nullptr;
+    TJ_CHECK(joins >= 1, TJ_ERR_INVALID, "a batch needs at least one join");
+    TJ_CHECK(n && d && precision, TJ_ERR_INVALID, "NULL shape arrays");
+    TJ_CHECK(out_capacity >= 0 && out_capacity < (1ll << 32), TJ_ERR_INVALID, "out_capacity must be in [0, 2^32)");
+    TJ_CHECK(cand_capacity >= 0 && cand_capacity < (1ll << 32), TJ_ERR_INVALID, "cand_capacity must be in [0, 2^32)");
+    std::vector<BatchJoin> J(joins);
+    int64_t rows = 0, max_n = 0, pairs = 0;
+    for (int t = 0; t < joins; ++t) {
+        TJ_CHECK(n[t] >= 0 && n[t] < (1ll << 31), TJ_ERR_INVALID, "join %d: bad size %lld", t, (long long)n[t]);
+        TJ_CHECK(d[t] >= 1, TJ_ERR_INVALID, "join %d: bad width %d", t, d[t]);
+        TJ_CHECK(precision[t] == P_FP16 || precision[t] == P_FP8, TJ_ERR_UNSUPPORTED,
+                 "join %d: the batch screen serves TJ_PREC_FP16 and TJ_PREC_FP8 (got %d)", t, precision[t]);
+        const int dp = padded_dim(d[t], precision[t]);
+        const int combo = dp > 0 ? combo_of(precision[t], dp) : -1;
+        TJ_CHECK(combo >= 0, TJ_ERR_UNSUPPORTED, "join %d: the batch screen serves d <= 256 (d=%d)", t, d[t]);
+        J[t].gbase = rows;
+        J[t].n = (int32_t)n[t];
+        J[t].d = d[t];
+        J[t].dp = dp;
+        J[t].prec = precision[t];
+        J[t].bm = rows_per_wg(combo);
+        J[t].group = combo;  // remapped to a dense group index below
+        rows += n[t];
+        max_n = std::max<int64_t>(max_n, n[t]);
+        pairs += n[t] * (n[t] - 1) / 2;
+    }
+    // groups: one per (precision, width) present, in combo order
+    int gid[16];
+    std::vector<BatchGroup> G;
+    for (int c = 0; c < 16; ++c) gid[c] = -1;
+    std::vector<int64_t> grows;
+    for (int t = 0; t < joins; ++t) {
+        const int c = J[t].group;
+        if (gid[c] < 0) {
+            gid[c] = (int)G.size();
+            BatchGroup g{};
+            g.combo = c;
+            g.prec = J[t].prec;
+            g.dp = J[t].dp;
+            G.push_back(g);
+            grows.push_back(0);
+        }
+        const int g = gid[c];
+        J[t].group = g;
+        J[t].grow = grows[g];
+        grows[g] += J[t].n;
+    }
+    tj_batch* b = new (std::nothrow) tj_batch();
+    TJ_CHECK(b != nullptr, TJ_ERR_INVALID, "out of host memory");
+    b->nj = joins;
+    b->rows = rows;
+    b->max_n = max_n;
+    b->pairs_total = pairs;
+    b->out_cap = out_capacity;
+    b->cand_cap = cand_capacity;
+    b->flags = flags;
+    b->n_groups = (int)G.size();
+    b->ibits = bits_for(rows > 1 ? rows : 2);
+    b->jbits = bits_for(max_n > 1 ? max_n : 2);
+    b->keybits = b->ibits + b->jbits;
+    if (b->keybits > 63) {
+        delete b;
+        set_error("pair keys need more than 63 bits");
+        return TJ_ERR_UNSUPPORTED;
+    }
+    // workspace layout
+    size_t cur = 0;
+    b->off_maxabs = bcarve(cur, 4 * (size_t)joins + 8);
+    b->off_cursor = bcarve(cur, 8 * N_FAMILIES);
+    b->off_params = bcarve(cur, sizeof(JoinParams) * (size_t)joins);
+    for (size_t g = 0; g < G.size(); ++g) {
+        const int es = prec_esize(G[g].prec);
+        G[g].q_off = (int64_t)bcarve(cur, (size_t)grows[g] * G[g].dp * es + 16);
+        G[g].meta_off = (int64_t)bcarve(cur, sizeof(RowMeta) * (size_t)grows[g] + 16);
+    }
+    b->off_cand = bcarve(cur, 8 * (size_t)std::max<int64_t>(cand_capacity, 1));
+    b->off_aux = bcarve(cur, 4 * (size_t)std::max<int64_t>(cand_capacity, 1));
+    b->off_out = bcarve(cur, 8 * (size_t)std::max<int64_t>(out_capacity, 1));
+    b->off_sort = bcarve(cur, sort_temp_bytes(out_capacity));
+    b->total = (cur + 255) / 256 * 256;
+
+    // work list: every panel of every join with at least two rows, largest
+    // first (estimated MFMA steps plus a per-block overhead), ties in
+    // (join, panel) order
+    struct Item { int fam; double cost; int t, p; };
+    std::vector<Item> items;
+    for (int t = 0; t < joins; ++t) {
+        if (J[t].n < 2) continue;
+        const int bm = J[t].bm;
+        const int ks = J[t].dp / prec_kstep(J[t].prec);
+        const int rb = bm / 128;
+        const int panels = (J[t].n + bm - 1) / bm;
+        for (int p = 0; p < panels; ++p) {
+            const int64_t cols = (int64_t)J[t].n - (int64_t)p * bm;
+            const double nblk = (double)((cols + 31) / 32);
+            items.push_back({family_of(J[t].prec), nblk * (16.0 * ks * rb + 48.0), t, p});
+        }
+    }
+    std::stable_sort(items.begin(), items.end(), [](const Item& a, const Item& c) {
+        return a.fam != c.fam ? a.fam < c.fam : a.cost > c.cost;
+    });
+    b->n_items = (int64_t)items.size();
+    for (int f = 0; f <= N_FAMILIES; ++f) b->fam_begin[f] = 0;
+    for (const Item& x : items) b->fam_begin[x.fam + 1]++;
+    for (int f = 0; f < N_FAMILIES; ++f) b->fam_begin[f + 1] += b->fam_begin[f];
+    std::vector<int2> it(items.size() > 0 ? items.size() : 1);
+    for (size_t k = 0; k < items.size(); ++k) it[k] = make_int2(items[k].t, items[k].p);
+    std::vector<int32_t> rj(rows > 0 ? rows : 1);
+    for (int t = 0; t < joins; ++t)
+        for (int64_t r = 0; r < J[t].n; ++r) rj[J[t].gbase + r] = t;
+    for (int f = 0; f < N_FAMILIES; ++f) {
+        const int64_t nf = b->fam_begin[f + 1] - b->fam_begin[f];
+        b->fam_grid[f] = (int)std::max<int64_t>(1, std::min<int64_t>(nf, (int64_t)screen_batch_occupancy(f) * num_cus()));
+    }
+
+    // device tables
+    size_t tcur = 0;
+    const size_t o_j = bcarve(tcur, sizeof(BatchJoin) * J.size());
+    const size_t o_g = bcarve(tcur, sizeof(BatchGroup) * G.size());
+    const size_t o_i = bcarve(tcur, sizeof(int2) * it.size());
+    const size_t o_r = bcarve(tcur, sizeof(int32_t) * rj.size());
+    hipError_t e = hipMalloc(&b->dev, tcur);
+    if (e != hipSuccess) {
+        delete b;
+        return hip_fail(e, "hipMalloc(batch tables)");
+    }
+    uint8_t* base = (uint8_t*)b->dev;
+    b->d_joins = (BatchJoin*)(base + o_j);
+    b->d_groups = (BatchGroup*)(base + o_g);
+    b->d_items = (int2*)(base + o_i);
+    b->d_row_join = (int32_t*)(base + o_r);
+    if ((e = hipMemcpy(b->d_joins, J.data(), sizeof(BatchJoin) * J.size(), hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(b->d_groups, G.data(), sizeof(BatchGroup) * G.size(), hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(b->d_items, it.data(), sizeof(int2) * it.size(), hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(b->d_row_join, rj.data(), sizeof(int32_t) * rj.size(), hipMemcpyHostToDevice)) != hipSuccess) {
+        (void)hipFree(b->dev);
+        delete b;
+        return hip_fail(e, "hipMemcpy(batch tables)");
+    }
+    b->timing = false;
+    if (flags & TJ_FLAG_TIMING) {
+        for (int k = 0; k < 5; ++k) {
+            if ((e = hipEventCreate(&b->ev[k])) != hipSuccess) {
+                for (int q = 0; q < k; ++q) (void)hipEventDestroy(b->ev[q]);
+                (void)hipFree(b->dev);
+                delete b;
+                return hip_fail(e, "hipEventCreate");
+            }
+        }
+        b->timing = true;
+    }
+    b->has_graph = false;
+    *out = b;
+    return TJ_OK;
+}
+
+int tj_batch_destroy(tj_batch* b) {
+    if (!b) return TJ_OK;
+    if (b->has_graph) (void)hipGraphExecDestroy(b->graph);
+    if (b->timing)
+        for (int k = 0; k < 5; ++k) (void)hipEventDestroy(b->ev[k]);
+    (void)hipFree(b->dev);
+    delete b;
+    return TJ_OK;
+}
+
+int tj_batch_workspace_bytes(const tj_batch* b, size_t* bytes) {
+    TJ_CHECK(b && bytes, TJ_ERR_INVALID, "NULL argument");
+    *bytes = b->total;
+    return TJ_OK;
+}
+
+int tj_batch_describe(const tj_batch* b, char* buf, size_t len) {
+    TJ_CHECK(b && buf, TJ_ERR_INVALID, "NULL argument");
+    snprintf(buf, len,
+             "{\"joins\":%d,\"rows\":%lld,\"max_n\":%lld,\"pairs\":%lld,\"groups\":%d,\"items\":%lld,"
+             "\"items_fp16\":%lld,\"items_fp8\":%lld,\"grid_fp16\":%d,\"grid_fp8\":%d,"
+             "\"ibits\":%d,\"jbits\":%d,\"keybits\":%d,\"out_capacity\":%lld,\"cand_capacity\":%lld,"
+             "\"workspace\":%zu}",
+             b->nj, (long long)b->rows, (long long)b->max_n, (long long)b->pairs_total, b->n_groups,
+             (long long)b->n_items, (long long)(b->fam_begin[1] - b->fam_begin[0]),
+             (long long)(b->fam_begin[2] - b->fam_begin[1]), b->fam_grid[0], b->fam_grid[1], b->ibits, b->jbits, b->keybits, (long long)b->out_cap,
+             (long long)b->cand_cap, b->total);
+    return TJ_OK;
+}
+
+int tj_batch_stage_times(tj_batch* b, float* ms, int n) {
+    TJ_CHECK(b && ms, TJ_ERR_INVALID, "NULL argument");
+    TJ_CHECK(b->timing, TJ_ERR_INVALID, "batch was created without TJ_FLAG_TIMING");
+    TJ_HIP(hipEventSynchronize(b->ev[4]));
+    for (int k = 0; k < n && k < 4; ++k) TJ_HIP(hipEventElapsedTime(&ms[k], b->ev[k], b->ev[k + 1]));
+    return TJ_OK;
+}
+
+static int batch_check(const tj_batch* b, const float* const* x, const double* T, void* ws, int32_t* pairs,
+                       int64_t* offsets, uint64_t* stats) {
+    TJ_CHECK(b != nullptr, TJ_ERR_INVALID, "batch is NULL");
+    TJ_CHECK(x && T && ws && offsets && stats, TJ_ERR_INVALID, "NULL argument");
+    TJ_CHECK(pairs != nullptr || b->out_cap == 0, TJ_ERR_INVALID, "pairs buffer is NULL");
+    return TJ_OK;
+}
+
+int tj_batch_join(tj_batch* b, const float* const* x, const double* T, void* workspace, int32_t* pairs,
+                  int64_t* offsets, uint64_t* stats, void* stream) {
+    int rc = batch_check(b, x, T, workspace, pairs, offsets, stats);
+    if (rc) return rc;
+    const BatchCall c{b, x, T, workspace, pairs, offsets, stats, true};
+    return batch_enqueue(c, (hipStream_t)stream);
+}
+
+static int batch_capture_fn(void* ctx, hipStream_t s) { return batch_enqueue(*(const BatchCall*)ctx, s); }
+
+int tj_batch_capture(tj_batch* b, const float* const* x, const double* T, void* workspace, int32_t* pairs,
+                     int64_t* offsets, uint64_t* stats) {
+    int rc = batch_check(b, x, T, workspace, pairs, offsets, stats);
+    if (rc) return rc;
+    BatchCall c{b, x, T, workspace, pairs, offsets, stats, false};
+    hipGraphExec_t exec;
+    if ((rc = capture_graph(&exec, batch_capture_fn, &c))) return rc;
+    if (b->has_graph) (void)hipGraphExecDestroy(b->graph);
+    b->graph = exec;
+    b->has_graph = true;
+    return TJ_OK;
+}
+
+int tj_batch_replay(tj_batch* b, void* stream) {
+    TJ_CHECK(b != nullptr, TJ_ERR_INVALID, "batch is NULL");
+    TJ_CHECK(b->has_graph, TJ_ERR_INVALID, "no graph captured for this batch");
+    TJ_HIP(hipGraphLaunch(b->graph, (hipStream_t)stream));
+    return TJ_OK;
+}
+
+}  // extern "C"

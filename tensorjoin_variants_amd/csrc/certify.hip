@@ -1,125 +1,25 @@
-// certify.hip -- per-pair certificate for the screen's candidates, and the
-// FP64 refinement of the ambiguous band.
-//
-// A candidate is a pair the screen's fast reject could not exclude (all
-// pairs inside plus a thin band around the threshold).  For each one the
-// exact binary64 interval of its squared distance is formed from the MFMA
-// dot and the row certificate terms (classify, screen_impl.h): certified
-// inside -> output; certified outside -> dropped; ambiguous -> the
-// sequential FP64 difference form decides (the refinement of
-// CEJ/src/run_g2a_tensorjoin.py:193-232, in the order of
-// CEJ/src/audit_g2b_accepted_fp64.py:51-57).  This is the reference's
-// cascade of accept / ambiguous queue / FP64 terminal
-// (TG/src/g2_operator_r1.py:52-57) with the queue kept on the device.
-//
-// Each refined pair also reports |d2_screen - d2_fp64| and its ratio to the
-// certified half-width; a pair outside its interval counts as a violation
-// (always 0 for a sound certificate).  With the audit flag, certified-inside
-// pairs are recomputed in FP64 as well.
-#include "screen_impl.h"
+// certify.hip -- the certificate pass of a single join (certify_impl.h).
+#include "certify_impl.h"
 
-namespace {
+namespace tjcert {
 
-__device__ inline void atomic_max_pos(uint64_t* slot, double v) {
-    if (v > 0.0) atomicMax((unsigned long long*)slot, (unsigned long long)__builtin_bit_cast(uint64_t, v));
-}
-
-__global__ void __launch_bounds__(256) certify_kernel(const float* __restrict__ R, const float* __restrict__ S, int d,
-                                                      int jbits, const uint64_t* __restrict__ ckeys,
-                                                      const float* __restrict__ cp, int64_t ccap,
-                                                      const RowMeta* __restrict__ ma, const RowMeta* __restrict__ mb,
-                                                      const JoinParams* __restrict__ pp, uint64_t* __restrict__ out_keys,
-                                                      int64_t out_cap, uint64_t* __restrict__ stats, int audit) {
-    const JoinParams P = *pp;
-    int64_t count = (int64_t)stats[TJ_ST_CANDIDATES];
-    if (count > ccap) count = ccap;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const uint64_t jmask = (1ull << jbits) - 1;
-    double max_err = 0.0, max_ratio = 0.0;
-    uint64_t viol = 0, n_cert = 0, n_amb = 0, n_amb_in = 0;
-    // accepted keys are staged per wave and appended with one atomic per
-    // OCAP keys (most candidates are accepted: one atomic per iteration would
-    // serialise on the output cursor)
-    constexpr int OCAP = 512;
-    __shared__ uint64_t sOut[4][OCAP];
-    int nO = 0;
-    auto flush = [&]() {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        uint64_t pos = 0;
-        if (lane == 0) pos = atomicAdd((unsigned long long*)&stats[TJ_ST_OUT], (unsigned long long)nO);
-        pos = __shfl(pos, 0);
-        for (int t = lane; t < nO; t += 64)
-            if (pos + (uint64_t)t < (uint64_t)out_cap) out_keys[pos + t] = sOut[wave][t];
-        __builtin_amdgcn_wave_barrier();
-        nO = 0;
-    };
-    const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
-    const int64_t w = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave;
-    for (int64_t base = w * 64; base < count; base += nwaves * 64) {
-        const int64_t t = base + lane;
-        bool inside = false;
-        uint64_t key = 0;
-        if (t < count) {
-            key = ckeys[t];
-            const float p = cp[t];
-            const uint64_t i = key >> jbits, j = key & jmask;
-            float est, hw;
-            const int cls = tjscreen::classify(p, ma[i], mb[j], P, &est, &hw);
-            inside = cls == 1;
-            if (cls == 1) n_cert++;
-            if (cls == 2 || (audit && cls == 1)) {
-                const double d2 = d2_seq(R + i * (uint64_t)d, S + j * (uint64_t)d, d);
-                const bool in64 = d2 <= P.T;
-                const double dj = d2 / P.scale_real;  // exact power-of-two scaling
-                const double err = fabs(dj - (double)est);
-                max_err = fmax(max_err, err * P.scale_real);
-                if (hw > 0.0f) max_ratio = fmax(max_ratio, err / (double)hw);
-                // the interval encloses the exact d2; allow the binary64 gap g
-                // and the binary32 rounding of the centre
-                const bool outside_interval = err > (double)hw + 2.0 * P.g64 * dj + 0x1p-23 * fabs((double)est);
-                if (outside_interval || (cls == 1 && !in64)) {
-                    viol++;
-                    stats[TJ_ST_VIOL_KEY] = key | (cls == 1 ? (1ull << 63) : 0ull);
-                    stats[TJ_ST_VIOL_D2] = __builtin_bit_cast(uint64_t, dj);
-                    stats[TJ_ST_VIOL_EST] = (uint64_t)__builtin_bit_cast(uint32_t, est) |
-                                            ((uint64_t)__builtin_bit_cast(uint32_t, hw) << 32);
-                }
-                if (cls == 2) {
-                    n_amb++;
-                    if (in64) n_amb_in++;
-                    inside = in64;
-                }
-            }
-        }
-        const uint64_t m = __ballot(inside);
-        if (m) {
-            const int k = __popcll(m);
-            if (nO + k > OCAP) flush();
-            if (inside) sOut[wave][nO + tjscreen::lanes_below(m)] = key;
-            nO += k;
-        }
+// Resolves a pair key of one join: both rows' metadata and FP32 rows, and the
+// join's parameter block.
+struct SingleSrc {
+    const float* R; const float* S; int d; int jbits;
+    const RowMeta* ma; const RowMeta* mb; const JoinParams* pp;
+    __device__ inline void fetch(uint64_t key, PairRef& r) const {
+        const uint64_t i = key >> jbits, j = key & ((1ull << jbits) - 1);
+        r.P = pp;
+        r.mi = ma[i];
+        r.mj = mb[j];
+        r.xi = R + i * (uint64_t)d;
+        r.xj = S + j * (uint64_t)d;
+        r.d = d;
     }
-    if (nO) flush();
-    for (int o = 32; o > 0; o >>= 1) {
-        max_err = fmax(max_err, __shfl_xor(max_err, o));
-        max_ratio = fmax(max_ratio, __shfl_xor(max_ratio, o));
-        viol += __shfl_xor(viol, o);
-        n_cert += __shfl_xor(n_cert, o);
-        n_amb += __shfl_xor(n_amb, o);
-        n_amb_in += __shfl_xor(n_amb_in, o);
-    }
-    if (lane == 0) {
-        atomic_max_pos(&stats[TJ_ST_MAX_ERR], max_err);
-        atomic_max_pos(&stats[TJ_ST_MAX_RATIO], max_ratio);
-        if (viol) atomicAdd((unsigned long long*)&stats[TJ_ST_VIOLATIONS], (unsigned long long)viol);
-        if (n_cert) atomicAdd((unsigned long long*)&stats[TJ_ST_CERT_ACCEPT], (unsigned long long)n_cert);
-        if (n_amb) atomicAdd((unsigned long long*)&stats[TJ_ST_AMB], (unsigned long long)n_amb);
-        if (n_amb_in) atomicAdd((unsigned long long*)&stats[TJ_ST_FP64_ACCEPT], (unsigned long long)n_amb_in);
-    }
-}
+};
 
-}  // namespace
+}  // namespace tjcert
 
 namespace tj {
 int launch_certify(const float* R, const float* S, int d, int jbits, const uint64_t* cand_keys, const float* cand_p,
@@ -129,8 +29,9 @@ int launch_certify(const float* R, const float* S, int d, int jbits, const uint6
     int64_t blocks = (cand_cap + 255) / 256;
     const int64_t maxb = (int64_t)num_cus() * 8;
     if (blocks > maxb) blocks = maxb;
-    hipLaunchKernelGGL(certify_kernel, dim3((unsigned)blocks), dim3(256), 0, s, R, S, d, jbits, cand_keys, cand_p,
-                       cand_cap, ma, mb, p, out_keys, out_cap, stats, audit);
+    const tjcert::SingleSrc src{R, S, d, jbits, ma, mb, p};
+    hipLaunchKernelGGL(tjcert::certify_kernel<tjcert::SingleSrc>, dim3((unsigned)blocks), dim3(256), 0, s, src,
+                       cand_keys, cand_p, cand_cap, out_keys, out_cap, stats, audit);
     TJ_HIP(hipGetLastError());
     return TJ_OK;
 }

@@ -7,7 +7,7 @@
 // FP64 residual bound, invalid flag) and gpu_norm_metadata
 // (G16/src/gpu_norms.py:11-25).  Bounds are re-derived here for general
 // magnitudes (the reference admitted only |x| <= 1).
-#include "tj_internal.h"
+#include "meta_impl.h"
 
 namespace {
 
@@ -35,173 +35,23 @@ __global__ void __launch_bounds__(256) scale_scan_kernel(const float* __restrict
     }
 }
 
-// -------------------------------------------------------------- parameters
 __global__ void params_kernel(const uint32_t* maxabs, double T, int d, int dp, int prec, JoinParams* p,
                               uint64_t* stats) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    JoinParams q;
-    uint32_t mb = *maxabs;
-    int G = 0;
-    if (mb != 0) G = ilogb((double)__builtin_bit_cast(float, mb)) - prec_top_exp(prec);
-    q.G = G;
-    q.T = T;
-    q.Tj = ldexp(T, -2 * G);
-    q.d = d;
-    q.dp = dp;
-    q.prec = prec;
-    q.terms = prec_terms(prec);
-    q.esize = prec_esize(prec);
-    // Relative gap between the exact squared distance and its sequential
-    // binary64 evaluation: every term suffers one rounding in the difference,
-    // one in the square and at most d-1 in the running sum, so
-    // |d2_seq - d2| <= gamma_{d+2} d2 with unit roundoff 2^-53; (d+4)*2^-52
-    // dominates gamma_{d+2} with room for the products below.
-    q.g64 = (double)(d + 4) * 0x1p-52;
-    double two_g = 2.0 * q.g64;
-    // Tj*(1-2g) rounded down / Tj*(1+2g) rounded up (one binary64 rounding
-    // each, absorbed by stepping one more ulp outward).
-    q.Tj_acc = nextafter(q.Tj * (1.0 - two_g), 0.0);
-    q.Tj_rej = nextafter(q.Tj * (1.0 + two_g), INFINITY);
-    // MFMA FP32 accumulation model (order- and rounding-mode-agnostic): the
-    // k products of one accumulator chain are exact in binary32 (fp16/bf16/
-    // fp8 significands multiply exactly); each of at most 2k+2 additions or
-    // internal alignments loses at most 2^-23 relative, so
-    // |p - sum a b| <= gamma * sum |a b| + alpha, gamma = m u / (1 - m u),
-    // m = 2k+2, u = 2^-23; alpha covers flush-to-zero of subnormal partials.
-    // Same form as the reference's FP16 premise (F16C/NUMERICAL_SCOPE.md:11-43,
-    // gamma = 1026/(2^23-1026) at k = 512), k counted over every term of the
-    // hi/lo split here.
-    // (k counts the products of every term of the hi/lo split; +1 for the
-    // accumulator seed t_i of the fast path)
-    double k = (double)prec_terms(prec) * dp + 1.0;
-    double mu = (2.0 * k + 2.0) * 0x1p-23;
-    q.gamma_acc = mu / (1.0 - mu) * (1.0 + 0x1p-40);
-    // The FP8 MFMA (scaled f8f6f4 and plain fp8 alike) sums the products of
-    // one instruction in a fixed-point window aligned to the largest product,
-    // about 18 bits wide: measured on the device (tj_mfma_probe), a dominant
-    // product plus 63 products 2^-k smaller loses up to 2^-12.4 of the
-    // largest product, random operands up to 2^-11.6 of sum |a b|.  The
-    // premise adds 2^-9 sum |a b| (a 6x margin) to the binary32 chain term.
-    if (prec_is_fp8(prec)) q.gamma_acc += 0x1p-9;
-    q.alpha = (2.0 * k + 2.0) * 0x1p-126;
-    q.rho = prec_rho(prec);
-    q.scale_real = ldexp(1.0, 2 * G);
-    q.pad[0] = q.pad[1] = 0;
+    const JoinParams q = tjmeta::make_params(*maxabs, T, d, dp, prec);
     *p = q;
-    stats[TJ_ST_SCALE_EXP] = (uint64_t)(int64_t)G;
+    stats[TJ_ST_SCALE_EXP] = (uint64_t)(int64_t)q.G;
 }
 
-// ------------------------------------------------------------------ rowmeta
-// One wave per row.  Lane t handles coordinates t, t+64, ... of the padded
-// row.  Sums of non-negative binary64 terms: relative error <= (n+1) 2^-53.
 template <int PREC>
 __global__ void __launch_bounds__(256) rowmeta_kernel(const float* __restrict__ x, int64_t n, int d,
                                                       int dp, const JoinParams* __restrict__ pp,
                                                       uint8_t* __restrict__ qhi, uint8_t* __restrict__ qlo,
                                                       RowMeta* __restrict__ meta) {
-    constexpr int TERMS = (PREC == P_FP16X3 || PREC == P_BF16X3 || PREC == P_FP8X3) ? 3 : 1;
-    constexpr bool FP8 = (PREC == P_FP8 || PREC == P_FP8X3);
-    constexpr bool BF16 = (PREC == P_BF16 || PREC == P_BF16X3);
-    constexpr int ES = FP8 ? 1 : 2;
-    const int lane = threadIdx.x & 63;
     const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (row >= n) return;
     const JoinParams P = *pp;
-    const double sc = ldexp(1.0, -P.G);
-    const float* xr = x + row * (int64_t)d;
-
-    // FP8: the per-row scale needs the row max first.
-    int e_hi = 0, e_lo = 0;
-    if (FP8) {
-        double m = 0.0;
-        for (int k = lane; k < d; k += 64) m = fmax(m, fabs((double)xr[k] * sc));
-        for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
-        e_hi = fp8_row_exp(m);
-        if (TERMS == 3) {
-            // lo carries the residual of the hi rounding: at most half an ulp
-            // of e4m3 (2^-4 relative) of the largest element.
-            double mr = 0.0;
-            for (int k = lane; k < d; k += 64) {
-                double v = (double)xr[k] * sc, b;
-                enc_e4m3(ldexp(v, -e_hi), &b);
-                mr = fmax(mr, fabs(v - ldexp(b, e_hi)));
-            }
-            for (int o = 32; o > 0; o >>= 1) mr = fmax(mr, __shfl_xor(mr, o));
-            e_lo = fp8_row_exp(mr);
-        }
-    }
-
-    double nx = 0.0, nh = 0.0, nl = 0.0, nr = 0.0;
-    for (int k = lane; k < dp; k += 64) {
-        double v = k < d ? (double)xr[k] * sc : 0.0;
-        double bh = 0.0, bl = 0.0;
-        if (FP8) {
-            uint8_t ch = enc_e4m3(ldexp(v, -e_hi), &bh);
-            bh = ldexp(bh, e_hi);
-            qhi[row * dp + k] = ch;
-            if (TERMS == 3) {
-                uint8_t cl = enc_e4m3(ldexp(v - bh, -e_lo), &bl);
-                bl = ldexp(bl, e_lo);
-                qlo[row * dp + k] = cl;
-            }
-        } else {
-            uint16_t ch = BF16 ? enc_bf16(v, &bh) : enc_fp16(v, &bh);
-            ((uint16_t*)qhi)[row * dp + k] = ch;
-            if (TERMS == 3) {
-                uint16_t cl = BF16 ? enc_bf16(v - bh, &bl) : enc_fp16(v - bh, &bl);
-                ((uint16_t*)qlo)[row * dp + k] = cl;
-            }
-        }
-        double r = v - bh - bl;  // exact: the codes are roundings of v and of v - bh
-        nx += v * v;
-        nh += bh * bh;
-        nl += bl * bl;
-        nr += r * r;
-    }
-    for (int o = 32; o > 0; o >>= 1) {
-        nx += __shfl_xor(nx, o);
-        nh += __shfl_xor(nh, o);
-        nl += __shfl_xor(nl, o);
-        nr += __shfl_xor(nr, o);
-    }
-    if (lane != 0) return;
-    (void)ES;
-    const double rel = (double)(dp + 4) * 0x1p-52;  // sum of dp non-negative terms, each squared once
-    const double root_pad = 1.0 + 0x1p-48;           // sqrt rounding
-    const float nlo = f32_down(nx * (1.0 - rel));
-    const float nhi = f32_up(nx * (1.0 + rel));
-    const float H = f32_up(sqrt(nh * (1.0 + rel)) * root_pad);
-    const float L = TERMS == 3 ? f32_up(sqrt(nl * (1.0 + rel)) * root_pad) : 0.0f;
-    const float E = f32_up(sqrt(nr * (1.0 + rel)) * root_pad);
-
-    // Fast-path terms.  The screen seeds the MFMA accumulator of row i with
-    // t_i, so one chain yields s_ij = t_i + p_ij (p the dot of the quantized
-    // rows) with |s - (t_i + sum a b)| <= gamma (|t_i| + sum |a b|) + alpha.
-    // With Z = H + L the residual part of the exact dot is bounded by
-    //   L_i L_j + Z_i E_j + E_i Z_j + E_i E_j,  sum |a b| <= H_i H_j [+ H_i L_j + L_i H_j],
-    // and 2ab <= rho a^2 + b^2/rho splits everything except gamma |t_i| into
-    //   beta_i + beta_j,  beta = ((gamma + rho) Z^2 + L^2 + (1/rho + 1) E^2) / 2.
-    // The exact squared distance then satisfies
-    //   d2 >= nlo_i + nlo_j - 2 s + 2 t_i - 2 gamma |t_i| - 2 beta_i - 2 beta_j - 2 alpha.
-    // Choosing 2 t_i - 2 gamma |t_i| >= D_i = Tj_rej + 2 beta_i + 2 alpha - nlo_i
-    // leaves  d2 > Tj_rej  whenever  s < v_j = (nlo_j - 2 beta_j) / 2:
-    // the whole fast reject is one binary32 compare of the MFMA output.
-    const double Z = (double)H + (double)L;
-    const double rho = P.rho;
-    const double pad = 1.0 + 0x1p-44;
-    const double beta = ((P.gamma_acc + rho) * Z * Z + (double)L * L + (1.0 / rho + 1.0) * (double)E * E) * 0.5 * pad;
-    double D = P.Tj_rej + 2.0 * beta + 2.0 * P.alpha - (double)nlo;
-    D += fabs(D) * 0x1p-50 + 0x1p-1000;  // binary64 rounding of D, outward
-    const double tseed = D >= 0.0 ? D / (2.0 * (1.0 - P.gamma_acc)) : D / (2.0 * (1.0 + P.gamma_acc));
-    const float t32 = f32_up(tseed + fabs(tseed) * 0x1p-50 + 0x1p-1000);  // rounded upward
-    double v = 0.5 * ((double)nlo - 2.0 * beta);
-    v -= fabs(v) * 0x1p-50 + 0x1p-1000;
-    const float v32 = f32_down(v);
-    RowMeta m;
-    m.m0 = make_float4(nlo, nhi, H, E);
-    uint32_t packed = (uint32_t)(e_hi + 32768) | ((uint32_t)(e_lo + 32768) << 16);
-    m.m1 = make_float4(L, t32, v32, __builtin_bit_cast(float, packed));
-    meta[row] = m;
+    tjmeta::rowmeta_row<PREC>(x + row * (int64_t)d, row, d, dp, P, threadIdx.x & 63, qhi, qlo, meta);
 }
 
 __global__ void fill_u64_kernel(uint64_t* p, int64_t n, uint64_t v) {

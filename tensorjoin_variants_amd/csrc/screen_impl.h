@@ -192,16 +192,250 @@ struct Geo {
 
 constexpr int CCAP = 256;  // per-wave staged candidates
 
+// Operands of one panel sweep.  Row indices are positions in the quantized
+// arrays; the emitted key of pair (i, j) is key_pack(i + koff_a, j + koff_b),
+// which lets a batch of joins share one row space and one key space.
+struct PanelIO {
+    const uint8_t* qa[2]; const RowMeta* ma; int64_t na;  // rows [.., na) are valid
+    const uint8_t* qb[2]; const RowMeta* mb; int64_t nb;  // columns [.., nb) are valid
+    int64_t koff_a, koff_b;
+    int jbits;
+};
+
+// The global candidate queue (pair key, MFMA output s) and its cursor.
+struct CandOut {
+    uint64_t* keys; float* aux; int64_t cap; uint64_t* counter;
+};
+
+// Append the nC candidates staged in this wave's LDS buffers to the queue
+// (one atomic per flush; entries beyond the capacity are counted, not kept).
+__device__ __forceinline__ void cand_flush(const CandOut& co, const uint64_t* sCK, const float* sCP, int& nC, int lane) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    uint64_t base = 0;
+    if (lane == 0) base = atomicAdd((unsigned long long*)co.counter, (unsigned long long)nC);
+    base = uni64(base);
+    for (int t = lane; t < nC; t += 64) {
+        const uint64_t idx = base + (uint64_t)t;
+        if (idx < (uint64_t)co.cap) {
+            co.keys[idx] = sCK[t];
+            co.aux[idx] = sCP[t];
+        }
+    }
+    // keep the in-order vmcnt accounting of the prefetch pipeline exact
+    __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __builtin_amdgcn_wave_barrier();
+    nC = 0;
+}
+
+// LDS bytes of the S ring of one (precision, width) instantiation.
+template <int PREC, int DP>
+constexpr int ring_bytes() { return Geo<PREC, DP>::NSTAGE * Geo<PREC, DP>::BLKB; }
+
+// One work item: the panel of BM = 4 x RB x 32 stationary rows starting at r0
+// against columns [cb, ce), all 4 waves of the workgroup.  sB/sNp/sSc are the
+// workgroup's S ring (NSTAGE slots), sP/sCK/sCP this wave's staging buffers.
 template <int PREC, int DP, int RB, bool SELF>
-__global__ void __launch_bounds__(256, 2) screen_kernel(const tj::ScreenArgs A) {
+__device__ __forceinline__ void screen_panel(const PanelIO& io, const CandOut& co, int64_t r0, int64_t cb, int64_t ce,
+                                             uint8_t* sB, float* sNp, uint32_t* sSc, float* sP, uint64_t* sCK,
+                                             float* sCP, int& nC) {
     using C = Cfg<PREC>;
     using Gm = Geo<PREC, DP>;
     constexpr int NSTAGE = Gm::NSTAGE;
-    constexpr int BM = 4 * RB * 32;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int hh = lane >> 5, c32 = lane & 31;
+    const int64_t na = io.na, nb = io.nb;
+    const int64_t rw0 = r0 + (int64_t)wave * RB * 32;
+    const int nblk = (int)((ce - cb + 31) / 32);
 
-    __shared__ __attribute__((aligned(16))) uint8_t sB[NSTAGE][Gm::BLKB];
-    __shared__ __attribute__((aligned(16))) float sNp[NSTAGE][64];
-    __shared__ __attribute__((aligned(16))) uint32_t sSc[NSTAGE][64];
+    // ---- stationary fragments, thresholds and scales (VGPR resident)
+    Frag<C::CH> fa[C::NT][RB][Gm::KS];
+    f32x16 seed[RB];
+    int sa[C::NT][RB];
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) {
+        const int64_t row = rw0 + rb * 32 + c32;
+        const int64_t rowc = row < na ? row : na - 1;
+#pragma unroll
+        for (int a = 0; a < C::NT; ++a)
+#pragma unroll
+            for (int t = 0; t < Gm::KS; ++t) ld_frag(fa[a][rb][t], io.qa[a] + rowc * Gm::ROWB + (2 * t + hh) * C::CH);
+        if constexpr (C::FP8) {
+            const uint32_t pk = __builtin_bit_cast(uint32_t, io.ma[rowc].m1.w);
+            sa[0][rb] = (int)(pk & 0xFFFFu) - 32768 + 127;
+            if (C::NT == 2) sa[C::NT - 1][rb] = (int)(pk >> 16) - 32768 + 127;
+        } else {
+#pragma unroll
+            for (int a = 0; a < C::NT; ++a) sa[a][rb] = 127;
+        }
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int64_t ri = rw0 + rb * 32 + 8 * (r >> 2) + 4 * hh + (r & 3);
+            seed[rb][r] = ri < na ? io.ma[ri].m1.y : -INFINITY;
+        }
+    }
+
+    // Consume the stationary registers here, so that the compiler's wait
+    // counts (blind to the asm DMA below) settle before the block loop
+    // instead of inserting vmcnt(0) inside it, which would drain the ring.
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) {
+#pragma unroll
+        for (int a = 0; a < C::NT; ++a)
+#pragma unroll
+            for (int t = 0; t < Gm::KS; ++t) {
+                if constexpr (C::CH == 16) {
+                    const uint4 q = fa[a][rb][t].v;
+                    __asm__ volatile("" ::"v"(q.x), "v"(q.y), "v"(q.z), "v"(q.w));
+                } else {
+                    const uint4 q0 = fa[a][rb][t].v0, q1 = fa[a][rb][t].v1;
+                    __asm__ volatile("" ::"v"(q0.x), "v"(q0.y), "v"(q0.z), "v"(q0.w));
+                    __asm__ volatile("" ::"v"(q1.x), "v"(q1.y), "v"(q1.z), "v"(q1.w));
+                }
+            }
+#pragma unroll
+        for (int r = 0; r < 16; ++r) __asm__ volatile("" ::"v"(seed[rb][r]));
+    }
+
+    // ---- S blocks: direct global->LDS loads into an NSTAGE ring
+    auto issue = [&](int blk) {
+        const int slot = blk % NSTAGE;
+        const int64_t c0 = cb + (int64_t)blk * 32;
+        const uint32_t dst = __builtin_amdgcn_readfirstlane(lds_addr(sB + slot * Gm::BLKB));
+#pragma unroll
+        for (int u = 0; u * 256 < Gm::UNITS; ++u) {
+            const int pbase = wave * 64 + u * 256;  // this wave's 64 consecutive units
+            if (pbase < Gm::UNITS) {
+                const int p = pbase + lane;
+                const int a = p / (32 * Gm::R16);
+                const int rem = p % (32 * Gm::R16);
+                const int col = rem / Gm::R16;
+                const int qs = rem % Gm::R16;
+                const int q = qs ^ ((col >> Gm::FSH) & Gm::FMASK);
+                int64_t j = c0 + col;
+                if (j >= nb) j = nb - 1;
+                // a select, not io.qb[a]: a runtime index would put io on the stack
+                const uint8_t* src = (C::NT == 2 && a != 0) ? io.qb[1] : io.qb[0];
+                dma16(src + j * Gm::ROWB + q * 16, __builtin_amdgcn_readfirstlane(dst + (uint32_t)pbase * 16u));
+            }
+        }
+        if (wave == 0 && lane < 32) {
+            int64_t j = c0 + lane;
+            if (j >= nb) j = nb - 1;
+            dma4(&io.mb[j].m1.z, __builtin_amdgcn_readfirstlane(lds_addr(sNp + slot * 64)));
+            if constexpr (C::FP8) dma4(&io.mb[j].m1.w, __builtin_amdgcn_readfirstlane(lds_addr(sSc + slot * 64)));
+        }
+    };
+    const int pre = nblk < NSTAGE - 1 ? nblk : NSTAGE - 1;
+    for (int b = 0; b < pre; ++b) issue(b);
+
+    for (int blk = 0; blk < nblk; ++blk) {
+        const int younger = (nblk - 1 - blk) < (NSTAGE - 2) ? (nblk - 1 - blk) : (NSTAGE - 2);
+        wait_vm(younger * Gm::LPB);
+        lds_barrier();
+        if (blk + NSTAGE - 1 < nblk) issue(blk + NSTAGE - 1);
+        const int slot = blk % NSTAGE;
+        const int64_t c0 = cb + (int64_t)blk * 32;
+        const int64_t jcol = c0 + c32;
+        const bool skip = SELF && (c0 + 31 <= rw0);
+        if (skip) continue;
+        float nj = INFINITY;
+        int sbh = 127, sbl = 127;
+        if (jcol < nb) {
+            nj = sNp[slot * 64 + c32];
+            if constexpr (C::FP8) {
+                const uint32_t pk = sSc[slot * 64 + c32];
+                sbh = (int)(pk & 0xFFFFu) - 32768 + 127;
+                sbl = (int)(pk >> 16) - 32768 + 127;
+            }
+        }
+        f32x16 acc[RB];
+        const uint8_t* bbase = sB + slot * Gm::BLKB;
+#pragma unroll
+        for (int t = 0; t < Gm::KS; ++t) {
+            Frag<C::CH> fb;
+            if constexpr (C::CH == 16) {
+                fb.v = *(const uint4*)(bbase + Gm::unit(c32, 2 * t + hh) * 16);
+            } else {
+                fb.v0 = *(const uint4*)(bbase + Gm::unit(c32, 2 * (2 * t + hh)) * 16);
+                fb.v1 = *(const uint4*)(bbase + Gm::unit(c32, 2 * (2 * t + hh) + 1) * 16);
+            }
+            if constexpr (C::NT == 1) {
+#pragma unroll
+                for (int rb = 0; rb < RB; ++rb) {
+                    if (t == 0) acc[rb] = mma_c<PREC>(fa[0][rb][t], fb, seed[rb], sa[0][rb], sbh);
+                    else mma<PREC>(acc[rb], fa[0][rb][t], fb, sa[0][rb], sbh);
+                }
+            } else {
+                Frag<C::CH> fl;
+                const uint8_t* lbase = bbase + 32 * Gm::R16 * 16;
+                if constexpr (C::CH == 16) {
+                    fl.v = *(const uint4*)(lbase + Gm::unit(c32, 2 * t + hh) * 16);
+                } else {
+                    fl.v0 = *(const uint4*)(lbase + Gm::unit(c32, 2 * (2 * t + hh)) * 16);
+                    fl.v1 = *(const uint4*)(lbase + Gm::unit(c32, 2 * (2 * t + hh) + 1) * 16);
+                }
+#pragma unroll
+                for (int rb = 0; rb < RB; ++rb) {
+                    if (t == 0) acc[rb] = mma_c<PREC>(fa[0][rb][t], fb, seed[rb], sa[0][rb], sbh);
+                    else mma<PREC>(acc[rb], fa[0][rb][t], fb, sa[0][rb], sbh);
+                    mma<PREC>(acc[rb], fa[0][rb][t], fl, sa[0][rb], sbl);
+                    mma<PREC>(acc[rb], fa[1][rb][t], fb, sa[1][rb], sbh);
+                }
+            }
+        }
+        // ---- fast reject: s = t_i + p_ij < v_j rejects rigorously; one
+        // max over the 16 rows of a lane, one compare, one ballot per block
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb) {
+            float mx = acc[rb][0];
+#pragma unroll
+            for (int r = 1; r < 16; ++r) mx = fmaxf(mx, acc[rb][r]);
+            if (__ballot(mx >= nj) == 0) continue;
+            // ---- rare: stage the candidates (pair key, s) for the certify
+            // pass.  A wave-uniform 16-bit mask names the registers that
+            // hold one; the block goes to LDS and only those are visited.
+            uint32_t rm = 0;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) rm |= (__ballot(acc[rb][r] >= nj) != 0 ? 1u : 0u) << r;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) sP[r * 64 + lane] = acc[rb][r];
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            rm = uni((int)rm);
+            while (rm) {
+                const int r = __builtin_ctz(rm);
+                rm &= rm - 1;
+                const float sv = sP[r * 64 + lane];
+                const int rloc = rb * 32 + 8 * (r >> 2) + 4 * hh + (r & 3);
+                const int64_t i = rw0 + rloc;
+                const bool cand = sv >= nj && i < na && jcol < nb && (!SELF || i < jcol);
+                const uint64_t m = __ballot(cand);
+                if (m == 0) continue;
+                const int k = uni(__popcll(m));
+                if (nC + k > CCAP) cand_flush(co, sCK, sCP, nC, lane);
+                if (cand) {
+                    const int pos = nC + lanes_below(m);
+                    sCK[pos] = key_pack((uint64_t)(i + io.koff_a), (uint64_t)(jcol + io.koff_b), io.jbits);
+                    sCP[pos] = sv;
+                }
+                nC += k;
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+    }
+}
+
+// The single-join screen: a persistent grid takes (panel, column split)
+// items from an atomic cursor until the item space is exhausted.
+template <int PREC, int DP, int RB, bool SELF>
+__global__ void __launch_bounds__(256, 2) screen_kernel(const tj::ScreenArgs A) {
+    using Gm = Geo<PREC, DP>;
+    constexpr int BM = 4 * RB * 32;
+    __shared__ __attribute__((aligned(16))) uint8_t sB[Gm::NSTAGE * Gm::BLKB];
+    __shared__ __attribute__((aligned(16))) float sNp[Gm::NSTAGE * 64];
+    __shared__ __attribute__((aligned(16))) uint32_t sSc[Gm::NSTAGE * 64];
     __shared__ float sP[4][16 * 64];
     __shared__ uint64_t sCK[4][CCAP];
     __shared__ float sCP[4][CCAP];
@@ -209,33 +443,14 @@ __global__ void __launch_bounds__(256, 2) screen_kernel(const tj::ScreenArgs A) 
 
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
-    const int hh = lane >> 5, c32 = lane & 31;
-    const int64_t na = A.na, nb = A.nb;
     const int total_items = A.panels * A.col_splits;
-    const uint8_t* qa[2] = {(const uint8_t*)A.qa_hi, (const uint8_t*)A.qa_lo};
-    const uint8_t* qb[2] = {(const uint8_t*)A.qb_hi, (const uint8_t*)A.qb_lo};
-
+    PanelIO io;
+    io.qa[0] = (const uint8_t*)A.qa_hi; io.qa[1] = (const uint8_t*)A.qa_lo; io.ma = A.ma; io.na = A.na;
+    io.qb[0] = (const uint8_t*)A.qb_hi; io.qb[1] = (const uint8_t*)A.qb_lo; io.mb = A.mb; io.nb = A.nb;
+    io.koff_a = 0; io.koff_b = 0;
+    io.jbits = A.jbits;
+    const CandOut co{A.amb_keys, A.amb_aux, A.amb_cap, &A.stats[TJ_ST_CANDIDATES]};
     int nC = 0;
-    uint64_t cntCand = 0;
-
-    auto flush = [&]() {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        uint64_t base = 0;
-        if (lane == 0) base = atomicAdd((unsigned long long*)&A.stats[TJ_ST_CANDIDATES], (unsigned long long)nC);
-        base = uni64(base);
-        for (int t = lane; t < nC; t += 64) {
-            const uint64_t idx = base + (uint64_t)t;
-            if (idx < (uint64_t)A.amb_cap) {
-                A.amb_keys[idx] = sCK[wave][t];
-                A.amb_aux[idx] = sCP[wave][t];
-            }
-        }
-        // keep the in-order vmcnt accounting of the prefetch pipeline exact
-        __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
-        __builtin_amdgcn_wave_barrier();
-        nC = 0;
-    };
 
     for (;;) {
         if (tid == 0) sItem = atomicAdd((unsigned long long*)&A.stats[TJ_ST_WORK], 1ull);
@@ -246,200 +461,22 @@ __global__ void __launch_bounds__(256, 2) screen_kernel(const tj::ScreenArgs A) 
         const int panel = item % A.panels;
         const int split = item / A.panels;
         const int64_t r0 = (int64_t)panel * BM;
-        const int64_t rw0 = r0 + (int64_t)wave * RB * 32;
         int64_t cb, ce;
-        if (SELF) { cb = r0; ce = nb; }
-        else { cb = (int64_t)split * A.cols_per_split; ce = cb + A.cols_per_split; if (ce > nb) ce = nb; }
+        if (SELF) { cb = r0; ce = A.nb; }
+        else { cb = (int64_t)split * A.cols_per_split; ce = cb + A.cols_per_split; if (ce > A.nb) ce = A.nb; }
         if (cb >= ce) continue;
-        const int nblk = (int)((ce - cb + 31) / 32);
-
-        // ---- stationary fragments, thresholds and scales (VGPR resident)
-        Frag<C::CH> fa[C::NT][RB][Gm::KS];
-        f32x16 seed[RB];
-        int sa[C::NT][RB];
-#pragma unroll
-        for (int rb = 0; rb < RB; ++rb) {
-            const int64_t row = rw0 + rb * 32 + c32;
-            const int64_t rowc = row < na ? row : na - 1;
-#pragma unroll
-            for (int a = 0; a < C::NT; ++a)
-#pragma unroll
-                for (int t = 0; t < Gm::KS; ++t) ld_frag(fa[a][rb][t], qa[a] + rowc * Gm::ROWB + (2 * t + hh) * C::CH);
-            if constexpr (C::FP8) {
-                const uint32_t pk = __builtin_bit_cast(uint32_t, A.ma[rowc].m1.w);
-                sa[0][rb] = (int)(pk & 0xFFFFu) - 32768 + 127;
-                if (C::NT == 2) sa[C::NT - 1][rb] = (int)(pk >> 16) - 32768 + 127;
-            } else {
-#pragma unroll
-                for (int a = 0; a < C::NT; ++a) sa[a][rb] = 127;
-            }
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int64_t ri = rw0 + rb * 32 + 8 * (r >> 2) + 4 * hh + (r & 3);
-                seed[rb][r] = ri < na ? A.ma[ri].m1.y : -INFINITY;
-            }
-        }
-
-        // Consume the stationary registers here, so that the compiler's wait
-        // counts (blind to the asm DMA below) settle before the block loop
-        // instead of inserting vmcnt(0) inside it, which would drain the ring.
-#pragma unroll
-        for (int rb = 0; rb < RB; ++rb) {
-#pragma unroll
-            for (int a = 0; a < C::NT; ++a)
-#pragma unroll
-                for (int t = 0; t < Gm::KS; ++t) {
-                    if constexpr (C::CH == 16) {
-                        const uint4 q = fa[a][rb][t].v;
-                        __asm__ volatile("" ::"v"(q.x), "v"(q.y), "v"(q.z), "v"(q.w));
-                    } else {
-                        const uint4 q0 = fa[a][rb][t].v0, q1 = fa[a][rb][t].v1;
-                        __asm__ volatile("" ::"v"(q0.x), "v"(q0.y), "v"(q0.z), "v"(q0.w));
-                        __asm__ volatile("" ::"v"(q1.x), "v"(q1.y), "v"(q1.z), "v"(q1.w));
-                    }
-                }
-#pragma unroll
-            for (int r = 0; r < 16; ++r) __asm__ volatile("" ::"v"(seed[rb][r]));
-        }
-
-        // ---- S blocks: direct global->LDS loads into an NSTAGE ring
-        auto issue = [&](int blk) {
-            const int slot = blk % NSTAGE;
-            const int64_t c0 = cb + (int64_t)blk * 32;
-            const uint32_t dst = __builtin_amdgcn_readfirstlane(lds_addr(sB[slot]));
-#pragma unroll
-            for (int u = 0; u * 256 < Gm::UNITS; ++u) {
-                const int pbase = wave * 64 + u * 256;  // this wave's 64 consecutive units
-                if (pbase < Gm::UNITS) {
-                    const int p = pbase + lane;
-                    const int a = p / (32 * Gm::R16);
-                    const int rem = p % (32 * Gm::R16);
-                    const int col = rem / Gm::R16;
-                    const int qs = rem % Gm::R16;
-                    const int q = qs ^ ((col >> Gm::FSH) & Gm::FMASK);
-                    int64_t j = c0 + col;
-                    if (j >= nb) j = nb - 1;
-                    dma16(qb[a] + j * Gm::ROWB + q * 16, __builtin_amdgcn_readfirstlane(dst + (uint32_t)pbase * 16u));
-                }
-            }
-            if (wave == 0 && lane < 32) {
-                int64_t j = c0 + lane;
-                if (j >= nb) j = nb - 1;
-                dma4(&A.mb[j].m1.z, __builtin_amdgcn_readfirstlane(lds_addr(&sNp[slot][0])));
-                if constexpr (C::FP8) dma4(&A.mb[j].m1.w, __builtin_amdgcn_readfirstlane(lds_addr(&sSc[slot][0])));
-            }
-        };
-        const int pre = nblk < NSTAGE - 1 ? nblk : NSTAGE - 1;
-        for (int b = 0; b < pre; ++b) issue(b);
-
-        for (int blk = 0; blk < nblk; ++blk) {
-            const int younger = (nblk - 1 - blk) < (NSTAGE - 2) ? (nblk - 1 - blk) : (NSTAGE - 2);
-            wait_vm(younger * Gm::LPB);
-            lds_barrier();
-            if (blk + NSTAGE - 1 < nblk) issue(blk + NSTAGE - 1);
-            const int slot = blk % NSTAGE;
-            const int64_t c0 = cb + (int64_t)blk * 32;
-            const int64_t jcol = c0 + c32;
-            const bool skip = SELF && (c0 + 31 <= rw0);
-            if (skip) continue;
-            float nj = INFINITY;
-            int sbh = 127, sbl = 127;
-            if (jcol < nb) {
-                nj = sNp[slot][c32];
-                if constexpr (C::FP8) {
-                    const uint32_t pk = sSc[slot][c32];
-                    sbh = (int)(pk & 0xFFFFu) - 32768 + 127;
-                    sbl = (int)(pk >> 16) - 32768 + 127;
-                }
-            }
-            f32x16 acc[RB];
-            const uint8_t* bbase = sB[slot];
-#pragma unroll
-            for (int t = 0; t < Gm::KS; ++t) {
-                Frag<C::CH> fb;
-                if constexpr (C::CH == 16) {
-                    fb.v = *(const uint4*)(bbase + Gm::unit(c32, 2 * t + hh) * 16);
-                } else {
-                    fb.v0 = *(const uint4*)(bbase + Gm::unit(c32, 2 * (2 * t + hh)) * 16);
-                    fb.v1 = *(const uint4*)(bbase + Gm::unit(c32, 2 * (2 * t + hh) + 1) * 16);
-                }
-                if constexpr (C::NT == 1) {
-#pragma unroll
-                    for (int rb = 0; rb < RB; ++rb) {
-                        if (t == 0) acc[rb] = mma_c<PREC>(fa[0][rb][t], fb, seed[rb], sa[0][rb], sbh);
-                        else mma<PREC>(acc[rb], fa[0][rb][t], fb, sa[0][rb], sbh);
-                    }
-                } else {
-                    Frag<C::CH> fl;
-                    const uint8_t* lbase = bbase + 32 * Gm::R16 * 16;
-                    if constexpr (C::CH == 16) {
-                        fl.v = *(const uint4*)(lbase + Gm::unit(c32, 2 * t + hh) * 16);
-                    } else {
-                        fl.v0 = *(const uint4*)(lbase + Gm::unit(c32, 2 * (2 * t + hh)) * 16);
-                        fl.v1 = *(const uint4*)(lbase + Gm::unit(c32, 2 * (2 * t + hh) + 1) * 16);
-                    }
-#pragma unroll
-                    for (int rb = 0; rb < RB; ++rb) {
-                        if (t == 0) acc[rb] = mma_c<PREC>(fa[0][rb][t], fb, seed[rb], sa[0][rb], sbh);
-                        else mma<PREC>(acc[rb], fa[0][rb][t], fb, sa[0][rb], sbh);
-                        mma<PREC>(acc[rb], fa[0][rb][t], fl, sa[0][rb], sbl);
-                        mma<PREC>(acc[rb], fa[1][rb][t], fb, sa[1][rb], sbh);
-                    }
-                }
-            }
-            // ---- fast reject: s = t_i + p_ij < v_j rejects rigorously; one
-            // max over the 16 rows of a lane, one compare, one ballot per block
-#pragma unroll
-            for (int rb = 0; rb < RB; ++rb) {
-                float mx = acc[rb][0];
-#pragma unroll
-                for (int r = 1; r < 16; ++r) mx = fmaxf(mx, acc[rb][r]);
-                if (__ballot(mx >= nj) == 0) continue;
-                // ---- rare: stage the candidates (pair key, s) for the certify
-                // pass.  A wave-uniform 16-bit mask names the registers that
-                // hold one; the block goes to LDS and only those are visited.
-                uint32_t rm = 0;
-#pragma unroll
-                for (int r = 0; r < 16; ++r) rm |= (__ballot(acc[rb][r] >= nj) != 0 ? 1u : 0u) << r;
-#pragma unroll
-                for (int r = 0; r < 16; ++r) sP[wave][r * 64 + lane] = acc[rb][r];
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                rm = uni((int)rm);
-                while (rm) {
-                    const int r = __builtin_ctz(rm);
-                    rm &= rm - 1;
-                    const float sv = sP[wave][r * 64 + lane];
-                    const int rloc = rb * 32 + 8 * (r >> 2) + 4 * hh + (r & 3);
-                    const int64_t i = rw0 + rloc;
-                    const bool cand = sv >= nj && i < na && jcol < nb && (!SELF || i < jcol);
-                    const uint64_t m = __ballot(cand);
-                    if (m == 0) continue;
-                    const int k = uni(__popcll(m));
-                    if (nC + k > CCAP) flush();
-                    if (cand) {
-                        const int pos = nC + lanes_below(m);
-                        sCK[wave][pos] = key_pack((uint64_t)i, (uint64_t)jcol, A.jbits);
-                        sCP[wave][pos] = sv;
-                    }
-                    nC += k;
-                    cntCand += (uint64_t)k;
-                }
-                __builtin_amdgcn_wave_barrier();
-            }
-        }
+        screen_panel<PREC, DP, RB, SELF>(io, co, r0, cb, ce, sB, sNp, sSc, sP[wave], sCK[wave], sCP[wave], nC);
     }
-    if (nC) flush();
-    (void)cntCand;
+    if (nC) cand_flush(co, sCK[wave], sCP[wave], nC, lane);
 }
 
-// Rows per wave: as many 32-row blocks as keep the stationary fragments
-// within 128 VGPRs (512 bytes per lane), at most 2.
+// Rows per wave: two 32-row blocks while their stationary fragments stay
+// within 64 VGPRs (256 bytes per lane) together, else one.
 template <int PREC, int DP>
 constexpr int rb_for() {
     using C = Cfg<PREC>;
     constexpr int bytes_per_block = C::NT * DP * C::ES / 2;  // per lane, one 32-row block (64 lanes x 2 halves)
-    return bytes_per_block * 2 <= 512 ? 2 : 1;
+    return bytes_per_block * 2 <= 256 ? 2 : 1;
 }
 
 template <int PREC, int DP, bool SELF>

@@ -102,3 +102,34 @@ def test_stats_decoding():
     raw[12] = np.array([-3]).view(np.uint64)[0]
     s = _lib.decode_stats(raw)
     assert s["out"] == 5 and s["max_err"] == 0.25 and s["scale_exp"] == -3
+
+
+def _batch(L, n, d, prec, out=1024, cand=1024):
+    J = len(n)
+    b = C.c_void_p()
+    rc = L.tj_batch_create(C.byref(b), J, (C.c_int64 * J)(*n), (C.c_int * J)(*d), (C.c_int * J)(*prec),
+                           out, cand, 0)
+    return rc, b
+
+
+@pytest.mark.parametrize("n,d,prec,code", [
+    ([], [], [], -1),                       # no joins
+    ([10, 20], [16, 0], [1, 1], -1),        # zero width
+    ([10, -1], [16, 16], [1, 1], -1),       # negative size
+    ([10, 20], [16, 300], [1, 1], -3),      # beyond the batch screen's widest body
+    ([10, 20], [16, 64], [1, 2], -3),       # hi/lo split not served by the batch screen
+    ([10, 20], [16, 64], [1, 3], -3),       # bf16 not served by the batch screen
+])
+def test_batch_rejects_bad_arguments(L, n, d, prec, code):
+    if not n:
+        b = C.c_void_p()
+        rc = L.tj_batch_create(C.byref(b), 0, None, None, None, 16, 16, 0)
+    else:
+        rc, b = _batch(L, n, d, prec)
+    assert rc == code
+    assert len(L.tj_last_error_string()) > 0
+
+
+def test_batch_capacity_bounds(L):
+    rc, _ = _batch(L, [10], [16], [1], out=1 << 33)
+    assert rc == -1
