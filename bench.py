@@ -16,7 +16,9 @@ Measured in the same run:
 
 Other workloads (--workload): clusters_selfjoin (8192 x 32 self-join),
 grid_rxs (4,194,304^2, d = 3, uniform grid), heavytail_fp8 (262,144^2,
-d = 512, Student-t, FP8 screen).  Prints one JSON line.
+d = 512, Student-t, FP8 screen), batched_uneven (4,096 self-joins of
+log-uniform size in [64, 16384], d in {16, 64, 256}, one pipeline replayed
+as a hipGraph).  Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -100,7 +102,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="certified_d128",
-                    choices=["certified_d128", "heavytail_fp8", "clusters_selfjoin", "grid_rxs"])
+                    choices=["certified_d128", "heavytail_fp8", "clusters_selfjoin", "grid_rxs", "batched_uneven"])
     ap.add_argument("--precision", default=None)
     ap.add_argument("--scale", type=float, default=1.0, help="row-count scale (1.0 = full workload)")
     ap.add_argument("--no-plain", action="store_true")
@@ -110,6 +112,9 @@ def main() -> int:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the multi-GPU sharding is "
               "described in DESIGN.md and not run", file=sys.stderr)
         return 2
+
+    if args.workload == "batched_uneven":
+        return bench_batched(args)
 
     import torch
     from tensorjoin_variants_amd import _lib
@@ -211,6 +216,134 @@ def main() -> int:
     if not args.no_cpu:
         log("cpu baseline")
         result["cpu_baseline"] = cpu_baseline(w)
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+def bench_batched(args) -> int:
+    """4,096 independent self-joins in one pipeline (hipGraph replay)."""
+    import torch
+    from tensorjoin_variants_amd import workloads
+    from tensorjoin_variants_amd.autotune import BatchTuner, width_class
+    from tensorjoin_variants_amd.batch import BatchPlan, initial_capacities
+    from tensorjoin_variants_amd.join import Plan, _as_device
+
+    joins = max(1, int(4096 * args.scale))
+    log(f"generating batched_uneven ({joins} joins)")
+    t0 = time.perf_counter()
+    ws = workloads.batched_uneven(joins=joins)
+    sets = [w.R for w in ws]
+    Ts = np.array([w.T for w in ws])
+    ns = [x.shape[0] for x in sets]
+    ds = [x.shape[1] for x in sets]
+    pairs_total = sum(n * (n - 1) // 2 for n in ns)
+    flops = float(sum(2.0 * d * (n * (n - 1) // 2) for n, d in zip(ns, ds)))
+    log(f"generated in {time.perf_counter() - t0:.1f}s: rows {sum(ns)}, pairs {pairs_total:.4e}")
+
+    # per-width precision from the batch tuner (timed on the largest joins of each width)
+    if args.precision:
+        precs = [args.precision] * joins
+        tune = None
+    else:
+        tuner = BatchTuner()
+        tune = tuner.tune(sets, Ts)
+        precs = tuner.choose(ds)
+        log("tuner", json.dumps(tune))
+    oc, cc = initial_capacities(ns)
+    plan = BatchPlan(ns, ds, precs, oc, cc, timing=True)
+    res, st = plan.run(sets, Ts)  # sizes the capacities
+    log("first run", {k: v for k, v in st.items() if k != "plan"})
+    xs, ptrs, Td = plan.inputs(sets, Ts)
+    plan.capture(ptrs, Td)
+
+    def run_steps(k, graph):
+        for _ in range(k):
+            plan.replay() if graph else plan.enqueue(ptrs, Td)
+
+    run_steps(args.warmup, True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run_steps(args.steps, True)
+    torch.cuda.synchronize()
+    ms_per_step = (time.perf_counter() - t0) * 1e3 / args.steps
+    got, stats = plan.results()
+    assert stats["out"] <= plan.out_capacity and stats["candidates"] <= plan.cand_capacity
+    assert stats["violations"] == 0
+    value = pairs_total / (ms_per_step / 1e3)
+
+    t0 = time.perf_counter()
+    run_steps(args.steps, False)
+    torch.cuda.synchronize()
+    eager_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    stages = []
+    for _ in range(3):
+        plan.enqueue(ptrs, Td)
+        stages.append(plan.stage_times())
+    st_ms = {k: float(np.mean([s[k] for s in stages])) for k in stages[0]}
+    log("stages ms", st_ms)
+    achieved = flops / (st_ms["screen"] / 1e3)
+    roofline = {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK["fp16"] / 1e12, "unit": "TFLOP/s",
+                "frac": achieved / PEAK["fp16"], "traffic": None,
+                "kernel": "batch_screen_kernel (one persistent launch per precision family)",
+                "algorithmic_flops_per_launch": flops}
+    result = {
+        "metric": METRIC, "value": value, "unit": "pair-evals/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp16" if all(p == "fp16" for p in precs) else "fp16+fp8",
+        "data": "synthetic",
+        "config": {"workload": "batched_uneven", "joins": joins, "rows": sum(ns), "pairs": pairs_total,
+                   "d": sorted(set(ds)), "n_range": [min(ns), max(ns)], "percentile": 0.5,
+                   "output": "per join sorted int32 (i, j), i < j, plus offsets", "launch": "hipGraph replay",
+                   "precision_by_width": {str(width_class(d)): p for d, p in zip(ds, precs)}},
+        "stages_ms": st_ms, "eager_ms_per_step": eager_ms, "roofline": roofline, "tuner": tune,
+        "plan": plan.describe(),
+        "pairs_out": stats["out"], "ambiguous": stats["ambiguous"], "candidates": stats["candidates"],
+        "ambiguous_fraction": stats["ambiguous"] / pairs_total,
+        "max_abs_d2_err_vs_fp64": stats["max_err"], "max_err_over_halfwidth": stats["max_ratio"],
+        "certificate_violations": stats["violations"],
+    }
+
+    if not args.no_plain:
+        # the plain FP64 GPU method, one join at a time, on the same device inputs
+        log("plain FP64 GPU method (per join)")
+        diffs = 0
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        plain_ms = 0.0
+        for k, (x, n, d) in enumerate(zip(xs, ns, ds)):
+            if n < 2:
+                continue
+            pp = Plan(n, None, d, "plain_fp64", "none", out_capacity=max(1024, got[k].shape[0] + 1024),
+                      amb_capacity=0)
+            a = time.perf_counter()
+            r = pp.run(x, None, float(Ts[k]), np.int32)
+            plain_ms += (time.perf_counter() - a) * 1e3
+            pk = r.numpy()
+            if pk.shape != got[k].shape or not np.array_equal(pk, got[k]):
+                ka = got[k][:, 0].astype(np.int64) * n + got[k][:, 1]
+                kb = pk[:, 0].astype(np.int64) * n + pk[:, 1]
+                diffs += int(np.setdiff1d(ka, kb).size + np.setdiff1d(kb, ka).size)
+            pp.close()
+        result["plain_gpu"] = {"value": pairs_total / (plain_ms / 1e3), "ms": plain_ms,
+                               "note": "one plain FP64 join per set, each synchronised (includes its readback)"}
+        result["speedup_vs_plain"] = value / (pairs_total / (plain_ms / 1e3))
+        result["pair_diffs_vs_plain_fp64"] = diffs
+        log(f"plain {plain_ms:.1f} ms, diffs {diffs}")
+
+    if not args.no_cpu:
+        from oracle import oracle
+        oracle.build()
+        threads = len(os.sched_getaffinity(0))
+        pick = list(range(0, joins, max(1, joins // 64)))[:64]
+        sp = sum(ns[k] * (ns[k] - 1) // 2 for k in pick)
+        t0 = time.perf_counter()
+        cnt = 0
+        for k in pick:
+            cnt += oracle.join(sets[k], None, float(Ts[k]), threads=threads, count_only=True)
+        dt = time.perf_counter() - t0
+        result["cpu_baseline"] = {"value": sp / dt, "unit": "pair-evals/s", "cores": threads, "kind": "port",
+                                  "sample": f"{len(pick)} whole joins (every {max(1, joins // 64)}th), "
+                                            f"{sp:.3e} pairs, {cnt} inside, {dt:.2f} s"}
     print(json.dumps(result), flush=True)
     return 0
 

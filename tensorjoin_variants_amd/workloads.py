@@ -129,22 +129,35 @@ def heavytail_fp8(nR: int = 262_144, nS: int = 262_144, d: int = 512, selectivit
                          seed=SEED_BASE + seed_offset))
 
 
+def _mixture_join(seed, n: int, d: int, pct: float, samples: int):
+    rng = np.random.default_rng(seed)
+    centres = (rng.standard_normal((4, d)) * 3.0).astype(np.float32)
+    X = rng.standard_normal((n, d), dtype=np.float32)
+    X += centres[rng.integers(0, 4, n)]
+    eps = percentile_eps(X, None, pct / 100.0, min(samples, n * (n - 1) // 2), rng)
+    return X, eps
+
+
 def batched_uneven(joins: int = 4096, nmin: int = 64, nmax: int = 16384, dims=(16, 64, 256),
-                   pct: float = 0.5, samples: int = 20_000, seed_offset: int = 5):
-    """Independent self-joins of log-uniform size, 4-cluster mixtures, eps per join."""
+                   pct: float = 0.5, samples: int = 20_000, seed_offset: int = 5, threads: int = 0):
+    """Independent self-joins of log-uniform size, 4-cluster mixtures (centres
+    ~ N(0, 3^2 I), points = centre + N(0, I)), eps per join at the ``pct``-th
+    percentile of sampled pair distances.  Join t draws from its own child
+    seed, so the sets do not depend on how many threads generate them."""
+    from concurrent.futures import ThreadPoolExecutor
+    import os
     rng = _rng(seed_offset)
     sizes = np.exp(rng.uniform(np.log(nmin), np.log(nmax + 1), joins)).astype(np.int64)
     sizes = np.clip(sizes, nmin, nmax)
     ds = rng.choice(np.asarray(dims), joins)
-    out = []
-    for t in range(joins):
-        n, d = int(sizes[t]), int(ds[t])
-        centres = rng.standard_normal((4, d)) * 3.0
-        X = (centres[rng.integers(0, 4, n)] + rng.standard_normal((n, d))).astype(np.float32)
-        eps = percentile_eps(X, None, pct / 100.0, min(samples, n * (n - 1) // 2), rng)
-        out.append(Workload(f"batched_uneven[{t}]", X, None, eps, eps * eps, np.int32,
-                            dict(n=n, d=d, percentile=pct)))
-    return out
+    seeds = np.random.SeedSequence(SEED_BASE + seed_offset).spawn(joins)
+    threads = threads or min(16, len(os.sched_getaffinity(0)))
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        made = list(ex.map(lambda t: _mixture_join(seeds[t], int(sizes[t]), int(ds[t]), pct, samples),
+                           range(joins)))
+    return [Workload(f"batched_uneven[{t}]", X, None, eps, eps * eps, np.int32,
+                     dict(n=int(sizes[t]), d=int(ds[t]), percentile=pct))
+            for t, (X, eps) in enumerate(made)]
 
 
 REGISTRY = {
