@@ -63,9 +63,10 @@ extern "C" {
  * extended with the screen error statistics. */
 #define TJ_ST_OUT 0           /* pairs written to the output (certified + refined) */
 #define TJ_ST_AMB 1           /* ambiguous pairs decided by the FP64 refinement */
-#define TJ_ST_CERT_ACCEPT 2   /* pairs certified inside by the screen */
+#define TJ_ST_CERT_ACCEPT 2   /* pairs certified inside (screen fast accept + per-pair interval) */
 #define TJ_ST_FP64_ACCEPT 3   /* ambiguous pairs the FP64 refinement accepted */
-#define TJ_ST_CANDIDATES 4    /* pairs the screen queued for the per-pair interval test (the amb_capacity queue) */
+#define TJ_ST_CANDIDATES 4    /* pairs the screen queued for the per-pair interval test (the amb_capacity queue):
+                                 the band between the fast reject and the fast accept (every survivor in audit mode) */
 #define TJ_ST_MAX_ERR 5       /* max |d2_screen - d2_fp64| over refined pairs (binary64 bits, real units) */
 #define TJ_ST_MAX_RATIO 6     /* max |d2_screen - d2_fp64| / certified half-width (binary64 bits) */
 #define TJ_ST_VIOLATIONS 7    /* refined pairs outside their certified interval (must be 0) */
@@ -125,7 +126,8 @@ int tj_plan_replay(tj_plan* plan, void* stream);
 /* --------------------------------------------------------- batches of joins
  * Many independent self-joins of uneven sizes and widths in one pipeline
  * with one persistent screen launch over the panels of every join, largest
- * first.  Replaces the host loop over 4,096-tile batches with a blocking
+ * first (one launch per kernel family present: FP16 widths 16/64/256, FP16
+ * widths 32/128, FP8).  Replaces the host loop over 4,096-tile batches with a blocking
  * count read after every stage (CEJ/src/run_g5_tensorjoin_public.py:
  * 377-381,410-535; TG/src/g2_operator_r1.py:45-57).
  * tj_batch_create (synchronous) fixes the joins' sizes n[t] (host array),
@@ -167,6 +169,10 @@ int tj_scale_scan(const float* x, int64_t n, int d, uint32_t* maxabs_bits, uint6
                   void* stream);
 int tj_params(const uint32_t* maxabs_bits, double T, int d, int precision, void* params,
               uint64_t* stats, void* stream);
+/* meta: 12 floats per row {nlo, nhi, H, E, L, t, v, e, w, a, 0, 0}: the
+ * bracket of ||x||^2, the bounds of ||hi||, ||residual|| and ||lo||, the
+ * screen's accumulator seed t, reject level v and accept terms w, a, and
+ * the FP8 scale exponents e (join units; see tj_params). */
 int tj_rowmeta(const float* x, int64_t n, int d, int precision, const void* params,
                void* q_hi, void* q_lo, float* meta, uint64_t* stats, void* stream);
 

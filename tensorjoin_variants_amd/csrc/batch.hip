@@ -42,13 +42,19 @@ struct BatchGroup {
 };
 
 // (precision, padded width) instantiations of the batched screen body, in
-// two families: a kernel holding FP16 and FP8 bodies together exceeds 256
-// VGPRs and spills, each family alone does not.
-#define TJ_BATCH_FP16(X) X(0, P_FP16, 16) X(1, P_FP16, 32) X(2, P_FP16, 64) X(3, P_FP16, 128) X(4, P_FP16, 256)
-#define TJ_BATCH_FP8(X) X(5, P_FP8, 64) X(6, P_FP8, 128) X(7, P_FP8, 256)
-#define TJ_BATCH_COMBOS(X) TJ_BATCH_FP16(X) TJ_BATCH_FP8(X)
-constexpr int N_FAMILIES = 2;
-inline int family_of(int prec) { return prec == P_FP8 ? 1 : 0; }
+// four families, one persistent kernel each: a kernel holding more bodies
+// exceeds 256 VGPRs and spills.  A batch whose joins all fall in one family
+// (widths 16/64/256 at FP16, say) is a single launch.
+#define TJ_BATCH_F16A(X) X(0, P_FP16, 16) X(2, P_FP16, 64) X(4, P_FP16, 256)
+#define TJ_BATCH_F16B(X) X(1, P_FP16, 32) X(3, P_FP16, 128)
+#define TJ_BATCH_F8A(X) X(5, P_FP8, 64) X(7, P_FP8, 256)
+#define TJ_BATCH_F8B(X) X(6, P_FP8, 128)
+#define TJ_BATCH_COMBOS(X) TJ_BATCH_F16A(X) TJ_BATCH_F16B(X) TJ_BATCH_F8A(X) TJ_BATCH_F8B(X)
+constexpr int N_FAMILIES = 4;
+inline int family_of(int prec, int dp) {
+    if (prec == P_FP8) return dp == 128 ? 3 : 2;
+    return (dp == 32 || dp == 128) ? 1 : 0;
+}
 
 constexpr int cmax(int a, int b) { return a > b ? a : b; }
 #define TJ_X_RING(ID, PR, DPV) cmax(tjscreen::ring_bytes<PR, DPV>(),
@@ -165,6 +171,7 @@ struct ScreenBatchArgs {
     tjscreen::CandOut co;
     uint64_t* cursor;
     int jbits;
+    int fast_accept;
 };
 
 // One persistent launch over the panels of every join of one precision
@@ -175,8 +182,8 @@ __global__ void __launch_bounds__(256, 2) batch_screen_kernel(const ScreenBatchA
     using namespace tjscreen;
     __shared__ __attribute__((aligned(16))) uint8_t sB[RING_MAX];
     __shared__ __attribute__((aligned(16))) float sNp[NSTAGE_MAX * 64];
+    __shared__ __attribute__((aligned(16))) float sW[NSTAGE_MAX * 64];
     __shared__ __attribute__((aligned(16))) uint32_t sSc[NSTAGE_MAX * 64];
-    __shared__ float sP[4][16 * 64];
     __shared__ uint64_t sCK[4][CCAP];
     __shared__ float sCP[4][CCAP];
     __shared__ int64_t sItem;
@@ -186,7 +193,7 @@ __global__ void __launch_bounds__(256, 2) batch_screen_kernel(const ScreenBatchA
     for (;;) {
         if (tid == 0) sItem = (int64_t)atomicAdd((unsigned long long*)A.cursor, 1ull);
         __syncthreads();
-        const int64_t item = sItem;
+        const int64_t item = (int64_t)uni64((uint64_t)sItem);  // uniform: scalar loads of the tables below
         __syncthreads();
         if (item >= A.n_items) break;
         const int2 it = A.items[item];
@@ -205,17 +212,27 @@ __global__ void __launch_bounds__(256, 2) batch_screen_kernel(const ScreenBatchA
         const int64_t ce = J.grow + J.n;
 #define X(ID, PR, DPV)                                                                                          \
     case ID:                                                                                                    \
-        screen_panel<PR, DPV, rb_for<PR, DPV>(), true>(io, A.co, r0, r0, ce, sB, sNp, sSc, sP[wave], sCK[wave], \
-                                                       sCP[wave], nC);                                          \
+        screen_panel<PR, DPV, rb_for<PR, DPV>(), true>(io, A.co, r0, r0, ce, sB, sNp, sW, sSc, sCK[wave], \
+                                                       sCP[wave], nC, A.fast_accept);                           \
         break;
         if constexpr (FAM == 0) {
             switch (uni(G.combo)) {
-                TJ_BATCH_FP16(X)
+                TJ_BATCH_F16A(X)
+                default: break;
+            }
+        } else if constexpr (FAM == 1) {
+            switch (uni(G.combo)) {
+                TJ_BATCH_F16B(X)
+                default: break;
+            }
+        } else if constexpr (FAM == 2) {
+            switch (uni(G.combo)) {
+                TJ_BATCH_F8A(X)
                 default: break;
             }
         } else {
             switch (uni(G.combo)) {
-                TJ_BATCH_FP8(X)
+                TJ_BATCH_F8B(X)
                 default: break;
             }
         }
@@ -287,8 +304,13 @@ using namespace tjbatch;
 namespace tj {
 int screen_batch_occupancy(int fam) {
     int blocks = 0;
-    const hipError_t e = fam == 0 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, batch_screen_kernel<0>, 256, 0)
-                                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, batch_screen_kernel<1>, 256, 0);
+    hipError_t e;
+    switch (fam) {
+        case 0: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, batch_screen_kernel<0>, 256, 0); break;
+        case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, batch_screen_kernel<1>, 256, 0); break;
+        case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, batch_screen_kernel<2>, 256, 0); break;
+        default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, batch_screen_kernel<3>, 256, 0); break;
+    }
     if (e != hipSuccess) return 1;
     return blocks > 0 ? blocks : 1;
 }
@@ -381,11 +403,18 @@ static int batch_enqueue(const BatchCall& c, hipStream_t s) {
         a.joins = b->d_joins;
         a.groups = b->d_groups;
         a.ws = ws;
-        a.co = tjscreen::CandOut{cand, aux, b->cand_cap, &stats[TJ_ST_CANDIDATES]};
+        a.co = tjscreen::CandOut{cand, aux, b->cand_cap, &stats[TJ_ST_CANDIDATES],
+                                 out_keys, b->out_cap, &stats[TJ_ST_OUT], &stats[TJ_ST_CERT_ACCEPT]};
         a.cursor = cursors + f;
         a.jbits = b->jbits;
-        if (f == 0) hipLaunchKernelGGL(batch_screen_kernel<0>, dim3((unsigned)b->fam_grid[f]), dim3(256), 0, s, a);
-        else hipLaunchKernelGGL(batch_screen_kernel<1>, dim3((unsigned)b->fam_grid[f]), dim3(256), 0, s, a);
+        a.fast_accept = (b->flags & TJ_FLAG_AUDIT) ? 0 : 1;
+        const dim3 grid((unsigned)b->fam_grid[f]);
+        switch (f) {
+            case 0: hipLaunchKernelGGL(batch_screen_kernel<0>, grid, dim3(256), 0, s, a); break;
+            case 1: hipLaunchKernelGGL(batch_screen_kernel<1>, grid, dim3(256), 0, s, a); break;
+            case 2: hipLaunchKernelGGL(batch_screen_kernel<2>, grid, dim3(256), 0, s, a); break;
+            default: hipLaunchKernelGGL(batch_screen_kernel<3>, grid, dim3(256), 0, s, a); break;
+        }
         TJ_HIP(hipGetLastError());
     }
     TJ_MARK(2);
@@ -475,6 +504,11 @@ int tj_batch_create(tj_batch** out, int joins, const int64_t* n, const int* d, c
     b->cand_cap = cand_capacity;
     b->flags = flags;
     b->n_groups = (int)G.size();
+    if (rows >= (1ll << 31)) {
+        delete b;
+        set_error("a batch holds fewer than 2^31 rows in total");
+        return TJ_ERR_UNSUPPORTED;
+    }
     b->ibits = bits_for(rows > 1 ? rows : 2);
     b->jbits = bits_for(max_n > 1 ? max_n : 2);
     b->keybits = b->ibits + b->jbits;
@@ -513,7 +547,7 @@ int tj_batch_create(tj_batch** out, int joins, const int64_t* n, const int* d, c
         for (int p = 0; p < panels; ++p) {
             const int64_t cols = (int64_t)J[t].n - (int64_t)p * bm;
             const double nblk = (double)((cols + 31) / 32);
-            items.push_back({family_of(J[t].prec), nblk * (16.0 * ks * rb + 48.0), t, p});
+            items.push_back({family_of(J[t].prec, J[t].dp), nblk * (16.0 * ks * rb + 48.0), t, p});
         }
     }
     std::stable_sort(items.begin(), items.end(), [](const Item& a, const Item& c) {
@@ -594,12 +628,16 @@ int tj_batch_describe(const tj_batch* b, char* buf, size_t len) {
     TJ_CHECK(b && buf, TJ_ERR_INVALID, "NULL argument");
     snprintf(buf, len,
              "{\"joins\":%d,\"rows\":%lld,\"max_n\":%lld,\"pairs\":%lld,\"groups\":%d,\"items\":%lld,"
-             "\"items_fp16\":%lld,\"items_fp8\":%lld,\"grid_fp16\":%d,\"grid_fp8\":%d,"
+             "\"items_fp16\":%lld,\"items_fp16b\":%lld,\"items_fp8\":%lld,\"items_fp8b\":%lld,\"launches\":%d,"
              "\"ibits\":%d,\"jbits\":%d,\"keybits\":%d,\"out_capacity\":%lld,\"cand_capacity\":%lld,"
              "\"workspace\":%zu}",
              b->nj, (long long)b->rows, (long long)b->max_n, (long long)b->pairs_total, b->n_groups,
              (long long)b->n_items, (long long)(b->fam_begin[1] - b->fam_begin[0]),
-             (long long)(b->fam_begin[2] - b->fam_begin[1]), b->fam_grid[0], b->fam_grid[1], b->ibits, b->jbits, b->keybits, (long long)b->out_cap,
+             (long long)(b->fam_begin[2] - b->fam_begin[1]), (long long)(b->fam_begin[3] - b->fam_begin[2]),
+             (long long)(b->fam_begin[4] - b->fam_begin[3]),
+             (int)(b->fam_begin[1] > b->fam_begin[0]) + (int)(b->fam_begin[2] > b->fam_begin[1]) +
+                 (int)(b->fam_begin[3] > b->fam_begin[2]) + (int)(b->fam_begin[4] > b->fam_begin[3]),
+             b->ibits, b->jbits, b->keybits, (long long)b->out_cap,
              (long long)b->cand_cap, b->total);
     return TJ_OK;
 }

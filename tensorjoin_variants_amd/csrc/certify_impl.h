@@ -80,7 +80,11 @@ __global__ void __launch_bounds__(256) certify_kernel(const Src src, const uint6
             const int cls = tjscreen::classify(p, pr.mi, pr.mj, P, &est, &hw);
             inside = cls == 1;
             if (cls == 1) n_cert++;
-            if (cls == 2 || (audit && cls == 1)) {
+            // The screen's fast accept (s >= a_i + w_j, with the largest a_i
+            // of its wave) never reaches this pass outside audit mode; in
+            // audit mode every pair it would accept is checked here.
+            const bool fast = (double)p >= (double)pr.mi.m2.y + (double)pr.mj.m2.x;
+            if (cls == 2 || (audit && (cls == 1 || fast))) {
                 const double d2 = d2_seq(pr.xi, pr.xj, pr.d);
                 const bool in64 = d2 <= P.T;
                 const double dj = d2 / P.scale_real;  // exact power-of-two scaling
@@ -90,7 +94,7 @@ __global__ void __launch_bounds__(256) certify_kernel(const Src src, const uint6
                 // the interval encloses the exact d2; allow the binary64 gap g
                 // and the binary32 rounding of the centre
                 const bool outside_interval = err > (double)hw + 2.0 * P.g64 * dj + 0x1p-23 * fabs((double)est);
-                if (outside_interval || (cls == 1 && !in64)) {
+                if (outside_interval || ((cls == 1 || fast) && !in64)) {
                     viol++;
                     stats[TJ_ST_VIOL_KEY] = key | (cls == 1 ? (1ull << 63) : 0ull);
                     stats[TJ_ST_VIOL_D2] = __builtin_bit_cast(uint64_t, dj);

@@ -156,10 +156,22 @@ __device__ inline void rowmeta_row(const float* __restrict__ xr, int64_t row, in
     double v = 0.5 * ((double)nlo - 2.0 * beta);
     v -= fabs(v) * 0x1p-50 + 0x1p-1000;
     const float v32 = f32_down(v);
+    // Fast accept, from the same model in the other direction:
+    //   d2 <= nhi_i + nhi_j - 2 s + 2 t_i + 2 gamma |t_i| + 2 alpha + 2 beta_i + 2 beta_j,
+    // so s >= a_i + w_j with
+    //   a_i >= (nhi_i + 2 t_i + 2 gamma |t_i| + 2 beta_i + 2 alpha - Tj_acc) / 2,
+    //   w_j >= (nhi_j + 2 beta_j) / 2
+    // certifies d2 <= Tj_acc (t_i is the binary32 seed actually used).
+    const double tt = (double)t32;
+    double a = 0.5 * ((double)nhi + 2.0 * tt + 2.0 * P.gamma_acc * fabs(tt) + 2.0 * beta + 2.0 * P.alpha - P.Tj_acc);
+    a += fabs(a) * 0x1p-50 + 0x1p-1000;
+    double w = 0.5 * ((double)nhi + 2.0 * beta);
+    w += fabs(w) * 0x1p-50 + 0x1p-1000;
     RowMeta m;
     m.m0 = make_float4(nlo, nhi, H, E);
     uint32_t packed = (uint32_t)(e_hi + 32768) | ((uint32_t)(e_lo + 32768) << 16);
     m.m1 = make_float4(L, t32, v32, __builtin_bit_cast(float, packed));
+    m.m2 = make_float4(f32_up(w), f32_up(a), 0.0f, 0.0f);
     meta[row] = m;
 }
 

@@ -127,26 +127,22 @@ __device__ inline int classify(float s, const RowMeta& mi, const RowMeta& mj, co
     return 2;
 }
 
-// Wait until at most n vector-memory operations of this wave are in flight
-// (n is a small compile-time bound of the prefetch pipeline).
-__device__ inline void wait_vm(int n) {
-    switch (n) {
-        case 0: __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory"); break;
-        case 1: __asm__ volatile("s_waitcnt vmcnt(1)" ::: "memory"); break;
-        case 2: __asm__ volatile("s_waitcnt vmcnt(2)" ::: "memory"); break;
-        case 3: __asm__ volatile("s_waitcnt vmcnt(3)" ::: "memory"); break;
-        case 4: __asm__ volatile("s_waitcnt vmcnt(4)" ::: "memory"); break;
-        case 5: __asm__ volatile("s_waitcnt vmcnt(5)" ::: "memory"); break;
-        case 6: __asm__ volatile("s_waitcnt vmcnt(6)" ::: "memory"); break;
-        case 7: __asm__ volatile("s_waitcnt vmcnt(7)" ::: "memory"); break;
-        case 8: __asm__ volatile("s_waitcnt vmcnt(8)" ::: "memory"); break;
-        case 9: __asm__ volatile("s_waitcnt vmcnt(9)" ::: "memory"); break;
-        case 10: __asm__ volatile("s_waitcnt vmcnt(10)" ::: "memory"); break;
-        case 11: __asm__ volatile("s_waitcnt vmcnt(11)" ::: "memory"); break;
-        case 12: __asm__ volatile("s_waitcnt vmcnt(12)" ::: "memory"); break;
-        case 13: __asm__ volatile("s_waitcnt vmcnt(13)" ::: "memory"); break;
-        case 14: __asm__ volatile("s_waitcnt vmcnt(14)" ::: "memory"); break;
-        default: __asm__ volatile("s_waitcnt vmcnt(15)" ::: "memory"); break;
+// Wait until at most N vector-memory operations of this wave are in flight.
+template <int N>
+__device__ inline void wait_vm_c() {
+    static_assert(N >= 0 && N <= 63, "vmcnt field is 6 bits");
+    __asm__ volatile("s_waitcnt vmcnt(%0)" ::"n"(N) : "memory");
+}
+// ... at most younger * P operations: the loads of the `younger` blocks
+// issued after the one about to be consumed (younger <= 4).
+template <int P>
+__device__ inline void wait_blocks(int younger) {
+    switch (younger) {
+        case 0: wait_vm_c<0>(); break;
+        case 1: wait_vm_c<P>(); break;
+        case 2: wait_vm_c<2 * P>(); break;
+        case 3: wait_vm_c<3 * P>(); break;
+        default: wait_vm_c<4 * P>(); break;
     }
 }
 // Workgroup barrier without the implicit vmcnt(0) of __syncthreads (which
@@ -171,6 +167,16 @@ __device__ inline void dma16(const void* src, uint32_t lds_wave_base) {
 }
 __device__ inline void dma4(const void* src, uint32_t lds_wave_base) {
     __asm__ volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dword %1, off" ::"s"(lds_wave_base), "v"(src) : "memory", "m0");
+}
+// Scalar-base forms: a wave-uniform 64-bit base in SGPRs plus a per-lane
+// 32-bit byte offset, so the per-block address math is scalar.
+__device__ inline void dma16s(uint64_t sbase, uint32_t voff, uint32_t lds_wave_base) {
+    __asm__ volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, %2" ::"s"(lds_wave_base), "v"(voff), "s"(sbase)
+                     : "memory", "m0");
+}
+__device__ inline void dma4s(uint64_t sbase, uint32_t voff, uint32_t lds_wave_base) {
+    __asm__ volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dword %1, %2" ::"s"(lds_wave_base), "v"(voff), "s"(sbase)
+                     : "memory", "m0");
 }
 
 template <int PREC, int DP>
@@ -202,25 +208,64 @@ struct PanelIO {
     int jbits;
 };
 
-// The global candidate queue (pair key, MFMA output s) and its cursor.
+// Where the screen's survivors go: certified-inside pairs (fast accept) as
+// keys straight to the output, the band as (key, s) to the candidate queue
+// of the certificate pass.  Entries beyond a capacity are counted, not kept.
 struct CandOut {
-    uint64_t* keys; float* aux; int64_t cap; uint64_t* counter;
+    uint64_t* keys; float* aux; int64_t cap; uint64_t* counter;  // band queue and its cursor
+    uint64_t* out; int64_t out_cap; uint64_t* out_counter;       // output keys and cursor
+    uint64_t* cert_counter;                                      // count of fast accepts
 };
 
-// Append the nC candidates staged in this wave's LDS buffers to the queue
-// (one atomic per flush; entries beyond the capacity are counted, not kept).
+constexpr uint64_t ACCEPT_BIT = 1ull << 63;  // staged-key flag: certified inside
+
+// Route the nC entries staged in this wave's LDS buffers: flagged keys to
+// the output, the others with their s to the band queue.  Two atomics per
+// flush.  Keys use at most 62 bits, so bit 63 is free for the flag.
 __device__ __forceinline__ void cand_flush(const CandOut& co, const uint64_t* sCK, const float* sCP, int& nC, int lane) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
-    uint64_t base = 0;
-    if (lane == 0) base = atomicAdd((unsigned long long*)co.counter, (unsigned long long)nC);
-    base = uni64(base);
-    for (int t = lane; t < nC; t += 64) {
-        const uint64_t idx = base + (uint64_t)t;
-        if (idx < (uint64_t)co.cap) {
-            co.keys[idx] = sCK[t];
-            co.aux[idx] = sCP[t];
+    nC = uni(nC);
+    int n_acc = 0, n_band = 0;
+    for (int t = lane; t - lane < nC; t += 64) {
+        const bool valid = t < nC;
+        const bool acc = valid && (sCK[t] & ACCEPT_BIT) != 0;
+        n_acc += __popcll(__ballot(acc));
+        n_band += __popcll(__ballot(valid && !acc));
+    }
+    // wave-uniform by construction; say so, or the atomics below become
+    // per-lane loops
+    n_acc = uni(n_acc);
+    n_band = uni(n_band);
+    uint64_t base_o = 0, base_b = 0;
+    if (lane == 0) {
+        if (n_acc) {
+            base_o = atomicAdd((unsigned long long*)co.out_counter, (unsigned long long)n_acc);
+            atomicAdd((unsigned long long*)co.cert_counter, (unsigned long long)n_acc);
         }
+        if (n_band) base_b = atomicAdd((unsigned long long*)co.counter, (unsigned long long)n_band);
+    }
+    base_o = uni64(base_o);
+    base_b = uni64(base_b);
+    for (int t = lane; t - lane < nC; t += 64) {
+        const bool valid = t < nC;
+        const uint64_t key = valid ? sCK[t] : 0;
+        const bool acc = valid && (key & ACCEPT_BIT) != 0;
+        const bool band = valid && !acc;
+        const uint64_t ma = __ballot(acc), mb = __ballot(band);
+        if (acc) {
+            const uint64_t idx = base_o + (uint64_t)lanes_below(ma);
+            if (idx < (uint64_t)co.out_cap) co.out[idx] = key & ~ACCEPT_BIT;
+        }
+        if (band) {
+            const uint64_t idx = base_b + (uint64_t)lanes_below(mb);
+            if (idx < (uint64_t)co.cap) {
+                co.keys[idx] = key;
+                co.aux[idx] = sCP[t];
+            }
+        }
+        base_o += (uint64_t)__popcll(ma);
+        base_b += (uint64_t)__popcll(mb);
     }
     // keep the in-order vmcnt accounting of the prefetch pipeline exact
     __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
@@ -233,12 +278,14 @@ template <int PREC, int DP>
 constexpr int ring_bytes() { return Geo<PREC, DP>::NSTAGE * Geo<PREC, DP>::BLKB; }
 
 // One work item: the panel of BM = 4 x RB x 32 stationary rows starting at r0
-// against columns [cb, ce), all 4 waves of the workgroup.  sB/sNp/sSc are the
-// workgroup's S ring (NSTAGE slots), sP/sCK/sCP this wave's staging buffers.
+// against columns [cb, ce), all 4 waves of the workgroup.  sB/sNp/sW/sSc are
+// the workgroup's S ring (NSTAGE slots: quantized columns, reject levels v,
+// accept terms w, FP8 scales), sCK/sCP this wave's staging buffers.  With
+// fast_accept == 0 every survivor goes to the band queue (audit mode).
 template <int PREC, int DP, int RB, bool SELF>
 __device__ __forceinline__ void screen_panel(const PanelIO& io, const CandOut& co, int64_t r0, int64_t cb, int64_t ce,
-                                             uint8_t* sB, float* sNp, uint32_t* sSc, float* sP, uint64_t* sCK,
-                                             float* sCP, int& nC) {
+                                             uint8_t* sB, float* sNp, float* sW, uint32_t* sSc, uint64_t* sCK,
+                                             float* sCP, int& nC, int fast_accept) {
     using C = Cfg<PREC>;
     using Gm = Geo<PREC, DP>;
     constexpr int NSTAGE = Gm::NSTAGE;
@@ -275,6 +322,16 @@ __device__ __forceinline__ void screen_panel(const PanelIO& io, const CandOut& c
             seed[rb][r] = ri < na ? io.ma[ri].m1.y : -INFINITY;
         }
     }
+    // largest accept term a_i over this wave's valid rows: s >= w_j + a_w
+    // then certifies every row of the wave against column j
+    float a_w = -INFINITY;
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) {
+        const int64_t row = rw0 + rb * 32 + c32;
+        if (row < na) a_w = fmaxf(a_w, io.ma[row].m2.y);
+    }
+    for (int o = 32; o > 0; o >>= 1) a_w = fmaxf(a_w, __shfl_xor(a_w, o));
+    if (!fast_accept) a_w = INFINITY;
 
     // Consume the stationary registers here, so that the compiler's wait
     // counts (blind to the asm DMA below) settle before the block loop
@@ -297,61 +354,114 @@ __device__ __forceinline__ void screen_panel(const PanelIO& io, const CandOut& c
 #pragma unroll
         for (int r = 0; r < 16; ++r) __asm__ volatile("" ::"v"(seed[rb][r]));
     }
+    __asm__ volatile("" ::"v"(a_w));
 
-    // ---- S blocks: direct global->LDS loads into an NSTAGE ring
-    auto issue = [&](int blk) {
-        const int slot = blk % NSTAGE;
-        const int64_t c0 = cb + (int64_t)blk * 32;
-        const uint32_t dst = __builtin_amdgcn_readfirstlane(lds_addr(sB + slot * Gm::BLKB));
+    // ---- S blocks: direct global->LDS loads into an NSTAGE ring.  The
+    // block's UNITS 16-byte units are spread evenly over the 4 waves (64
+    // lanes per load, or UNITS/4 lanes when UNITS < 256), and every wave
+    // also loads 8 columns of v, w (and the FP8 scales), so every wave
+    // issues the same P loads per block and waits on one uniform count.
+    // Unit p is 16-byte chunk q of column col; its byte offset from the
+    // block's first row, col * ROWB + q * 16, is fixed per lane, so each
+    // block only adds the scalar c0 * ROWB.  The tail block (columns past
+    // nb) recomputes offsets clamped to the last row.
+    constexpr bool WIDE = Gm::UNITS >= 256;
+    constexpr int NU = WIDE ? Gm::UNITS / 256 : 1;  // 16-byte loads per wave per block
+    constexpr int UPW = WIDE ? 64 : Gm::UNITS / 4;  // active lanes per load
+    static_assert(!WIDE || Gm::UNITS % 256 == 0, "unit count is a multiple of 256 or below it");
+    constexpr int NM = C::FP8 ? 3 : 2;              // per-column metadata loads per wave per block
+    constexpr int P = NU + NM;
+    const int wv = uni(wave);
+    uint32_t voff[NU];
 #pragma unroll
-        for (int u = 0; u * 256 < Gm::UNITS; ++u) {
-            const int pbase = wave * 64 + u * 256;  // this wave's 64 consecutive units
-            if (pbase < Gm::UNITS) {
-                const int p = pbase + lane;
-                const int a = p / (32 * Gm::R16);
-                const int rem = p % (32 * Gm::R16);
-                const int col = rem / Gm::R16;
-                const int qs = rem % Gm::R16;
-                const int q = qs ^ ((col >> Gm::FSH) & Gm::FMASK);
-                int64_t j = c0 + col;
-                if (j >= nb) j = nb - 1;
-                // a select, not io.qb[a]: a runtime index would put io on the stack
-                const uint8_t* src = (C::NT == 2 && a != 0) ? io.qb[1] : io.qb[0];
-                dma16(src + j * Gm::ROWB + q * 16, __builtin_amdgcn_readfirstlane(dst + (uint32_t)pbase * 16u));
+    for (int u = 0; u < NU; ++u) {
+        const int p = WIDE ? wv * 64 + u * 256 + lane : wv * UPW + (lane < UPW ? lane : 0);
+        const int rem = p % (32 * Gm::R16);
+        const int col = rem / Gm::R16;
+        const int q = (rem % Gm::R16) ^ ((col >> Gm::FSH) & Gm::FMASK);
+        voff[u] = (uint32_t)(col * Gm::ROWB + q * 16);
+    }
+    const uint64_t qb0 = uni64((uint64_t)(uintptr_t)io.qb[0]);
+    const uint64_t qb1 = uni64((uint64_t)(uintptr_t)io.qb[1]);
+    const uint64_t mbb = uni64((uint64_t)(uintptr_t)io.mb);
+    const uint32_t ldsB = uni((int)lds_addr(sB)), ldsNp = uni((int)lds_addr(sNp)), ldsW = uni((int)lds_addr(sW)),
+                   ldsSc = uni((int)lds_addr(sSc));
+    const int mcol = 8 * wv + (lane & 7);  // this lane's metadata column within a block (lanes 0-7)
+    auto issue = [&](int blk, int slot) {
+        const int64_t c0 = cb + (int64_t)blk * 32;
+        const uint32_t dst = ldsB + (uint32_t)(slot * Gm::BLKB);
+        const uint64_t mbase = mbb + (uint64_t)c0 * sizeof(RowMeta);
+        const uint32_t mo = (uint32_t)(slot * 64 + 8 * wv) * 4u;
+        if (c0 + 32 <= nb) {
+            // every column valid: the offsets are the per-lane constants
+            if (lane < UPW) {
+#pragma unroll
+                for (int u = 0; u < NU; ++u) {
+                    const int pbase = WIDE ? wv * 64 + u * 256 : wv * UPW;
+                    const int a = pbase / (32 * Gm::R16);  // hi or lo array: uniform
+                    const uint64_t base = ((C::NT == 2 && a != 0) ? qb1 : qb0) + (uint64_t)c0 * Gm::ROWB;
+                    dma16s(base, voff[u], dst + (uint32_t)pbase * 16u);
+                }
             }
-        }
-        if (wave == 0 && lane < 32) {
-            int64_t j = c0 + lane;
-            if (j >= nb) j = nb - 1;
-            dma4(&io.mb[j].m1.z, __builtin_amdgcn_readfirstlane(lds_addr(sNp + slot * 64)));
-            if constexpr (C::FP8) dma4(&io.mb[j].m1.w, __builtin_amdgcn_readfirstlane(lds_addr(sSc + slot * 64)));
+            if (lane < 8) {
+                const uint32_t off = (uint32_t)mcol * (uint32_t)sizeof(RowMeta);
+                dma4s(mbase, off + (uint32_t)offsetof(RowMeta, m1) + 8, ldsNp + mo);
+                dma4s(mbase, off + (uint32_t)offsetof(RowMeta, m2), ldsW + mo);
+                if constexpr (C::FP8) dma4s(mbase, off + (uint32_t)offsetof(RowMeta, m1) + 12, ldsSc + mo);
+            }
+        } else {
+            // tail block: columns past nb read the last row instead
+            const int last = (int)(nb - 1 - c0);
+            if (lane < UPW) {
+#pragma unroll
+                for (int u = 0; u < NU; ++u) {
+                    const int pbase = WIDE ? wv * 64 + u * 256 : wv * UPW;
+                    const int a = pbase / (32 * Gm::R16);
+                    const uint64_t base = ((C::NT == 2 && a != 0) ? qb1 : qb0) + (uint64_t)c0 * Gm::ROWB;
+                    const int col = ((pbase + lane) % (32 * Gm::R16)) / Gm::R16;
+                    const uint32_t off = col > last ? (uint32_t)(last * Gm::ROWB) + (voff[u] - (uint32_t)(col * Gm::ROWB))
+                                                    : voff[u];
+                    dma16s(base, off, dst + (uint32_t)pbase * 16u);
+                }
+            }
+            if (lane < 8) {
+                const uint32_t off = (uint32_t)(mcol > last ? last : mcol) * (uint32_t)sizeof(RowMeta);
+                dma4s(mbase, off + (uint32_t)offsetof(RowMeta, m1) + 8, ldsNp + mo);
+                dma4s(mbase, off + (uint32_t)offsetof(RowMeta, m2), ldsW + mo);
+                if constexpr (C::FP8) dma4s(mbase, off + (uint32_t)offsetof(RowMeta, m1) + 12, ldsSc + mo);
+            }
         }
     };
     const int pre = nblk < NSTAGE - 1 ? nblk : NSTAGE - 1;
-    for (int b = 0; b < pre; ++b) issue(b);
+    for (int b = 0; b < pre; ++b) issue(b, b);
+    int slot = 0, islot = pre == NSTAGE - 1 ? NSTAGE - 1 : 0;  // consumed / next issued ring slots
 
     for (int blk = 0; blk < nblk; ++blk) {
-        const int younger = (nblk - 1 - blk) < (NSTAGE - 2) ? (nblk - 1 - blk) : (NSTAGE - 2);
-        wait_vm(younger * Gm::LPB);
+        static_assert(NSTAGE - 2 <= 4, "wait_blocks covers up to 4 younger blocks");
+        if (blk + NSTAGE - 2 < nblk) wait_vm_c<P * (NSTAGE - 2)>();  // steady state
+        else wait_blocks<P>(nblk - 1 - blk);
         lds_barrier();
-        if (blk + NSTAGE - 1 < nblk) issue(blk + NSTAGE - 1);
-        const int slot = blk % NSTAGE;
+        if (blk + NSTAGE - 1 < nblk) {
+            issue(blk + NSTAGE - 1, islot);
+            islot = islot + 1 == NSTAGE ? 0 : islot + 1;
+        }
+        const int cslot = slot;
+        slot = slot + 1 == NSTAGE ? 0 : slot + 1;
         const int64_t c0 = cb + (int64_t)blk * 32;
         const int64_t jcol = c0 + c32;
         const bool skip = SELF && (c0 + 31 <= rw0);
         if (skip) continue;
-        float nj = INFINITY;
+        // v_j (+inf past the last column: never a survivor) and FP8 scales
+        float nj = sNp[cslot * 64 + c32];
         int sbh = 127, sbl = 127;
-        if (jcol < nb) {
-            nj = sNp[slot * 64 + c32];
-            if constexpr (C::FP8) {
-                const uint32_t pk = sSc[slot * 64 + c32];
-                sbh = (int)(pk & 0xFFFFu) - 32768 + 127;
-                sbl = (int)(pk >> 16) - 32768 + 127;
-            }
+        if constexpr (C::FP8) {
+            const uint32_t pk = sSc[cslot * 64 + c32];
+            sbh = (int)(pk & 0xFFFFu) - 32768 + 127;
+            sbl = (int)(pk >> 16) - 32768 + 127;
         }
+        if (c0 + 32 > nb && c32 > (int)(nb - 1 - c0)) nj = INFINITY;
         f32x16 acc[RB];
-        const uint8_t* bbase = sB + slot * Gm::BLKB;
+        const uint8_t* bbase = sB + cslot * Gm::BLKB;
 #pragma unroll
         for (int t = 0; t < Gm::KS; ++t) {
             Frag<C::CH> fb;
@@ -393,50 +503,63 @@ __device__ __forceinline__ void screen_panel(const PanelIO& io, const CandOut& c
 #pragma unroll
             for (int r = 1; r < 16; ++r) mx = fmaxf(mx, acc[rb][r]);
             if (__ballot(mx >= nj) == 0) continue;
-            // ---- rare: stage the candidates (pair key, s) for the certify
-            // pass.  A wave-uniform 16-bit mask names the registers that
-            // hold one; the block goes to LDS and only those are visited.
-            uint32_t rm = 0;
+            // ---- survivors: register r of this lane is row
+            // ibase + off(r), off(r) = 8 (r / 4) + r % 4, column jcol.  Each
+            // register with a survivor in any lane costs one ballot and one
+            // staged (key, s) per survivor; the key's bit 63 marks a fast
+            // accept, s >= u_j = w_j + a_w (rounded up).
+            const int64_t ibase = rw0 + rb * 32 + 4 * hh;
+            const uint64_t kb = ((uint64_t)(ibase + io.koff_a) << io.jbits) | (uint64_t)(jcol + io.koff_b);
+            // (columns past nb have nj = +inf: no survivor reads uj there)
+            const float u = sW[cslot * 64 + c32] + a_w;
+            const float uj = u + (fabsf(u) * 0x1p-22f + 0x1p-126f);  // >= w_j + a_w despite the rounding of the sum
+            // self-join: only i < j; off(r) < e <=> ibase + off(r) < jcol
+            int e = 64;
+            if (SELF && rw0 + rb * 32 + 31 >= c0) {
+                const int64_t df = jcol - ibase;
+                e = df < 0 ? 0 : (df > 64 ? 64 : (int)df);
+            }
 #pragma unroll
-            for (int r = 0; r < 16; ++r) rm |= (__ballot(acc[rb][r] >= nj) != 0 ? 1u : 0u) << r;
-#pragma unroll
-            for (int r = 0; r < 16; ++r) sP[r * 64 + lane] = acc[rb][r];
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            rm = uni((int)rm);
-            while (rm) {
-                const int r = __builtin_ctz(rm);
-                rm &= rm - 1;
-                const float sv = sP[r * 64 + lane];
-                const int rloc = rb * 32 + 8 * (r >> 2) + 4 * hh + (r & 3);
-                const int64_t i = rw0 + rloc;
-                const bool cand = sv >= nj && i < na && jcol < nb && (!SELF || i < jcol);
+            for (int r = 0; r < 16; ++r) {
+                const int off = 8 * (r >> 2) + (r & 3);
+                const float sv = acc[rb][r];
+                const bool cand = sv >= nj && off < e;
                 const uint64_t m = __ballot(cand);
                 if (m == 0) continue;
-                const int k = uni(__popcll(m));
+                const int k = __popcll(m);
+                nC = uni(nC);  // wave-uniform: keeps the capacity test a scalar branch
                 if (nC + k > CCAP) cand_flush(co, sCK, sCP, nC, lane);
                 if (cand) {
                     const int pos = nC + lanes_below(m);
-                    sCK[pos] = key_pack((uint64_t)(i + io.koff_a), (uint64_t)(jcol + io.koff_b), io.jbits);
+                    uint64_t key = kb + ((uint64_t)off << io.jbits);
+                    if (sv >= uj) key |= ACCEPT_BIT;
+                    sCK[pos] = key;
                     sCP[pos] = sv;
                 }
-                nC += k;
+                nC = uni(nC + k);
             }
-            __builtin_amdgcn_wave_barrier();
         }
     }
 }
 
 // The single-join screen: a persistent grid takes (panel, column split)
 // items from an atomic cursor until the item space is exhausted.
+// Workgroups per CU the compiler should plan registers for: two, unless the
+// stationary fragments alone take 128 VGPRs (then one, with up to 512).
+template <int PREC, int DP, int RB>
+constexpr int min_blocks() {
+    using C = Cfg<PREC>;
+    return C::NT * RB * (DP / C::KSTEP) * (C::CH / 4) >= 128 ? 1 : 2;
+}
+
 template <int PREC, int DP, int RB, bool SELF>
-__global__ void __launch_bounds__(256, 2) screen_kernel(const tj::ScreenArgs A) {
+__global__ void __launch_bounds__(256, (min_blocks<PREC, DP, RB>())) screen_kernel(const tj::ScreenArgs A) {
     using Gm = Geo<PREC, DP>;
     constexpr int BM = 4 * RB * 32;
     __shared__ __attribute__((aligned(16))) uint8_t sB[Gm::NSTAGE * Gm::BLKB];
     __shared__ __attribute__((aligned(16))) float sNp[Gm::NSTAGE * 64];
+    __shared__ __attribute__((aligned(16))) float sW[Gm::NSTAGE * 64];
     __shared__ __attribute__((aligned(16))) uint32_t sSc[Gm::NSTAGE * 64];
-    __shared__ float sP[4][16 * 64];
     __shared__ uint64_t sCK[4][CCAP];
     __shared__ float sCP[4][CCAP];
     __shared__ int sItem;
@@ -449,13 +572,14 @@ __global__ void __launch_bounds__(256, 2) screen_kernel(const tj::ScreenArgs A) 
     io.qb[0] = (const uint8_t*)A.qb_hi; io.qb[1] = (const uint8_t*)A.qb_lo; io.mb = A.mb; io.nb = A.nb;
     io.koff_a = 0; io.koff_b = 0;
     io.jbits = A.jbits;
-    const CandOut co{A.amb_keys, A.amb_aux, A.amb_cap, &A.stats[TJ_ST_CANDIDATES]};
+    const CandOut co{A.amb_keys, A.amb_aux, A.amb_cap, &A.stats[TJ_ST_CANDIDATES],
+                     A.out_keys, A.out_cap, &A.stats[TJ_ST_OUT], &A.stats[TJ_ST_CERT_ACCEPT]};
     int nC = 0;
 
     for (;;) {
         if (tid == 0) sItem = atomicAdd((unsigned long long*)&A.stats[TJ_ST_WORK], 1ull);
         __syncthreads();
-        const int item = sItem;
+        const int item = uni(sItem);
         __syncthreads();
         if (item >= total_items) break;
         const int panel = item % A.panels;
@@ -465,7 +589,7 @@ __global__ void __launch_bounds__(256, 2) screen_kernel(const tj::ScreenArgs A) 
         if (SELF) { cb = r0; ce = A.nb; }
         else { cb = (int64_t)split * A.cols_per_split; ce = cb + A.cols_per_split; if (ce > A.nb) ce = A.nb; }
         if (cb >= ce) continue;
-        screen_panel<PREC, DP, RB, SELF>(io, co, r0, cb, ce, sB, sNp, sSc, sP[wave], sCK[wave], sCP[wave], nC);
+        screen_panel<PREC, DP, RB, SELF>(io, co, r0, cb, ce, sB, sNp, sW, sSc, sCK[wave], sCP[wave], nC, !A.audit);
     }
     if (nC) cand_flush(co, sCK[wave], sCP[wave], nC, lane);
 }

@@ -152,14 +152,17 @@ __device__ inline int fp8_row_exp(double maxabs) {
 }
 
 // ------------------------------------------------------------- row meta
-// Two float4 per row:
+// Three float4 per row:
 //   m0 = {nlo, nhi, H, E}     ||x_join||^2 bracket, ||hi||, ||residual|| upper bounds
 //   m1 = {L, t, v, e}         ||lo|| bound (hi/lo split), accumulator seed t of the
 //                             row as the stationary side, reject level v of the row
 //                             as the streamed side (s = t_i + p_ij < v_j rejects),
 //                             FP8 scale exponents packed as int bits
 //                             (e_hi in bits 0-15, e_lo in 16-31, biased by 32768).
-struct RowMeta { float4 m0, m1; };
+//   m2 = {w, a, 0, 0}         accept terms: s = t_i + p_ij >= a_i + w_j certifies
+//                             the pair inside (a of the row as the stationary
+//                             side, w as the streamed side).
+struct RowMeta { float4 m0, m1, m2; };
 
 // Sequential binary64 difference form, k ascending, no FMA (every file is
 // compiled with -ffp-contract=off): the predicate of

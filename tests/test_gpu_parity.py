@@ -31,15 +31,20 @@ def oracle_pairs(oracle, R, S, T):
 
 
 def run_check(tj, oracle, R, S, T, prec, method="certified", audit=True, name=""):
-    say(name, prec, method, R.shape, None if S is None else S.shape, T)
-    got, st = tj.join(R, S, T=T, method=method, precision=prec, out_dtype=np.int64, audit=audit,
-                      return_stats=True)
+    """The production path (screen fast accept on) and, for certified joins,
+    the audit path (every certified or fast-acceptable pair re-checked in
+    FP64), both bit-exact against the oracle."""
     want = oracle_pairs(oracle, R, S, T)
-    assert got.shape == want.shape, (name, prec, got.shape, want.shape, st)
-    assert np.array_equal(got, want), (name, prec)
-    if method == "certified":
-        assert st["violations"] == 0, st
-        assert st["cert_accept"] + st["fp64_accept"] == st["out"], st
+    modes = [False, True] if (method == "certified" and audit) else [audit]
+    for mode in modes:
+        say(name, prec, method, "audit" if mode else "fast", R.shape, None if S is None else S.shape, T)
+        got, st = tj.join(R, S, T=T, method=method, precision=prec, out_dtype=np.int64, audit=mode,
+                          return_stats=True)
+        assert got.shape == want.shape, (name, prec, mode, got.shape, want.shape, st)
+        assert np.array_equal(got, want), (name, prec, mode)
+        if method == "certified":
+            assert st["violations"] == 0, st
+            assert st["cert_accept"] + st["fp64_accept"] == st["out"], st
     return st
 
 
