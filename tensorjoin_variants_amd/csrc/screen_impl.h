@@ -29,6 +29,7 @@
 //  * Compaction: wave ballot + mbcnt prefix into a per-wave LDS staging
 //    buffer, flushed to HBM with one atomic per 256 pairs.
 #pragma once
+#include <type_traits>
 #include "tj_internal.h"
 
 namespace tjscreen {
@@ -293,7 +294,7 @@ __device__ __forceinline__ void screen_panel(const PanelIO& io, const CandOut& c
     const int lane = tid & 63, wave = tid >> 6;
     const int hh = lane >> 5, c32 = lane & 31;
     const int64_t na = io.na, nb = io.nb;
-    const int64_t rw0 = r0 + (int64_t)wave * RB * 32;
+    const int64_t rw0 = r0 + (int64_t)uni(wave) * RB * 32;  // uniform: tile tests stay scalar
     const int nblk = (int)((ce - cb + 31) / 32);
 
     // ---- stationary fragments, thresholds and scales (VGPR resident)
@@ -513,37 +514,43 @@ __device__ __forceinline__ void screen_panel(const PanelIO& io, const CandOut& c
             // (columns past nb have nj = +inf: no survivor reads uj there)
             const float u = sW[cslot * 64 + c32] + a_w;
             const float uj = u + (fabsf(u) * 0x1p-22f + 0x1p-126f);  // >= w_j + a_w despite the rounding of the sum
-            // self-join: only i < j; off(r) < e <=> ibase + off(r) < jcol
-            int e = 64;
-            if (SELF && rw0 + rb * 32 + 31 >= c0) {
-                const int64_t df = jcol - ibase;
-                e = df < 0 ? 0 : (df > 64 ? 64 : (int)df);
-            }
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int off = 8 * (r >> 2) + (r & 3);
-                const float sv = acc[rb][r];
-                const bool cand = sv >= nj && off < e;
-                const uint64_t m = __ballot(cand);
-                if (m == 0) continue;
-                const int k = __popcll(m);
-                nC = uni(nC);  // wave-uniform: keeps the capacity test a scalar branch
-                if (nC + k > CCAP) cand_flush(co, sCK, sCP, nC, lane);
-                if (cand) {
-                    const int pos = nC + lanes_below(m);
-                    uint64_t key = kb + ((uint64_t)off << io.jbits);
-                    if (sv >= uj) key |= ACCEPT_BIT;
-                    sCK[pos] = key;
-                    sCP[pos] = sv;
+            // Stage register r's survivors; DIAG adds the self-join's i < j:
+            // off(r) < e <=> ibase + off(r) < jcol.  Ballots of single
+            // compares are the compare masks themselves.
+            auto stage = [&](auto diag) {
+                constexpr bool DIAG = decltype(diag)::value;
+                int e = 64;
+                if constexpr (DIAG) {
+                    // jcol - ibase = (c0 - first row of the tile) + c32 - 4 hh, small here
+                    const int df = (int)(c0 - (rw0 + rb * 32)) + c32 - 4 * hh;
+                    e = df < 0 ? 0 : (df > 64 ? 64 : df);
                 }
-                nC = uni(nC + k);
-            }
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const int off = 8 * (r >> 2) + (r & 3);
+                    const float sv = acc[rb][r];
+                    uint64_t m = __builtin_amdgcn_ballot_w64(sv >= nj);
+                    if constexpr (DIAG) m &= __builtin_amdgcn_ballot_w64(off < e);
+                    if (m == 0) continue;
+                    const int k = __popcll(m);
+                    nC = uni(nC);  // wave-uniform: keeps the capacity test a scalar branch
+                    if (nC + k > CCAP) cand_flush(co, sCK, sCP, nC, lane);
+                    if ((m >> lane) & 1) {
+                        const int pos = nC + lanes_below(m);
+                        uint64_t key = kb + ((uint64_t)off << io.jbits);
+                        if (sv >= uj) key |= ACCEPT_BIT;
+                        sCK[pos] = key;
+                        sCP[pos] = sv;
+                    }
+                    nC = uni(nC + k);
+                }
+            };
+            if (SELF && rw0 + rb * 32 + 31 >= c0) stage(std::true_type{});
+            else stage(std::false_type{});
         }
     }
 }
 
-// The single-join screen: a persistent grid takes (panel, column split)
-// items from an atomic cursor until the item space is exhausted.
 // Workgroups per CU the compiler should plan registers for: two, unless the
 // stationary fragments alone take 128 VGPRs (then one, with up to 512).
 template <int PREC, int DP, int RB>
