@@ -461,6 +461,7 @@ __device__ __forceinline__ void screen_panel(const PanelIO& io, const CandOut& c
             sbl = (int)(pk >> 16) - 32768 + 127;
         }
         if (c0 + 32 > nb && c32 > (int)(nb - 1 - c0)) nj = INFINITY;
+        const float wj = sW[cslot * 64 + c32];  // read with v_j: the survivor path then has it in a register
         f32x16 acc[RB];
         const uint8_t* bbase = sB + cslot * Gm::BLKB;
 #pragma unroll
@@ -512,7 +513,7 @@ __device__ __forceinline__ void screen_panel(const PanelIO& io, const CandOut& c
             const int64_t ibase = rw0 + rb * 32 + 4 * hh;
             const uint64_t kb = ((uint64_t)(ibase + io.koff_a) << io.jbits) | (uint64_t)(jcol + io.koff_b);
             // (columns past nb have nj = +inf: no survivor reads uj there)
-            const float u = sW[cslot * 64 + c32] + a_w;
+            const float u = wj + a_w;
             const float uj = u + (fabsf(u) * 0x1p-22f + 0x1p-126f);  // >= w_j + a_w despite the rounding of the sum
             // Stage register r's survivors; DIAG adds the self-join's i < j:
             // off(r) < e <=> ibase + off(r) < jcol.  Ballots of single
@@ -529,13 +530,14 @@ __device__ __forceinline__ void screen_panel(const PanelIO& io, const CandOut& c
                 for (int r = 0; r < 16; ++r) {
                     const int off = 8 * (r >> 2) + (r & 3);
                     const float sv = acc[rb][r];
+                    const bool mine = DIAG ? (sv >= nj && off < e) : (sv >= nj);
                     uint64_t m = __builtin_amdgcn_ballot_w64(sv >= nj);
                     if constexpr (DIAG) m &= __builtin_amdgcn_ballot_w64(off < e);
                     if (m == 0) continue;
                     const int k = __popcll(m);
                     nC = uni(nC);  // wave-uniform: keeps the capacity test a scalar branch
-                    if (nC + k > CCAP) cand_flush(co, sCK, sCP, nC, lane);
-                    if ((m >> lane) & 1) {
+                    if (__builtin_expect(nC + k > CCAP, 0)) cand_flush(co, sCK, sCP, nC, lane);  // out of line
+                    if (mine) {  // this lane's bit of m (the same compares)
                         const int pos = nC + lanes_below(m);
                         uint64_t key = kb + ((uint64_t)off << io.jbits);
                         if (sv >= uj) key |= ACCEPT_BIT;
