@@ -180,6 +180,16 @@ __device__ inline void dma4s(uint64_t sbase, uint32_t voff, uint32_t lds_wave_ba
                      : "memory", "m0");
 }
 
+// max of the 16 accumulator registers of a lane as a 3-level tree of max3
+// (a chain would be 8 dependent instructions long)
+__device__ inline float max16(const f32x16& a) {
+    const float m0 = fmaxf(fmaxf(a[0], a[1]), a[2]), m1 = fmaxf(fmaxf(a[3], a[4]), a[5]);
+    const float m2 = fmaxf(fmaxf(a[6], a[7]), a[8]), m3 = fmaxf(fmaxf(a[9], a[10]), a[11]);
+    const float m4 = fmaxf(fmaxf(a[12], a[13]), a[14]);
+    const float n0 = fmaxf(fmaxf(m0, m1), m2), n1 = fmaxf(fmaxf(m3, m4), a[15]);
+    return fmaxf(n0, n1);
+}
+
 template <int PREC, int DP>
 struct Geo {
     using C = Cfg<PREC>;
@@ -501,9 +511,7 @@ __device__ __forceinline__ void screen_panel(const PanelIO& io, const CandOut& c
         // max over the 16 rows of a lane, one compare, one ballot per block
 #pragma unroll
         for (int rb = 0; rb < RB; ++rb) {
-            float mx = acc[rb][0];
-#pragma unroll
-            for (int r = 1; r < 16; ++r) mx = fmaxf(mx, acc[rb][r]);
+            const float mx = max16(acc[rb]);
             if (__ballot(mx >= nj) == 0) continue;
             // ---- survivors: register r of this lane is row
             // ibase + off(r), off(r) = 8 (r / 4) + r % 4, column jcol.  Each
@@ -526,18 +534,25 @@ __device__ __forceinline__ void screen_panel(const PanelIO& io, const CandOut& c
                     const int df = (int)(c0 - (rw0 + rb * 32)) + c32 - 4 * hh;
                     e = df < 0 ? 0 : (df > 64 ? 64 : df);
                 }
+                // all 16 compare masks first (independent compares pipeline),
+                // then the scalar tests and the writes
+                uint64_t ms[16];
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    ms[r] = __builtin_amdgcn_ballot_w64(acc[rb][r] >= nj);
+                    if constexpr (DIAG) ms[r] &= __builtin_amdgcn_ballot_w64(8 * (r >> 2) + (r & 3) < e);
+                }
 #pragma unroll
                 for (int r = 0; r < 16; ++r) {
                     const int off = 8 * (r >> 2) + (r & 3);
                     const float sv = acc[rb][r];
-                    const bool mine = DIAG ? (sv >= nj && off < e) : (sv >= nj);
-                    uint64_t m = __builtin_amdgcn_ballot_w64(sv >= nj);
-                    if constexpr (DIAG) m &= __builtin_amdgcn_ballot_w64(off < e);
+                    const uint64_t m = ms[r];
                     if (m == 0) continue;
+                    const bool mine = (m >> lane) & 1;
                     const int k = __popcll(m);
                     nC = uni(nC);  // wave-uniform: keeps the capacity test a scalar branch
                     if (__builtin_expect(nC + k > CCAP, 0)) cand_flush(co, sCK, sCP, nC, lane);  // out of line
-                    if (mine) {  // this lane's bit of m (the same compares)
+                    if (mine) {
                         const int pos = nC + lanes_below(m);
                         uint64_t key = kb + ((uint64_t)off << io.jbits);
                         if (sv >= uj) key |= ACCEPT_BIT;
