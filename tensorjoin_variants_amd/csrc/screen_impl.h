@@ -568,12 +568,14 @@ __device__ __forceinline__ void screen_panel(const PanelIO& io, const CandOut& c
     }
 }
 
-// Workgroups per CU the compiler should plan registers for: two, unless the
-// stationary fragments alone take 128 VGPRs (then one, with up to 512).
+// Workgroups per CU the compiler should plan registers for: two; one when
+// the stationary fragments alone take 128 VGPRs (up to 512 each); three for
+// single-tile waves with at most 64 fragment VGPRs (up to 168 each).
 template <int PREC, int DP, int RB>
 constexpr int min_blocks() {
     using C = Cfg<PREC>;
-    return C::NT * RB * (DP / C::KSTEP) * (C::CH / 4) >= 128 ? 1 : 2;
+    constexpr int frag = C::NT * RB * (DP / C::KSTEP) * (C::CH / 4);
+    return frag >= 128 ? 1 : (RB == 1 && frag <= 64 ? 3 : 2);
 }
 
 template <int PREC, int DP, int RB, bool SELF>
