@@ -218,11 +218,16 @@ int tj_plan_create(tj_plan** out, int64_t nR, int64_t nS, int d, int method, int
             p->col_splits = 1;
             p->cols_per_split = (int)std::min<int64_t>(p->nS, 1ll << 30);
         } else {
-            // enough items to balance the persistent grid, wide enough column
-            // ranges to amortise loading the stationary rows
-            int64_t want = (4ll * maxg + std::max(p->panels, 1) - 1) / std::max(p->panels, 1);
-            int64_t by_width = (p->nS + 4095) / 4096;
-            int64_t cs = std::max<int64_t>(1, std::min(want, by_width));
+            // Column splits of about 4 MiB of quantized S (one XCD's L2),
+            // items ordered split-major: the workgroups running at any time
+            // sweep the same chunk of S, which then stays in L2, instead of
+            // each panel streaming all of S from memory.  Also enough items
+            // to balance the persistent grid when there are few panels.
+            const int64_t row_bytes = (int64_t)p->dp * es * (x3 ? 2 : 1);
+            const int64_t by_l2 = std::max<int64_t>(1, (p->nS * row_bytes + (4ll << 20) - 1) / (4ll << 20));
+            const int64_t want = (4ll * maxg + std::max(p->panels, 1) - 1) / std::max(p->panels, 1);
+            const int64_t by_width = (p->nS + 1023) / 1024;  // splits of at least 1024 columns
+            const int64_t cs = std::max<int64_t>(1, std::min(std::max(by_l2, want), by_width));
             int64_t per = ((p->nS + cs - 1) / cs + 31) / 32 * 32;
             p->cols_per_split = (int)std::max<int64_t>(per, 32);
             p->col_splits = (int)((p->nS + p->cols_per_split - 1) / p->cols_per_split);

@@ -250,3 +250,17 @@ def test_clusters_selfjoin_full(tj, oracle):
         assert st["violations"] == 0
     say("clusters_selfjoin plain")
     assert np.array_equal(tj.join(w.R, None, T=w.T, method="plain_fp64").astype(np.int64), want)
+
+
+@pytest.mark.parametrize("prec", ["fp16", "fp8"])
+def test_certified_column_splits(tj, oracle, prec):
+    """R x S wide enough that the screen splits S into L2-sized column ranges."""
+    rng = np.random.default_rng(77)
+    R = rng.standard_normal((3000, 256)).astype(np.float32)
+    S = rng.standard_normal((20000, 256)).astype(np.float32)
+    T = sample_T(oracle, R, S, 0.002)
+    from tensorjoin_variants_amd.join import Plan
+    desc = Plan(3000, 20000, 256, "certified", prec, 1024, 1024).describe()
+    say("column splits", prec, desc["col_splits"], desc["cols_per_split"])
+    assert desc["col_splits"] >= 2
+    run_check(tj, oracle, R, S, T, prec, name="column splits")
