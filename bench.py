@@ -43,6 +43,17 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
+def measured_traffic(workload: str):
+    """HBM bytes per launch of the dominant kernel, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE runs (scaled by a copy of known size);
+    None when this workload has no such measurement."""
+    try:
+        t = json.loads((ROOT / "profiles" / "traffic.json").read_text())[workload]
+    except (OSError, KeyError, ValueError):
+        return None, None
+    return float(t["traffic_bytes"]), f"{t['profile']} (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE on {t['kernel']})"
+
+
 def cpu_baseline(w, seconds_target: float = 12.0) -> dict:
     """Oracle (sequential FP64, OpenMP) on the first k rows of R against all of S."""
     from oracle import oracle
@@ -184,6 +195,7 @@ def main() -> int:
         roofline = {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
                     "frac": achieved / peak, "traffic": None, "kernel": "screen_kernel",
                     "algorithmic_flops_per_launch": flops}
+    roofline["traffic"], roofline["traffic_source"] = measured_traffic(args.workload)
 
     result = {
         "metric": METRIC, "value": value, "unit": "pair-evals/s", "n_gpus": 1, "steps": args.steps,
@@ -299,6 +311,7 @@ def bench_batched(args) -> int:
                 "frac": achieved / PEAK["fp16"], "traffic": None,
                 "kernel": "batch_screen_kernel (one persistent launch per precision family)",
                 "algorithmic_flops_per_launch": flops}
+    roofline["traffic"], roofline["traffic_source"] = measured_traffic("batched_uneven")
     result = {
         "metric": METRIC, "value": value, "unit": "pair-evals/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
