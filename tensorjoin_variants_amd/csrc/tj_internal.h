@@ -171,6 +171,25 @@ __device__ inline double d2_seq(const float* __restrict__ x, const float* __rest
     double s = 0.0;
     int k = 0;
     if (((((uintptr_t)x) | ((uintptr_t)y)) & 15) == 0) {
+        // 32 coordinates (one 128-byte line of each row) are loaded before
+        // any is used: a lane walking its own rows then takes each line from
+        // L2 once instead of once per 16-byte piece
+        for (; k + 32 <= d; k += 32) {
+            float4 a[8], b[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                a[q] = *(const float4*)(x + k + 4 * q);
+                b[q] = *(const float4*)(y + k + 4 * q);
+            }
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                double t;
+                t = (double)a[q].x - (double)b[q].x; s = s + t * t;
+                t = (double)a[q].y - (double)b[q].y; s = s + t * t;
+                t = (double)a[q].z - (double)b[q].z; s = s + t * t;
+                t = (double)a[q].w - (double)b[q].w; s = s + t * t;
+            }
+        }
         for (; k + 4 <= d; k += 4) {
             const float4 a = *(const float4*)(x + k);
             const float4 b = *(const float4*)(y + k);
