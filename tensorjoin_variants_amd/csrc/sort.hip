@@ -87,42 +87,93 @@ __device__ inline int lanes_below(uint64_t m) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
 }
 
+// Stable scatter of one block's chunk, in tiles of TILE = SB x KPT keys.
+// Tile order is key order (striped: key base + k SB + tid is slot k of
+// thread tid).  Per slot, each wave ranks its 64 keys among equal digits by
+// an 8-ballot multisplit; one pass over (slot, wave) per digit turns the
+// per-wave counts into offsets; the tile is counting-sorted into LDS and
+// written out as contiguous runs per digit (coalesced stores).
+constexpr int KPT = 8;
+constexpr int TILE = SB * KPT;
+
 __global__ void __launch_bounds__(SB) scatter_kernel(const uint64_t* __restrict__ src, uint64_t* __restrict__ dst,
                                                      const uint64_t* count, int64_t cap, int shift,
                                                      const uint32_t* __restrict__ offs, const uint32_t* __restrict__ bases,
                                                      int nb) {
-    __shared__ uint32_t run[256];
-    __shared__ uint32_t wcnt[4][256];
+    __shared__ uint32_t run[256];         // next global position per digit
+    __shared__ uint32_t tstart[256];      // first tile position per digit
+    __shared__ uint32_t ttot[256];        // keys per digit in the tile
+    __shared__ uint32_t wcnt[KPT][4][256];  // per (slot, wave) digit counts, then offsets
+    __shared__ uint64_t sorted[TILE];
     const int tid = threadIdx.x, wave = tid >> 6;
     const int64_t n = clamp_count(count, cap);
     const int64_t chunk = ((n + nb - 1) / nb + SB - 1) / SB * SB;
     const int64_t b0 = (int64_t)blockIdx.x * chunk;
     const int64_t b1 = b0 + chunk < n ? b0 + chunk : n;
     run[tid] = bases[tid] + offs[(int64_t)tid * nb + blockIdx.x];
-    for (int64_t base = b0; base < b1; base += SB) {
-        wcnt[0][tid] = 0; wcnt[1][tid] = 0; wcnt[2][tid] = 0; wcnt[3][tid] = 0;
+    for (int64_t base = b0; base < b1; base += TILE) {
+        for (int e = tid; e < KPT * 4 * 256; e += SB) (&wcnt[0][0][0])[e] = 0;
         __syncthreads();
-        const int64_t t = base + tid;
-        const bool valid = t < b1;
-        uint64_t key = valid ? src[t] : 0;
-        const uint32_t dg = (uint32_t)(key >> shift) & 255u;
-        uint64_t peers = __ballot(valid);
+        uint64_t key[KPT];
+        uint32_t dg[KPT];
+        int rank[KPT];
+        bool valid[KPT];
 #pragma unroll
-        for (int b = 0; b < 8; ++b) {
-            const bool bit = (dg >> b) & 1u;
-            const uint64_t bm = __ballot(bit);
-            peers &= bit ? bm : ~bm;
+        for (int k = 0; k < KPT; ++k) {
+            const int64_t t = base + (int64_t)k * SB + tid;
+            valid[k] = t < b1;
+            key[k] = valid[k] ? src[t] : 0;
+            dg[k] = (uint32_t)(key[k] >> shift) & 255u;
         }
-        const int rank = lanes_below(peers);
-        if (valid && rank == 0) wcnt[wave][dg] = (uint32_t)__popcll(peers);
-        __syncthreads();
-        if (valid) {
-            uint32_t pos = run[dg] + (uint32_t)rank;
-            for (int w = 0; w < wave; ++w) pos += wcnt[w][dg];
-            dst[pos] = key;
+#pragma unroll
+        for (int k = 0; k < KPT; ++k) {
+            uint64_t peers = __ballot(valid[k]);
+#pragma unroll
+            for (int b = 0; b < 8; ++b) {
+                const bool bit = (dg[k] >> b) & 1u;
+                const uint64_t bm = __ballot(bit);
+                peers &= bit ? bm : ~bm;
+            }
+            rank[k] = lanes_below(peers);
+            if (valid[k] && rank[k] == 0) wcnt[k][wave][dg[k]] = (uint32_t)__popcll(peers);
         }
         __syncthreads();
-        run[tid] += wcnt[0][tid] + wcnt[1][tid] + wcnt[2][tid] + wcnt[3][tid];
+        // per digit: exclusive offsets over (slot, wave) in tile order
+        {
+            uint32_t acc = 0;
+#pragma unroll
+            for (int k = 0; k < KPT; ++k)
+#pragma unroll
+                for (int w = 0; w < 4; ++w) {
+                    const uint32_t c = wcnt[k][w][tid];
+                    wcnt[k][w][tid] = acc;
+                    acc += c;
+                }
+            ttot[tid] = acc;
+            tstart[tid] = acc;
+        }
+        __syncthreads();
+        // exclusive scan of the tile's digit totals (Hillis-Steele on tstart)
+        for (int o = 1; o < 256; o <<= 1) {
+            const uint32_t v = tid >= o ? tstart[tid - o] : 0;
+            __syncthreads();
+            tstart[tid] += v;
+            __syncthreads();
+        }
+        tstart[tid] -= ttot[tid];
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < KPT; ++k)
+            if (valid[k]) sorted[tstart[dg[k]] + wcnt[k][wave][dg[k]] + (uint32_t)rank[k]] = key[k];
+        __syncthreads();
+        const int64_t m = b1 - base < TILE ? b1 - base : TILE;
+        for (int idx = tid; idx < m; idx += SB) {
+            const uint64_t kk = sorted[idx];
+            const uint32_t d = (uint32_t)(kk >> shift) & 255u;
+            dst[run[d] + (uint32_t)idx - tstart[d]] = kk;
+        }
+        __syncthreads();
+        run[tid] += ttot[tid];
         __syncthreads();
     }
 }

@@ -209,9 +209,18 @@ def test_sort_keys(tj):
     from tensorjoin_variants_amd import _lib
     L = _lib.load()
     rng = np.random.default_rng(3)
-    for n, bits in [(1, 5), (1000, 12), (300_001, 40), (70_000, 63)]:
-        say("sort", n, bits)
-        keys = rng.integers(0, 2 ** bits, n, dtype=np.uint64)
+    cases = [(0, 10, "uniform"), (1, 5, "uniform"), (1000, 12, "uniform"), (300_001, 40, "uniform"),
+             (70_000, 63, "uniform"), (2_000_003, 38, "uniform"), (100_000, 20, "three values"),
+             (500_000, 30, "skewed")]
+    for n, bits, kind in cases:
+        say("sort", n, bits, kind)
+        if kind == "three values":
+            keys = rng.choice(np.array([5, 2 ** 19 + 3, 77777], dtype=np.uint64), n)
+        elif kind == "skewed":  # most keys share their low digits
+            keys = (rng.integers(0, 2 ** (bits - 16), n, dtype=np.uint64) << np.uint64(16)) | np.uint64(0x1234)
+            keys[::7] = rng.integers(0, 2 ** bits, keys[::7].size, dtype=np.uint64)
+        else:
+            keys = rng.integers(0, 2 ** bits, n, dtype=np.uint64)
         cap = n + 100
         dev = torch.zeros(cap, dtype=torch.int64, device="cuda")
         dev[:n] = torch.from_numpy(keys.view(np.int64))
