@@ -83,43 +83,57 @@ int rows_per_wg(int combo) {
     }
 }
 
+// Work units of the two per-row passes.  A group of GW = group_width(dp)
+// lanes handles one row, so a wave holds 64 / GW rows of one join; the
+// row-metadata pass takes one such wave-row-set per unit, the scale scan
+// SCAN_SUB of them (one atomic per unit).  chunk prefix tables: units of
+// joins 0..t-1 (device int64 [joins + 1]).
+constexpr int SCAN_SUB = 8;
+
+__device__ inline int find_join(const int64_t* __restrict__ cs, int nj, int64_t c) {
+    int lo = 0, hi = nj;  // cs[lo] <= c < cs[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (cs[mid] <= c) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
 // ------------------------------------------------------------ scale scan
 // max |x| per join (binary32 bits) and the count of non-finite coordinates.
-// One wave takes 8 consecutive batch rows; one atomic per join run.
 __global__ void __launch_bounds__(256) batch_scan_kernel(const float* const* __restrict__ xs,
                                                          const BatchJoin* __restrict__ joins,
-                                                         const int32_t* __restrict__ row_join, int64_t rows,
+                                                         const int64_t* __restrict__ cstart, int nj,
                                                          uint32_t* __restrict__ maxabs, uint64_t* __restrict__ stats) {
     const int lane = threadIdx.x & 63;
-    const int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    const int64_t w0 = (int64_t)blockIdx.x * 4 + tjscreen::uni((int)(threadIdx.x >> 6));
+    const int64_t nw = (int64_t)gridDim.x * 4;
+    const int64_t units = cstart[nj];
     uint32_t bad = 0;
-    auto put = [&](int t, uint32_t m) {
-        for (int o = 32; o > 0; o >>= 1) {
-            const uint32_t om = __shfl_xor(m, o);
-            m = om > m ? om : m;
-        }
-        if (lane == 0 && m) atomicMax(&maxabs[t], m);
-    };
-    for (int64_t c = w * 8; c < rows; c += nw * 8) {
-        int cur = -1;
+    for (int64_t c = w0; c < units; c += nw) {
+        const int t = find_join(cstart, nj, c);
+        const BatchJoin J = joins[t];
+        const int gw = tjmeta::group_width(J.dp);
+        const int rpw = 64 / gw;
+        const int g = lane / gw, lg = lane % gw;
+        const float* x = xs[t];
         uint32_t m = 0;
-        for (int64_t r = c; r < c + 8 && r < rows; ++r) {
-            const int t = row_join[r];
-            if (t != cur) {
-                if (cur >= 0) put(cur, m);
-                cur = t;
-                m = 0;
-            }
-            const int d = joins[t].d;
-            const float* xr = xs[t] + (r - joins[t].gbase) * (int64_t)d;
-            for (int k = lane; k < d; k += 64) {
+        for (int sub = 0; sub < SCAN_SUB; ++sub) {
+            const int64_t local = (c - cstart[t]) * (SCAN_SUB * rpw) + sub * rpw + g;
+            if (local >= J.n) continue;
+            const float* xr = x + local * (int64_t)J.d;
+            for (int k = lg; k < J.d; k += gw) {
                 const uint32_t u = __builtin_bit_cast(uint32_t, xr[k]) & 0x7FFFFFFFu;
                 if (u >= 0x7F800000u) bad++;
                 else m = u > m ? u : m;
             }
         }
-        put(cur, m);
+        for (int o = 32; o > 0; o >>= 1) {
+            const uint32_t om = __shfl_xor(m, o);
+            m = om > m ? om : m;
+        }
+        if (lane == 0 && m) atomicMax(&maxabs[t], m);
     }
     for (int o = 32; o > 0; o >>= 1) bad += __shfl_xor(bad, o);
     if (lane == 0 && bad) atomicAdd((unsigned long long*)&stats[TJ_ST_INVALID], (unsigned long long)bad);
@@ -140,25 +154,44 @@ __global__ void batch_params_kernel(const BatchJoin* __restrict__ joins, int nj,
 }
 
 // ----------------------------------------------------------- row metadata
-// One wave per batch row, written into its group's arrays.
+// One unit (64 / GW rows of one join) per wave, written into its group's
+// arrays.
+template <int PREC, int GW>
+__device__ inline void rowmeta_unit(const float* xs_t, const BatchJoin& J, const JoinParams& P, int64_t local,
+                                    int lg, uint8_t* q, RowMeta* meta) {
+    const bool act = local < J.n;
+    const int64_t lc = act ? local : J.n - 1;  // past the end: shuffle along, store nothing
+    tjmeta::rowmeta_row<PREC, GW>(xs_t + lc * (int64_t)J.d, J.grow + lc, J.d, J.dp, P, lg, q, q, meta, act);
+}
+
 __global__ void __launch_bounds__(256) batch_rowmeta_kernel(const float* const* __restrict__ xs,
                                                             const BatchJoin* __restrict__ joins,
                                                             const BatchGroup* __restrict__ groups,
-                                                            const int32_t* __restrict__ row_join, int64_t rows,
+                                                            const int64_t* __restrict__ cstart, int nj,
                                                             const JoinParams* __restrict__ params, uint8_t* ws) {
-    const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (r >= rows) return;
+    const int64_t c = (int64_t)blockIdx.x * 4 + tjscreen::uni((int)(threadIdx.x >> 6));
+    if (c >= cstart[nj]) return;
     const int lane = threadIdx.x & 63;
-    const int t = row_join[r];
+    const int t = find_join(cstart, nj, c);
     const BatchJoin J = joins[t];
     const BatchGroup G = groups[J.group];
     const JoinParams P = params[t];
-    const int64_t local = r - J.gbase;
-    const float* xr = xs[t] + local * (int64_t)J.d;
     uint8_t* q = ws + G.q_off;
     RowMeta* meta = (RowMeta*)(ws + G.meta_off);
-    if (J.prec == P_FP8) tjmeta::rowmeta_row<P_FP8>(xr, J.grow + local, J.d, J.dp, P, lane, q, q, meta);
-    else tjmeta::rowmeta_row<P_FP16>(xr, J.grow + local, J.d, J.dp, P, lane, q, q, meta);
+    const int gw = tjmeta::group_width(J.dp);
+    const int64_t local = (c - cstart[t]) * (64 / gw) + lane / gw;
+    const int lg = lane % gw;
+    const float* x = xs[t];
+    if (J.prec == P_FP8) {
+        // FP8 widths are at least 64: one row per wave
+        rowmeta_unit<P_FP8, 64>(x, J, P, local, lg, q, meta);
+    } else if (gw == 16) {
+        rowmeta_unit<P_FP16, 16>(x, J, P, local, lg, q, meta);
+    } else if (gw == 32) {
+        rowmeta_unit<P_FP16, 32>(x, J, P, local, lg, q, meta);
+    } else {
+        rowmeta_unit<P_FP16, 64>(x, J, P, local, lg, q, meta);
+    }
 }
 
 // ------------------------------------------------------------------ screen
@@ -331,6 +364,9 @@ struct tj_batch {
     BatchGroup* d_groups;
     int2* d_items;
     int32_t* d_row_join;
+    int64_t* d_cmeta;  // row-metadata units before join t
+    int64_t* d_cscan;  // scale-scan units before join t
+    int64_t units_meta, units_scan;
     // workspace layout
     size_t off_maxabs, off_cursor, off_params, off_cand, off_aux, off_out, off_sort, total;
     hipEvent_t ev[5];
@@ -378,19 +414,19 @@ static int batch_enqueue(const BatchCall& c, hipStream_t s) {
     if ((rc = launch_fill_u64((uint64_t*)maxabs, (b->nj + 1) / 2, 0, s))) return rc;
     if ((rc = launch_fill_u64(cursors, N_FAMILIES, 0, s))) return rc;
     TJ_MARK(0);
-    if (b->rows > 0) {
-        int64_t blocks = (b->rows + 31) / 32;  // 8 rows per wave, 4 waves per block
-        if (blocks > 4096) blocks = 4096;
-        hipLaunchKernelGGL(batch_scan_kernel, dim3((unsigned)blocks), dim3(256), 0, s, c.x, b->d_joins,
-                           b->d_row_join, b->rows, maxabs, stats);
+    if (b->units_scan > 0) {
+        int64_t blocks = (b->units_scan + 3) / 4;  // one unit per wave, 4 waves per block
+        if (blocks > 8192) blocks = 8192;
+        hipLaunchKernelGGL(batch_scan_kernel, dim3((unsigned)blocks), dim3(256), 0, s, c.x, b->d_joins, b->d_cscan,
+                           b->nj, maxabs, stats);
         TJ_HIP(hipGetLastError());
     }
     hipLaunchKernelGGL(batch_params_kernel, dim3((unsigned)((b->nj + 255) / 256)), dim3(256), 0, s, b->d_joins, b->nj,
                        c.T, maxabs, params, stats);
     TJ_HIP(hipGetLastError());
-    if (b->rows > 0) {
-        hipLaunchKernelGGL(batch_rowmeta_kernel, dim3((unsigned)((b->rows + 3) / 4)), dim3(256), 0, s, c.x,
-                           b->d_joins, b->d_groups, b->d_row_join, b->rows, params, ws);
+    if (b->units_meta > 0) {
+        hipLaunchKernelGGL(batch_rowmeta_kernel, dim3((unsigned)((b->units_meta + 3) / 4)), dim3(256), 0, s, c.x,
+                           b->d_joins, b->d_groups, b->d_cmeta, b->nj, params, ws);
         TJ_HIP(hipGetLastError());
     }
     TJ_MARK(1);
@@ -562,6 +598,14 @@ int tj_batch_create(tj_batch** out, int joins, const int64_t* n, const int* d, c
     std::vector<int32_t> rj(rows > 0 ? rows : 1);
     for (int t = 0; t < joins; ++t)
         for (int64_t r = 0; r < J[t].n; ++r) rj[J[t].gbase + r] = t;
+    std::vector<int64_t> cmeta(joins + 1, 0), cscan(joins + 1, 0);
+    for (int t = 0; t < joins; ++t) {
+        const int rpw = 64 / tjmeta::group_width(J[t].dp);
+        cmeta[t + 1] = cmeta[t] + (J[t].n + rpw - 1) / rpw;
+        cscan[t + 1] = cscan[t] + (J[t].n + (int64_t)SCAN_SUB * rpw - 1) / ((int64_t)SCAN_SUB * rpw);
+    }
+    b->units_meta = cmeta[joins];
+    b->units_scan = cscan[joins];
     for (int f = 0; f < N_FAMILIES; ++f) {
         const int64_t nf = b->fam_begin[f + 1] - b->fam_begin[f];
         b->fam_grid[f] = (int)std::max<int64_t>(1, std::min<int64_t>(nf, (int64_t)screen_batch_occupancy(f) * num_cus()));
@@ -573,6 +617,8 @@ int tj_batch_create(tj_batch** out, int joins, const int64_t* n, const int* d, c
     const size_t o_g = bcarve(tcur, sizeof(BatchGroup) * G.size());
     const size_t o_i = bcarve(tcur, sizeof(int2) * it.size());
     const size_t o_r = bcarve(tcur, sizeof(int32_t) * rj.size());
+    const size_t o_cm = bcarve(tcur, sizeof(int64_t) * cmeta.size());
+    const size_t o_cs = bcarve(tcur, sizeof(int64_t) * cscan.size());
     hipError_t e = hipMalloc(&b->dev, tcur);
     if (e != hipSuccess) {
         delete b;
@@ -583,10 +629,14 @@ int tj_batch_create(tj_batch** out, int joins, const int64_t* n, const int* d, c
     b->d_groups = (BatchGroup*)(base + o_g);
     b->d_items = (int2*)(base + o_i);
     b->d_row_join = (int32_t*)(base + o_r);
+    b->d_cmeta = (int64_t*)(base + o_cm);
+    b->d_cscan = (int64_t*)(base + o_cs);
     if ((e = hipMemcpy(b->d_joins, J.data(), sizeof(BatchJoin) * J.size(), hipMemcpyHostToDevice)) != hipSuccess ||
         (e = hipMemcpy(b->d_groups, G.data(), sizeof(BatchGroup) * G.size(), hipMemcpyHostToDevice)) != hipSuccess ||
         (e = hipMemcpy(b->d_items, it.data(), sizeof(int2) * it.size(), hipMemcpyHostToDevice)) != hipSuccess ||
-        (e = hipMemcpy(b->d_row_join, rj.data(), sizeof(int32_t) * rj.size(), hipMemcpyHostToDevice)) != hipSuccess) {
+        (e = hipMemcpy(b->d_row_join, rj.data(), sizeof(int32_t) * rj.size(), hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(b->d_cmeta, cmeta.data(), sizeof(int64_t) * cmeta.size(), hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(b->d_cscan, cscan.data(), sizeof(int64_t) * cscan.size(), hipMemcpyHostToDevice)) != hipSuccess) {
         (void)hipFree(b->dev);
         delete b;
         return hip_fail(e, "hipMemcpy(batch tables)");

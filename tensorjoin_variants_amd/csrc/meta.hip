@@ -43,15 +43,18 @@ __global__ void params_kernel(const uint32_t* maxabs, double T, int d, int dp, i
     stats[TJ_ST_SCALE_EXP] = (uint64_t)(int64_t)q.G;
 }
 
-template <int PREC>
+template <int PREC, int GW>
 __global__ void __launch_bounds__(256) rowmeta_kernel(const float* __restrict__ x, int64_t n, int d,
                                                       int dp, const JoinParams* __restrict__ pp,
                                                       uint8_t* __restrict__ qhi, uint8_t* __restrict__ qlo,
                                                       RowMeta* __restrict__ meta) {
-    const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (row >= n) return;
+    // 64 / GW rows per wave, 4 waves per block
+    const int lane = threadIdx.x & 63;
+    const int64_t row = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * (64 / GW) + lane / GW;
+    const bool act = row < n;
     const JoinParams P = *pp;
-    tjmeta::rowmeta_row<PREC>(x + row * (int64_t)d, row, d, dp, P, threadIdx.x & 63, qhi, qlo, meta);
+    const int64_t rc = act ? row : n - 1;
+    tjmeta::rowmeta_row<PREC, GW>(x + rc * (int64_t)d, rc, d, dp, P, lane % GW, qhi, qlo, meta, act);
 }
 
 __global__ void fill_u64_kernel(uint64_t* p, int64_t n, uint64_t v) {
@@ -82,18 +85,22 @@ int launch_rowmeta(const float* x, int64_t n, int d, int dp, int prec, const Joi
                    void* qlo, RowMeta* meta, uint64_t* stats, hipStream_t s) {
     (void)stats;
     if (n == 0) return TJ_OK;
-    dim3 grid((unsigned)((n + 3) / 4)), block(256);
+    const int gw = tjmeta::group_width(dp);
+    const int64_t rows_per_block = 4 * (64 / gw);
+    dim3 grid((unsigned)((n + rows_per_block - 1) / rows_per_block)), block(256);
     uint8_t* h = (uint8_t*)qhi;
     uint8_t* l = (uint8_t*)qlo;
+#define TJ_RM(PR)                                                                                              \
+    case PR:                                                                                                   \
+        if (gw == 16) hipLaunchKernelGGL((rowmeta_kernel<PR, 16>), grid, block, 0, s, x, n, d, dp, p, h, l, meta); \
+        else if (gw == 32) hipLaunchKernelGGL((rowmeta_kernel<PR, 32>), grid, block, 0, s, x, n, d, dp, p, h, l, meta); \
+        else hipLaunchKernelGGL((rowmeta_kernel<PR, 64>), grid, block, 0, s, x, n, d, dp, p, h, l, meta);         \
+        break;
     switch (prec) {
-        case P_FP16: hipLaunchKernelGGL(rowmeta_kernel<P_FP16>, grid, block, 0, s, x, n, d, dp, p, h, l, meta); break;
-        case P_FP16X3: hipLaunchKernelGGL(rowmeta_kernel<P_FP16X3>, grid, block, 0, s, x, n, d, dp, p, h, l, meta); break;
-        case P_BF16: hipLaunchKernelGGL(rowmeta_kernel<P_BF16>, grid, block, 0, s, x, n, d, dp, p, h, l, meta); break;
-        case P_BF16X3: hipLaunchKernelGGL(rowmeta_kernel<P_BF16X3>, grid, block, 0, s, x, n, d, dp, p, h, l, meta); break;
-        case P_FP8: hipLaunchKernelGGL(rowmeta_kernel<P_FP8>, grid, block, 0, s, x, n, d, dp, p, h, l, meta); break;
-        case P_FP8X3: hipLaunchKernelGGL(rowmeta_kernel<P_FP8X3>, grid, block, 0, s, x, n, d, dp, p, h, l, meta); break;
+        TJ_RM(P_FP16) TJ_RM(P_FP16X3) TJ_RM(P_BF16) TJ_RM(P_BF16X3) TJ_RM(P_FP8) TJ_RM(P_FP8X3)
         default: set_error("rowmeta: unknown precision %d", prec); return TJ_ERR_INVALID;
     }
+#undef TJ_RM
     TJ_HIP(hipGetLastError());
     return TJ_OK;
 }

@@ -58,11 +58,30 @@ __device__ inline JoinParams make_params(uint32_t maxabs_bits, double T, int d, 
 }
 
 // ------------------------------------------------------------------ rowmeta
-// One wave per row.  Lane t handles coordinates t, t+64, ... of the padded
-// row.  Sums of non-negative binary64 terms: relative error <= (n+1) 2^-53.
-template <int PREC>
+// One group of GW lanes (16, 32 or 64, aligned within the wave) per row.
+// Lane t of the group handles coordinates t, t+GW, ... of the padded row.
+// Sums of non-negative binary64 terms, any order: relative error
+// <= (n+1) 2^-53.
+template <int GW>
+__device__ inline double group_sum(double x) {
+    for (int o = GW / 2; o > 0; o >>= 1) x += __shfl_xor(x, o);
+    return x;
+}
+template <int GW>
+__device__ inline double group_max(double x) {
+    for (int o = GW / 2; o > 0; o >>= 1) x = fmax(x, __shfl_xor(x, o));
+    return x;
+}
+
+// Lanes per row for a padded width: no idle lanes for narrow rows.
+__host__ __device__ inline int group_width(int dp) { return dp <= 16 ? 16 : (dp <= 32 ? 32 : 64); }
+
+// act == false: the group takes part in the shuffles but stores nothing
+// (a group past the last row of its join).
+template <int PREC, int GW = 64>
 __device__ inline void rowmeta_row(const float* __restrict__ xr, int64_t row, int d, int dp, const JoinParams& P, int lane,
-                            uint8_t* __restrict__ qhi, uint8_t* __restrict__ qlo, RowMeta* __restrict__ meta) {
+                            uint8_t* __restrict__ qhi, uint8_t* __restrict__ qlo, RowMeta* __restrict__ meta,
+                            bool act = true) {
     constexpr int TERMS = (PREC == P_FP16X3 || PREC == P_BF16X3 || PREC == P_FP8X3) ? 3 : 1;
     constexpr bool FP8 = (PREC == P_FP8 || PREC == P_FP8X3);
     constexpr bool BF16 = (PREC == P_BF16 || PREC == P_BF16X3);
@@ -73,42 +92,42 @@ __device__ inline void rowmeta_row(const float* __restrict__ xr, int64_t row, in
     int e_hi = 0, e_lo = 0;
     if (FP8) {
         double m = 0.0;
-        for (int k = lane; k < d; k += 64) m = fmax(m, fabs((double)xr[k] * sc));
-        for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+        for (int k = lane; k < d; k += GW) m = fmax(m, fabs((double)xr[k] * sc));
+        m = group_max<GW>(m);
         e_hi = fp8_row_exp(m);
         if (TERMS == 3) {
             // lo carries the residual of the hi rounding: at most half an ulp
             // of e4m3 (2^-4 relative) of the largest element.
             double mr = 0.0;
-            for (int k = lane; k < d; k += 64) {
+            for (int k = lane; k < d; k += GW) {
                 double v = (double)xr[k] * sc, b;
                 enc_e4m3(ldexp(v, -e_hi), &b);
                 mr = fmax(mr, fabs(v - ldexp(b, e_hi)));
             }
-            for (int o = 32; o > 0; o >>= 1) mr = fmax(mr, __shfl_xor(mr, o));
+            mr = group_max<GW>(mr);
             e_lo = fp8_row_exp(mr);
         }
     }
 
     double nx = 0.0, nh = 0.0, nl = 0.0, nr = 0.0;
-    for (int k = lane; k < dp; k += 64) {
+    for (int k = lane; k < dp; k += GW) {
         double v = k < d ? (double)xr[k] * sc : 0.0;
         double bh = 0.0, bl = 0.0;
         if (FP8) {
             uint8_t ch = enc_e4m3(ldexp(v, -e_hi), &bh);
             bh = ldexp(bh, e_hi);
-            qhi[row * dp + k] = ch;
+            if (act) qhi[row * dp + k] = ch;
             if (TERMS == 3) {
                 uint8_t cl = enc_e4m3(ldexp(v - bh, -e_lo), &bl);
                 bl = ldexp(bl, e_lo);
-                qlo[row * dp + k] = cl;
+                if (act) qlo[row * dp + k] = cl;
             }
         } else {
             uint16_t ch = BF16 ? enc_bf16(v, &bh) : enc_fp16(v, &bh);
-            ((uint16_t*)qhi)[row * dp + k] = ch;
+            if (act) ((uint16_t*)qhi)[row * dp + k] = ch;
             if (TERMS == 3) {
                 uint16_t cl = BF16 ? enc_bf16(v - bh, &bl) : enc_fp16(v - bh, &bl);
-                ((uint16_t*)qlo)[row * dp + k] = cl;
+                if (act) ((uint16_t*)qlo)[row * dp + k] = cl;
             }
         }
         double r = v - bh - bl;  // exact: the codes are roundings of v and of v - bh
@@ -117,13 +136,11 @@ __device__ inline void rowmeta_row(const float* __restrict__ xr, int64_t row, in
         nl += bl * bl;
         nr += r * r;
     }
-    for (int o = 32; o > 0; o >>= 1) {
-        nx += __shfl_xor(nx, o);
-        nh += __shfl_xor(nh, o);
-        nl += __shfl_xor(nl, o);
-        nr += __shfl_xor(nr, o);
-    }
-    if (lane != 0) return;
+    nx = group_sum<GW>(nx);
+    nh = group_sum<GW>(nh);
+    nl = group_sum<GW>(nl);
+    nr = group_sum<GW>(nr);
+    if (lane != 0 || !act) return;
     (void)ES;
     const double rel = (double)(dp + 4) * 0x1p-52;  // sum of dp non-negative terms, each squared once
     const double root_pad = 1.0 + 0x1p-48;           // sqrt rounding
