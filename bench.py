@@ -211,6 +211,7 @@ def main() -> int:
         "stages_ms": st_ms, "roofline": roofline,
         "pairs_out": stats["out"], "ambiguous": stats["ambiguous"], "candidates": stats["candidates"],
         "ambiguous_fraction": stats["ambiguous"] / pairs_total,
+        "fp32_decided": stats["fp32_accept"] + stats["fp32_reject"], "fp64_refined": stats["fp64_refined"],
         "max_abs_d2_err_vs_fp64": stats["max_err"], "max_err_over_halfwidth": stats["max_ratio"],
         "certificate_violations": stats["violations"],
     }
@@ -329,6 +330,7 @@ def bench_batched(args) -> int:
         "plan": plan.describe(),
         "pairs_out": stats["out"], "ambiguous": stats["ambiguous"], "candidates": stats["candidates"],
         "ambiguous_fraction": stats["ambiguous"] / pairs_total,
+        "fp32_decided": stats["fp32_accept"] + stats["fp32_reject"], "fp64_refined": stats["fp64_refined"],
         "max_abs_d2_err_vs_fp64": stats["max_err"], "max_err_over_halfwidth": stats["max_ratio"],
         "certificate_violations": stats["violations"],
     }

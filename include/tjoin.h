@@ -61,10 +61,10 @@ extern "C" {
  * Slot meanings follow the reference counters [accept, ambiguous, overflow,
  * fp64, reject, bitwise-equal] of CEJ/src/run_g3c_b_r1_gpu_cascade.py:108-129,
  * extended with the screen error statistics. */
-#define TJ_ST_OUT 0           /* pairs written to the output (certified + refined) */
+#define TJ_ST_OUT 0           /* pairs written to the output = CERT_ACCEPT + FP32_ACCEPT + FP64_ACCEPT */
 #define TJ_ST_AMB 1           /* ambiguous pairs decided by the FP64 refinement */
 #define TJ_ST_CERT_ACCEPT 2   /* pairs certified inside (screen fast accept + per-pair interval) */
-#define TJ_ST_FP64_ACCEPT 3   /* ambiguous pairs the FP64 refinement accepted */
+#define TJ_ST_FP64_ACCEPT 3   /* ambiguous pairs the FP64 refinement accepted (after the FP32 filter) */
 #define TJ_ST_CANDIDATES 4    /* pairs the screen queued for the per-pair interval test (the amb_capacity queue):
                                  the band between the fast reject and the fast accept (every survivor in audit mode) */
 #define TJ_ST_MAX_ERR 5       /* max |d2_screen - d2_fp64| over refined pairs (binary64 bits, real units) */
@@ -78,7 +78,10 @@ extern "C" {
 #define TJ_ST_VIOL_KEY 13      /* diagnostic: key of a refined pair outside its interval (flag bit 63 = certified) */
 #define TJ_ST_VIOL_D2 14       /* diagnostic: its binary64 d2 (join units, binary64 bits) */
 #define TJ_ST_VIOL_EST 15      /* diagnostic: its screen estimate and half-width (two binary32) */
-#define TJ_STATS_SLOTS 16
+#define TJ_ST_FP32_ACCEPT 16   /* ambiguous pairs the certified FP32 filter decided inside */
+#define TJ_ST_FP32_REJECT 17   /* ambiguous pairs the certified FP32 filter decided outside */
+#define TJ_ST_FP64_REFINED 18  /* pairs evaluated in the sequential FP64 form (all ambiguous ones in audit mode) */
+#define TJ_STATS_SLOTS 20
 
 /* ---------------------------------------------------------------- library */
 int tj_version(void);

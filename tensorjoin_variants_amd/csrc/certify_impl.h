@@ -33,6 +33,59 @@ struct PairRef {
     int d;
 };
 
+// Certified FP32 middle filter (the a8 stage of the reference's cascade,
+// certified_fp32_filter_i64, CEJ/src/run_g3c_b_r1_gpu_cascade.py:57-129),
+// re-derived: S = sum_k fl32(fl32(x_k - y_k)^2) in any summation order (eight
+// independent partial sums).  Each difference and each square of binary32
+// values is rounded once (relative 2^-24, whatever the cancellation) and
+// every term is non-negative, so |S - d2| <= gamma_{d+2} d2 plus an
+// underflow term below (d+2) 2^-125 (flushed subnormals included).  The
+// oracle's own gap g is charged by deciding against T(1 -+ 2g).
+// Returns 1 (certainly d2_seq <= T), 0 (certainly >), 2 (undecided: FP64).
+__device__ inline int middle32(const float* __restrict__ x, const float* __restrict__ y, int d, double T, double g64) {
+    float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    int k = 0;
+    if (((((uintptr_t)x) | ((uintptr_t)y)) & 15) == 0) {
+        for (; k + 32 <= d; k += 32) {
+            float4 a[8], b[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                a[q] = *(const float4*)(x + k + 4 * q);
+                b[q] = *(const float4*)(y + k + 4 * q);
+            }
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const float t0 = a[q].x - b[q].x, t1 = a[q].y - b[q].y, t2 = a[q].z - b[q].z, t3 = a[q].w - b[q].w;
+                acc[q] = acc[q] + t0 * t0;
+                acc[q] = acc[q] + t1 * t1;
+                acc[q] = acc[q] + t2 * t2;
+                acc[q] = acc[q] + t3 * t3;
+            }
+        }
+    }
+    for (; k < d; ++k) {
+        const float t = x[k] - y[k];
+        acc[k & 7] = acc[k & 7] + t * t;
+    }
+    const float S = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+    if (!(S < INFINITY)) return 2;  // overflow (or NaN): the FP64 form decides
+    const double n = (double)d + 2.0;
+    const double gam = n * 0x1p-24 / (1.0 - n * 0x1p-24);
+    const double absb = n * 0x1p-125;
+    const double Sd = (double)S;
+    // S >= d2 (1 - gam) - absb  and  S <= d2 (1 + gam) + absb, inverted for
+    // d2; a few binary64 roundings, absorbed by the outward 2^-50 steps
+    double hi = (Sd + absb) / (1.0 - gam);
+    hi += hi * 0x1p-50;
+    double lo = (Sd - absb) / (1.0 + gam);
+    lo -= fabs(lo) * 0x1p-50;
+    const double t_acc = nextafter(T * (1.0 - 2.0 * g64), 0.0);
+    const double t_rej = nextafter(T * (1.0 + 2.0 * g64), INFINITY);
+    if (hi <= t_acc) return 1;
+    if (lo > t_rej) return 0;
+    return 2;
+}
+
 __device__ inline void atomic_max_pos(uint64_t* slot, double v) {
     if (v > 0.0) atomicMax((unsigned long long*)slot, (unsigned long long)__builtin_bit_cast(uint64_t, v));
 }
@@ -46,7 +99,7 @@ __global__ void __launch_bounds__(256) certify_kernel(const Src src, const uint6
     if (count > ccap) count = ccap;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     double max_err = 0.0, max_ratio = 0.0;
-    uint64_t viol = 0, n_cert = 0, n_amb = 0, n_amb_in = 0;
+    uint64_t viol = 0, n_cert = 0, n_amb = 0, n_amb_in = 0, n_f32_in = 0, n_f32_out = 0, n_f64 = 0;
     // accepted keys are staged per wave and appended with one atomic per
     // OCAP keys (most candidates are accepted: one atomic per iteration would
     // serialise on the output cursor)
@@ -84,7 +137,18 @@ __global__ void __launch_bounds__(256) certify_kernel(const Src src, const uint6
             // of its wave) never reaches this pass outside audit mode; in
             // audit mode every pair it would accept is checked here.
             const bool fast = (double)p >= (double)pr.mi.m2.y + (double)pr.mj.m2.x;
-            if (cls == 2 || (audit && (cls == 1 || fast))) {
+            // ambiguous: the certified FP32 filter first, the sequential FP64
+            // form for what it leaves (audit mode: FP64 for every ambiguous
+            // pair, checking the filter's decisions too)
+            int f32 = 2;
+            if (cls == 2) {
+                n_amb++;
+                f32 = middle32(pr.xi, pr.xj, pr.d, P.T, P.g64);
+                if (f32 == 1) { n_f32_in++; inside = true; }
+                if (f32 == 0) n_f32_out++;
+            }
+            if ((cls == 2 && (f32 == 2 || audit)) || (audit && (cls == 1 || fast))) {
+                n_f64++;
                 const double d2 = d2_seq(pr.xi, pr.xj, pr.d);
                 const bool in64 = d2 <= P.T;
                 const double dj = d2 / P.scale_real;  // exact power-of-two scaling
@@ -94,15 +158,15 @@ __global__ void __launch_bounds__(256) certify_kernel(const Src src, const uint6
                 // the interval encloses the exact d2; allow the binary64 gap g
                 // and the binary32 rounding of the centre
                 const bool outside_interval = err > (double)hw + 2.0 * P.g64 * dj + 0x1p-23 * fabs((double)est);
-                if (outside_interval || ((cls == 1 || fast) && !in64)) {
+                const bool f32_wrong = (f32 == 1 && !in64) || (f32 == 0 && in64);
+                if (outside_interval || ((cls == 1 || fast) && !in64) || f32_wrong) {
                     viol++;
                     stats[TJ_ST_VIOL_KEY] = key | (cls == 1 ? (1ull << 63) : 0ull);
                     stats[TJ_ST_VIOL_D2] = __builtin_bit_cast(uint64_t, dj);
                     stats[TJ_ST_VIOL_EST] = (uint64_t)__builtin_bit_cast(uint32_t, est) |
                                             ((uint64_t)__builtin_bit_cast(uint32_t, hw) << 32);
                 }
-                if (cls == 2) {
-                    n_amb++;
+                if (cls == 2 && f32 == 2) {
                     if (in64) n_amb_in++;
                     inside = in64;
                 }
@@ -124,6 +188,9 @@ __global__ void __launch_bounds__(256) certify_kernel(const Src src, const uint6
         n_cert += __shfl_xor(n_cert, o);
         n_amb += __shfl_xor(n_amb, o);
         n_amb_in += __shfl_xor(n_amb_in, o);
+        n_f32_in += __shfl_xor(n_f32_in, o);
+        n_f32_out += __shfl_xor(n_f32_out, o);
+        n_f64 += __shfl_xor(n_f64, o);
     }
     if (lane == 0) {
         atomic_max_pos(&stats[TJ_ST_MAX_ERR], max_err);
@@ -132,6 +199,9 @@ __global__ void __launch_bounds__(256) certify_kernel(const Src src, const uint6
         if (n_cert) atomicAdd((unsigned long long*)&stats[TJ_ST_CERT_ACCEPT], (unsigned long long)n_cert);
         if (n_amb) atomicAdd((unsigned long long*)&stats[TJ_ST_AMB], (unsigned long long)n_amb);
         if (n_amb_in) atomicAdd((unsigned long long*)&stats[TJ_ST_FP64_ACCEPT], (unsigned long long)n_amb_in);
+        if (n_f32_in) atomicAdd((unsigned long long*)&stats[TJ_ST_FP32_ACCEPT], (unsigned long long)n_f32_in);
+        if (n_f32_out) atomicAdd((unsigned long long*)&stats[TJ_ST_FP32_REJECT], (unsigned long long)n_f32_out);
+        if (n_f64) atomicAdd((unsigned long long*)&stats[TJ_ST_FP64_REFINED], (unsigned long long)n_f64);
     }
 }
 

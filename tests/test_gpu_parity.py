@@ -44,7 +44,13 @@ def run_check(tj, oracle, R, S, T, prec, method="certified", audit=True, name=""
         assert np.array_equal(got, want), (name, prec, mode)
         if method == "certified":
             assert st["violations"] == 0, st
-            assert st["cert_accept"] + st["fp64_accept"] == st["out"], st
+            assert st["cert_accept"] + st["fp32_accept"] + st["fp64_accept"] == st["out"], st
+            assert st["fp32_accept"] + st["fp32_reject"] <= st["ambiguous"], st
+            if not mode:
+                # outside audit mode only what the FP32 filter leaves reaches FP64
+                assert st["fp64_refined"] == st["ambiguous"] - st["fp32_accept"] - st["fp32_reject"], st
+            else:
+                assert st["fp64_refined"] >= st["ambiguous"], st
     return st
 
 
