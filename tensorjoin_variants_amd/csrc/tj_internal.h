@@ -167,22 +167,23 @@ struct RowMeta { float4 m0, m1, m2; };
 // Sequential binary64 difference form, k ascending, no FMA (every file is
 // compiled with -ffp-contract=off): the predicate of
 // CEJ/src/audit_g2b_accepted_fp64.py:51-57.
+// CH float4 of each row are loaded before any is used (CH = 8: one 128-byte
+// line, so a lane walking its own rows takes each line from L2 once instead
+// of once per 16-byte piece; smaller CH where registers are scarcer).
+template <int CH = 8>
 __device__ inline double d2_seq(const float* __restrict__ x, const float* __restrict__ y, int d) {
     double s = 0.0;
     int k = 0;
     if (((((uintptr_t)x) | ((uintptr_t)y)) & 15) == 0) {
-        // 32 coordinates (one 128-byte line of each row) are loaded before
-        // any is used: a lane walking its own rows then takes each line from
-        // L2 once instead of once per 16-byte piece
-        for (; k + 32 <= d; k += 32) {
-            float4 a[8], b[8];
+        for (; k + 4 * CH <= d; k += 4 * CH) {
+            float4 a[CH], b[CH];
 #pragma unroll
-            for (int q = 0; q < 8; ++q) {
+            for (int q = 0; q < CH; ++q) {
                 a[q] = *(const float4*)(x + k + 4 * q);
                 b[q] = *(const float4*)(y + k + 4 * q);
             }
 #pragma unroll
-            for (int q = 0; q < 8; ++q) {
+            for (int q = 0; q < CH; ++q) {
                 double t;
                 t = (double)a[q].x - (double)b[q].x; s = s + t * t;
                 t = (double)a[q].y - (double)b[q].y; s = s + t * t;
