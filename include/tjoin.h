@@ -52,6 +52,7 @@ extern "C" {
 /* Output pair formats. */
 #define TJ_OUT_I32 0 /* int32 [P, 2] */
 #define TJ_OUT_I64 1 /* int64 [P, 2] */
+#define TJ_OUT_U64_CANONICAL 2 /* tj_host_join of a self-join: directed ids i*n+j, both directions and the n self pairs, ascending */
 
 /* Flags. */
 #define TJ_FLAG_AUDIT 1u  /* route certified accepts through FP64 too and count certificate violations */
@@ -62,7 +63,7 @@ extern "C" {
  * fp64, reject, bitwise-equal] of CEJ/src/run_g3c_b_r1_gpu_cascade.py:108-129,
  * extended with the screen error statistics. */
 #define TJ_ST_OUT 0           /* pairs written to the output = CERT_ACCEPT + FP32_ACCEPT + FP64_ACCEPT */
-#define TJ_ST_AMB 1           /* ambiguous pairs decided by the FP64 refinement */
+#define TJ_ST_AMB 1           /* ambiguous pairs of the per-pair interval (decided by the FP32 filter or the FP64 form) */
 #define TJ_ST_CERT_ACCEPT 2   /* pairs certified inside (screen fast accept + per-pair interval) */
 #define TJ_ST_FP64_ACCEPT 3   /* ambiguous pairs the FP64 refinement accepted (after the FP32 filter) */
 #define TJ_ST_CANDIDATES 4    /* pairs the screen queued for the per-pair interval test (the amb_capacity queue):
@@ -214,6 +215,36 @@ int tj_unpack_pairs(const uint64_t* keys, const uint64_t* count, int64_t capacit
 int tj_canonical_u64(void* temp, size_t* temp_bytes, const int64_t* pairs_i64,
                      const uint64_t* count, int64_t capacity, int64_t n, uint64_t* ids,
                      void* stream);
+
+/* ------------------------------------------------ host-to-host pipeline
+ * The join from pageable host memory to a sorted host result: the public
+ * runs of the reference time exactly this path (pageable FP32 source ->
+ * device stages -> sorted canonical ids on the host,
+ * CEJ/src/run_g5_tensorjoin_public.py:372-547).  Inputs go through pinned
+ * staging buffers (parallel host copies + DMA on a copy stream); an R x S
+ * join runs over chunks of R rows (chunk_rows, 0 = about nR/8) so that
+ * staging and copying out overlap the device work; capacities grow and a
+ * chunk re-runs when it overflows.  `threads` host copy threads (0 = up to 8).
+ * The library allocates device buffers and pinned staging for the handle
+ * (unlike tj_join, whose caller owns all memory). */
+typedef struct tj_host tj_host;
+
+int tj_host_create(tj_host** host, int64_t nR, int64_t nS /* 0 = self-join */, int d, int method,
+                   int precision, unsigned flags /* TJ_FLAG_AUDIT */, int64_t chunk_rows, int threads);
+int tj_host_destroy(tj_host* host);
+/* Synchronous.  R, S: host row-major FP32 (pageable is fine).  out_format
+ * TJ_OUT_I32 / TJ_OUT_I64 -> *result = P sorted (i, j) rows, *count = P;
+ * TJ_OUT_U64_CANONICAL (self-joins) -> *result = the directed ids, *count =
+ * 2P + n.  *result is malloc'ed by the library and released with
+ * tj_host_free.  stats (host, TJ_STATS_SLOTS, may be NULL): counters summed
+ * over chunks, maxima for TJ_ST_MAX_ERR / TJ_ST_MAX_RATIO. */
+int tj_host_join(tj_host* host, const float* R, const float* S, double T, int out_format, void** result,
+                 int64_t* count, uint64_t* stats);
+void tj_host_free(void* result);
+/* Host wall-clock milliseconds of the last tj_host_join: ms[0] total,
+ * ms[1] staging inputs and enqueueing chunks, ms[2] waiting for the device,
+ * ms[3] copying results out. */
+int tj_host_times(const tj_host* host, double* ms, int n);
 
 /* ------------------------------------------------------------ diagnostics
  * MFMA probe: nblocks independent 32x32 tiles, each the product of rows

@@ -20,6 +20,7 @@ ERRORS = {-1: "TJ_ERR_INVALID", -2: "TJ_ERR_HIP", -3: "TJ_ERR_UNSUPPORTED", -4: 
 PREC = {"none": 0, "fp16": 1, "fp16x3": 2, "bf16": 3, "bf16x3": 4, "fp8": 5, "fp8x3": 6}
 METHOD = {"certified": 0, "plain_fp64": 1, "grid": 2}
 OUT = {"int32": 0, "int64": 1}
+OUT_U64_CANONICAL = 2
 FLAG_AUDIT = 1
 FLAG_TIMING = 2
 
@@ -80,6 +81,11 @@ def load():
         "tj_batch_join": (i32, [vp, vp, vp, vp, vp, vp, vp, vp]),
         "tj_batch_capture": (i32, [vp, vp, vp, vp, vp, vp, vp]),
         "tj_batch_replay": (i32, [vp, vp]),
+        "tj_host_create": (i32, [C.POINTER(vp), i64, i64, i32, i32, i32, C.c_uint, i64, i32]),
+        "tj_host_destroy": (i32, [vp]),
+        "tj_host_join": (i32, [vp, vp, vp, f64, i32, C.POINTER(vp), C.POINTER(i64), vp]),
+        "tj_host_free": (None, [vp]),
+        "tj_host_times": (i32, [vp, C.POINTER(C.c_double), i32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -98,7 +104,7 @@ def check(rc: int, what: str = "") -> None:
 def declared_symbols() -> list[str]:
     """Entry points declared in include/tjoin.h."""
     text = HEADER.read_text()
-    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(tj_[a-z0-9_]+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|void|const char\*)\s+(tj_[a-z0-9_]+)\s*\(", text, re.M)))
 
 
 def decode_stats(raw) -> dict:

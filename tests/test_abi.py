@@ -133,3 +133,26 @@ def test_batch_rejects_bad_arguments(L, n, d, prec, code):
 def test_batch_capacity_bounds(L):
     rc, _ = _batch(L, [10], [16], [1], out=1 << 33)
     assert rc == -1
+
+
+@pytest.mark.parametrize("nR,nS,d,method,prec,chunk,code", [
+    (-1, 10, 4, 0, 1, 0, -1),      # negative rows
+    (10, 10, 0, 0, 1, 0, -1),      # d = 0
+    (10, 10, 4, 9, 1, 0, -1),      # unknown method
+    (10, 10, 4, 0, 42, 0, -1),     # unknown precision
+    (10, 10, 8, 2, 0, 0, -3),      # grid beyond d = 3
+    (10, 10, 4, 0, 1, -5, -1),     # negative chunk rows
+    (1 << 31, 10, 4, 0, 1, 0, -3),  # beyond 2^31 rows
+])
+def test_host_rejects_bad_arguments(L, nR, nS, d, method, prec, chunk, code):
+    """tj_host_create validates before it touches the device."""
+    h = C.c_void_p()
+    assert L.tj_host_create(C.byref(h), nR, nS, d, method, prec, 0, chunk, 0) == code
+    assert not h.value
+
+
+def test_host_join_rejects_null_handle(L):
+    res, cnt = C.c_void_p(), C.c_int64()
+    assert L.tj_host_join(None, None, None, 1.0, 0, C.byref(res), C.byref(cnt), None) == -1
+    assert L.tj_host_destroy(None) == 0
+    L.tj_host_free(None)
