@@ -12,13 +12,17 @@ Measured in the same run:
 * the plain GPU method (FP64 brute force, no certificate) on the same
   workload, and the pair-set difference between the two (must be 0);
 * the screen kernel against the MFMA peak of its precision (roofline);
-* the CPU oracle (C, OpenMP, sequential FP64) on a bounded row sample.
+* the CPU oracle (C, OpenMP, sequential FP64) on a bounded row sample;
+* the host-to-host pipeline (pageable NumPy inputs -> sorted pageable
+  result, PCIe and host copies included) on the same join -- reported
+  beside the headline value, never as it.
 
 Other workloads (--workload): clusters_selfjoin (8192 x 32 self-join),
 grid_rxs (4,194,304^2, d = 3, uniform grid), heavytail_fp8 (262,144^2,
 d = 512, Student-t, FP8 screen), batched_uneven (4,096 self-joins of
 log-uniform size in [64, 16384], d in {16, 64, 256}, one pipeline replayed
-as a hipGraph).  Prints one JSON line.
+as a hipGraph), gist_shape_selfjoin (60,000 x 512 self-join: the shape of
+the reference's public host-to-host run).  Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -96,6 +100,8 @@ def make_workload(name: str, scale: float):
         return workloads.clusters_selfjoin(n=int(8192 * scale))
     if name == "grid_rxs":
         return workloads.grid_rxs(n=int(4_194_304 * scale))
+    if name == "gist_shape_selfjoin":
+        return workloads.gist_shape_selfjoin(n=int(60_000 * scale))
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -117,11 +123,13 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="certified_d128",
-                    choices=["certified_d128", "heavytail_fp8", "clusters_selfjoin", "grid_rxs", "batched_uneven"])
+                    choices=["certified_d128", "heavytail_fp8", "clusters_selfjoin", "grid_rxs", "batched_uneven",
+                             "gist_shape_selfjoin"])
     ap.add_argument("--precision", default=None)
     ap.add_argument("--scale", type=float, default=1.0, help="row-count scale (1.0 = full workload)")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-host", action="store_true", help="skip the host-to-host pipeline leg")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-sized device buffer 3 times (scales the HBM byte counters of a "
                          "rocprofv3 --pmc run against a kernel of known traffic)")
@@ -242,6 +250,51 @@ def main() -> int:
         result["speedup_vs_plain"] = value / (pairs_total / pwall)
         result["pair_diffs_vs_plain_fp64"] = diffs
         log(f"plain {pwall*1e3:.1f} ms, diffs {diffs}")
+
+    if not args.no_host:
+        # the same join from pageable host arrays to a sorted host array
+        # (PCIe and host copies included); never the headline value
+        from tensorjoin_variants_amd.host import HostPlan
+        log("host-to-host pipeline")
+        hp = HostPlan(nR, nS, d, method, prec if not grid else "none")
+        hres, hst = hp.run(w.R, w.S, w.T, fmt)  # sizes the capacities
+        walls, parts = [], []
+        for _ in range(3):
+            del hres
+            t0 = time.perf_counter()
+            hres, hst = hp.run(w.R, w.S, w.T, fmt)
+            walls.append(time.perf_counter() - t0)
+            parts.append(hst["host_ms"])
+        hp.close()
+        k = int(np.argsort(walls)[len(walls) // 2])
+        n1 = stats["out"]
+        same = hres.shape[0] == n1 and bool(np.array_equal(hres, out[:n1].cpu().numpy()))
+        result["host_to_host"] = {
+            "value": pairs_total / walls[k], "unit": "pair-evals/s", "ms": walls[k] * 1e3,
+            "parts_ms": parts[k], "inputs": "pageable NumPy float32", "output": f"sorted {fmt} (i, j), pageable",
+            "input_bytes": int(w.R.nbytes + (0 if w.S is None else w.S.nbytes)),
+            "output_bytes": int(hres.nbytes), "identical_to_device_result": same}
+        log(f"host-to-host {walls[k]*1e3:.1f} ms (device-resident {ms_per_step:.1f} ms), identical={same}")
+        del hres
+        if w.S is None:
+            # the reference's output format: sorted directed u64 ids incl. self pairs
+            from tensorjoin_variants_amd.vecio import validate_canonical
+            hp = HostPlan(nR, None, d, method, prec if not grid else "none")
+            ids, _ = hp.run(w.R, None, w.T, "canonical")
+            cw = []
+            for _ in range(3):
+                del ids
+                t0 = time.perf_counter()
+                ids, cst = hp.run(w.R, None, w.T, "canonical")
+                cw.append(time.perf_counter() - t0)
+            hp.close()
+            rep = validate_canonical(ids, nR, expected_count=2 * n1 + nR)
+            result["host_to_host_canonical"] = {
+                "ms": float(np.median(cw)) * 1e3, "ids": int(ids.size), "sha256": rep["canonical_raw_u64_sha256"],
+                "contract_checks_pass": rep["exact_contract_pass"],
+                "output": "sorted directed uint64 i*n+j incl. self pairs, pageable"}
+            log(f"host-to-host canonical {np.median(cw)*1e3:.1f} ms, {ids.size} ids, checks {rep['exact_contract_pass']}")
+            del ids
 
     if not args.no_cpu:
         log("cpu baseline")

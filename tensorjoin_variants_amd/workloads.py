@@ -13,6 +13,7 @@ Sizes (``scale=1.0``):
                      eps at selectivity 1e-4
   heavytail_fp8      |R|=|S|=262,144, d=512, Student-t nu=3, eps at selectivity 1e-5
   batched_uneven     4096 self-joins, n log-uniform [64,16384], d in {16,64,256}
+  gist_shape_selfjoin  N=60,000, d=512 self-join (the reference's public run's shape), ~64 neighbours/pt
 ``scale`` shrinks row counts for tests (selectivities are kept).
 """
 from __future__ import annotations
@@ -160,8 +161,24 @@ def batched_uneven(joins: int = 4096, nmin: int = 64, nmax: int = 16384, dims=(1
             for t, (X, eps) in enumerate(made)]
 
 
+def gist_shape_selfjoin(n: int = 60_000, d: int = 512, selectivity: float = 1.074e-3,
+                        samples: int = 1_000_000, seed_offset: int = 6) -> Workload:
+    """The shape of the reference's public CIFAR-GIST run (60,000 x 512
+    self-join, about 64 neighbours per point: 1,933,039 upper pairs of
+    1.8e9 at its published eps, CEJ/DECISION_G2B_FORMAL.md:17-21) on
+    synthetic rows: non-negative half-normal, 40% zeros, L2-normalised.
+    The CIFAR-GIST file itself is not in the reference archive."""
+    rng = _rng(seed_offset)
+    X = _half_normal_sparse(rng, n, d, 0.4)
+    eps = percentile_eps(X, None, selectivity, samples, rng)
+    return Workload("gist_shape_selfjoin", X, None, eps, eps * eps, np.int32,
+                    dict(n=n, d=d, zero_frac=0.4, selectivity=selectivity, samples=samples,
+                         seed=SEED_BASE + seed_offset))
+
+
 REGISTRY = {
     "clusters_selfjoin": clusters_selfjoin,
+    "gist_shape_selfjoin": gist_shape_selfjoin,
     "grid_rxs": grid_rxs,
     "certified_d128": certified_d128,
     "heavytail_fp8": heavytail_fp8,
