@@ -57,6 +57,7 @@ extern "C" {
 /* Flags. */
 #define TJ_FLAG_AUDIT 1u  /* route certified accepts through FP64 too and count certificate violations */
 #define TJ_FLAG_TIMING 2u /* record HIP events between pipeline stages (read with tj_plan_stage_times) */
+#define TJ_FLAG_PRUNE 4u  /* batches: order each join's rows spatially and skip panel pairs whose bounding balls are certified apart */
 
 /* Slots of the uint64 stats block (TJ_STATS_SLOTS entries, device memory).
  * Slot meanings follow the reference counters [accept, ambiguous, overflow,
@@ -82,7 +83,9 @@ extern "C" {
 #define TJ_ST_FP32_ACCEPT 16   /* ambiguous pairs the certified FP32 filter decided inside */
 #define TJ_ST_FP32_REJECT 17   /* ambiguous pairs the certified FP32 filter decided outside */
 #define TJ_ST_FP64_REFINED 18  /* pairs evaluated in the sequential FP64 form (all ambiguous ones in audit mode) */
-#define TJ_STATS_SLOTS 20
+#define TJ_ST_PRUNED 19        /* batches: pairs certified outside by panel bounding balls (never screened) */
+#define TJ_ST_PRUNED_WORK 20   /* batches: sum over those pairs of the join width d (screen work not done) */
+#define TJ_STATS_SLOTS 24
 
 /* ---------------------------------------------------------------- library */
 int tj_version(void);

@@ -23,13 +23,14 @@ OUT = {"int32": 0, "int64": 1}
 OUT_U64_CANONICAL = 2
 FLAG_AUDIT = 1
 FLAG_TIMING = 2
+FLAG_PRUNE = 4
 
-STATS_SLOTS = 20
+STATS_SLOTS = 24
 STAT_NAMES = {
     0: "out", 1: "ambiguous", 2: "cert_accept", 3: "fp64_accept", 4: "candidates",
     5: "max_err", 6: "max_ratio", 7: "violations", 8: "work", 9: "invalid",
     10: "grid_candidates", 11: "pairs", 12: "scale_exp", 13: "viol_key", 14: "viol_d2", 15: "viol_est",
-    16: "fp32_accept", 17: "fp32_reject", 18: "fp64_refined",
+    16: "fp32_accept", 17: "fp32_reject", 18: "fp64_refined", 19: "pruned", 20: "pruned_work",
 }
 DOUBLE_SLOTS = {5, 6, 14}
 

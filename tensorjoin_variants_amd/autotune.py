@@ -109,6 +109,51 @@ class BatchTuner:
     def choose(self, ds, ns=None) -> list[str]:
         return [self.table.get(self.key(width_class(d)), "fp16") for d in ds]
 
+    def choose_prune(self) -> bool:
+        """The measured choice of tile pruning for batches (off if never tuned)."""
+        return bool(self.table.get("batch_prune", False))
+
+    def tune_prune(self, sets, Ts, precisions, joins: int = 24, reps: int = 2) -> dict:
+        """Times the batch pipeline on its ``joins`` largest sets with and
+        without tile pruning (spatial row order + panel bounding balls) and
+        keeps the faster.  Pruning pays where the screen's column sweep
+        dominates; where the survivors of the near pairs dominate, the order
+        and bound passes cost more than the skipped columns save."""
+        import numpy as np
+        from .batch import BatchPlan, initial_capacities
+        from .join import torch
+        t = torch()
+        Ts = np.broadcast_to(np.asarray(Ts, dtype=np.float64), (len(sets),))
+        idx = sorted(range(len(sets)), key=lambda k: -sets[k].shape[0])[:joins]
+        sub = [sets[k] for k in idx]
+        ns = [x.shape[0] for x in sub]
+        ds = [x.shape[1] for x in sub]
+        precs = [precisions[k] for k in idx] if not isinstance(precisions, str) else precisions
+        times = {}
+        for prune in (False, True):
+            oc, cc = initial_capacities(ns)
+            plan = BatchPlan(ns, ds, precs, oc, cc, prune=prune)
+            plan.run(sub, Ts[idx])
+            xs, ptrs, Td = plan.inputs(sub, Ts[idx])
+            best = float("inf")
+            for _ in range(reps):
+                a, b = t.cuda.Event(enable_timing=True), t.cuda.Event(enable_timing=True)
+                a.record()
+                plan.enqueue(ptrs, Td)
+                b.record()
+                b.synchronize()
+                best = min(best, a.elapsed_time(b))
+            times["prune" if prune else "no_prune"] = best
+            plan.close()
+        win = times["prune"] < times["no_prune"]
+        self.table["batch_prune"] = win
+        try:
+            self.cache.parent.mkdir(parents=True, exist_ok=True)
+            self.cache.write_text(json.dumps(self.table, indent=1, sort_keys=True))
+        except OSError:
+            pass
+        return {"prune": win, "ms": times, "joins": len(idx)}
+
     def tune(self, sets, Ts, per_class: int = 8, reps: int = 2) -> dict:
         import numpy as np
         from .batch import BatchPlan, initial_capacities

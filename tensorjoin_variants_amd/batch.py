@@ -29,7 +29,7 @@ class BatchPlan:
     """Sizes, widths and per-join precisions of a batch, and its workspace."""
 
     def __init__(self, ns, ds, precisions, out_capacity: int = 1 << 20, cand_capacity: int = 1 << 20,
-                 audit: bool = False, timing: bool = False, device: str = "cuda"):
+                 audit: bool = False, timing: bool = False, device: str = "cuda", prune: bool = False):
         self.ns = [int(n) for n in ns]
         self.ds = [int(d) for d in ds]
         if isinstance(precisions, str):
@@ -40,7 +40,7 @@ class BatchPlan:
         for p in self.precisions:
             if p not in BATCH_PRECISIONS:
                 raise ValueError(f"batch precision {p!r} not in {BATCH_PRECISIONS}")
-        self.audit, self.timing, self.device = audit, timing, device
+        self.audit, self.timing, self.device, self.prune = audit, timing, device, prune
         self.pairs_total = sum(n * (n - 1) // 2 for n in self.ns)
         self._b = None
         self._make(int(out_capacity), int(cand_capacity))
@@ -53,7 +53,8 @@ class BatchPlan:
         d = (C.c_int * J)(*self.ds)
         pr = (C.c_int * J)(*[_lib.PREC[p] for p in self.precisions])
         b = C.c_void_p()
-        flags = (_lib.FLAG_AUDIT if self.audit else 0) | (_lib.FLAG_TIMING if self.timing else 0)
+        flags = (_lib.FLAG_AUDIT if self.audit else 0) | (_lib.FLAG_TIMING if self.timing else 0) | \
+            (_lib.FLAG_PRUNE if self.prune else 0)
         _lib.check(L.tj_batch_create(C.byref(b), J, n, d, pr, out_cap, cand_cap, flags), "tj_batch_create")
         self._b = b
         self.out_capacity, self.cand_capacity = out_cap, cand_cap
@@ -158,14 +159,17 @@ def initial_capacities(ns) -> tuple[int, int]:
 
 
 def batched_join(sets, eps=None, *, T=None, precision="auto", audit: bool = False, return_stats: bool = False,
-                 tuner=None):
+                 tuner=None, prune=False):
     """Exact epsilon self-joins of many point sets at once.
 
     sets: list of [n_t, d_t] float32 arrays (d_t <= 256, sizes and widths may
     differ).  eps / T: one value for all sets or one per set (T = fl64(eps^2)
     unless T is given).  precision: 'auto' (per-width choice of the batch
-    tuner), 'fp16', 'fp8', or a list with one entry per set.  Returns a list
-    of sorted int32 (i, j), i < j, arrays -- identical to join(X, None, eps=e)
+    tuner), 'fp16', 'fp8', or a list with one entry per set.  prune: order
+    each set's rows spatially and skip panel pairs whose bounding balls are
+    certified apart -- True, False or 'auto' (the batch tuner's measured
+    choice); the result never depends on it.  Returns a list of
+    sorted int32 (i, j), i < j, arrays -- identical to join(X, None, eps=e)
     per set.
     """
     J = len(sets)
@@ -180,10 +184,14 @@ def batched_join(sets, eps=None, *, T=None, precision="auto", audit: bool = Fals
         Ts = np.broadcast_to(np.asarray(T, dtype=np.float64), (J,)).copy()
     ns = [int(x.shape[0]) for x in sets]
     ds = [int(x.shape[1]) for x in sets]
-    if precision == "auto":
+    if precision == "auto" or prune == "auto":
         from .autotune import BatchTuner
-        precision = (tuner or BatchTuner()).choose(ds, ns)
+        tuner = tuner or BatchTuner()
+        if precision == "auto":
+            precision = tuner.choose(ds, ns)
+        if prune == "auto":
+            prune = tuner.choose_prune()
     out_cap, cand_cap = initial_capacities(ns)
-    plan = BatchPlan(ns, ds, precision, out_cap, cand_cap, audit)
+    plan = BatchPlan(ns, ds, precision, out_cap, cand_cap, audit, prune=prune)
     res, st = plan.run(sets, Ts)
     return (res, st) if return_stats else res

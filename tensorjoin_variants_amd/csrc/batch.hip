@@ -31,8 +31,9 @@
 namespace tjbatch {
 
 struct BatchJoin {
-    int64_t gbase;  // first row in the batch row space
-    int64_t grow;   // first row in the group's arrays
+    int64_t gbase;     // first row in the batch row space
+    int64_t grow;      // first row in the group's arrays
+    int64_t ball_off;  // first float of the join's panel balls (tile pruning)
     int32_t n, d, dp, prec;
     int32_t group, bm;
 };
@@ -153,22 +154,343 @@ __global__ void batch_params_kernel(const BatchJoin* __restrict__ joins, int nj,
     params[t] = tjmeta::make_params(maxabs[t], Tt, J.d, J.dp, J.prec);
 }
 
+// ------------------------------------------- spatial order and tile bounds
+// Certified tile pruning.  The rows of each join are grouped by their nearest
+// of NPIV pivots (farthest-point sampling on a sample of the join's rows,
+// pivots chained nearest-first), so that a panel of consecutive rows is
+// spatially compact; every panel gets a bounding ball (centre c, radius
+// r >= max |x - c|, rounded up); and panel p sweeps only the panels q whose
+// ball bound
+//   |x_i - x_j| >= |c_p - c_q| - r_p - r_q   (triangle inequality)
+// does not exceed sqrt(T (1 + 2g)) -- pairs of the others are certified
+// outside for the binary64 predicate as well (d2_seq >= d2 (1 - g)).  The order is a
+// permutation of each join's rows; output keys are mapped back to the
+// caller's row numbers before the sort, so results do not depend on it.
+// No counterpart in the reference's engine; the reference studied tile
+// pruning from projected bounding boxes separately
+// (tensorjoin_20260908_block_feasibility/src/executor.py:76-84).
+constexpr int NPIV = 8;
+constexpr int PIV_BITS = 3;
+
+// perm[k] (k in the batch row space) = the caller's local row at permuted
+// position k; stored as the low jbits of the sorted order keys.
+__device__ inline int64_t perm_of(const uint64_t* perm, int64_t k, uint64_t jmask) {
+    return perm ? (int64_t)(perm[k] & jmask) : -1;
+}
+
+// One workgroup per join: farthest-point sampling over m = min(n, 64,
+// 8192 / d) sample rows held in LDS (binary32 distances: only the order's
+// quality depends on them).
+__global__ void __launch_bounds__(256) batch_pivot_kernel(const float* const* __restrict__ xs,
+                                                          const BatchJoin* __restrict__ joins,
+                                                          int32_t* __restrict__ pivots) {
+    __shared__ float sx[8192];
+    __shared__ float smin[64];
+    __shared__ int spick;
+    __shared__ int spiv[NPIV];
+    const int t = blockIdx.x;
+    const BatchJoin J = joins[t];
+    const int tid = threadIdx.x;
+    const int n = J.n, d = J.d;
+    int m = n < 64 ? n : 64;
+    if (m > 8192 / d) m = 8192 / d;
+    if (m < 1) m = 1;
+    if (n == 0) {
+        if (tid < NPIV) pivots[t * NPIV + tid] = 0;
+        return;
+    }
+    // (m >= 1 samples; every pivot below is a sample index < m)
+    const float* x = xs[t];
+    for (int e = tid; e < m * d; e += 256) {
+        const int s = e / d, k = e - s * d;
+        const int64_t row = (int64_t)s * n / m;
+        sx[e] = x[row * d + k];
+    }
+    if (tid < 64) smin[tid] = INFINITY;
+    if (tid == 0) spick = 0;
+    __syncthreads();
+    for (int k = 0; k < NPIV; ++k) {
+        const int pk = spick;
+        if (tid == 0) spiv[k] = pk;
+        if (tid < m) {
+            float acc = 0.0f;
+            for (int c = 0; c < d; ++c) {
+                const float u = sx[tid * d + c] - sx[pk * d + c];
+                acc += u * u;
+            }
+            smin[tid] = fminf(smin[tid], acc);
+        }
+        __syncthreads();
+        if (tid == 0) {
+            int best = 0;
+            float bv = -1.0f;
+            for (int s2 = 0; s2 < m; ++s2)
+                if (smin[s2] > bv) { bv = smin[s2]; best = s2; }
+            spick = best;
+        }
+        __syncthreads();
+    }
+    // Chain the pivots greedily (each next = the nearest one not yet taken),
+    // so that pivots of one cluster get adjacent buckets and a cluster's rows
+    // stay one contiguous run of the order.
+    if (tid == 0) {
+        bool used[NPIV] = {};
+        int cur = 0;
+        used[0] = true;
+        pivots[t * NPIV] = (int32_t)((int64_t)spiv[0] * n / m);
+        for (int pos = 1; pos < NPIV; ++pos) {
+            int best = -1;
+            float bv = INFINITY;
+            for (int k = 0; k < NPIV; ++k) {
+                if (used[k]) continue;
+                float acc = 0.0f;
+                for (int c = 0; c < d; ++c) {
+                    const float u = sx[spiv[cur] * d + c] - sx[spiv[k] * d + c];
+                    acc += u * u;
+                }
+                if (best < 0 || acc < bv) { bv = acc; best = k; }
+            }
+            used[best] = true;
+            cur = best;
+            pivots[t * NPIV + pos] = (int32_t)((int64_t)spiv[best] * n / m);
+        }
+    }
+}
+
+// Order keys, one per row: (join << (PIV_BITS + jbits)) | (nearest pivot <<
+// jbits) | local row.  Sorted, position k of the batch row space holds the
+// key of the row placed there (the joins stay contiguous and in order).
+// One scan unit (SCAN_SUB groups of 64 / GW rows of one join) per wave: the
+// pivots' coordinates of a lane (at most four: d <= 256 and GW >= d / 4 or
+// GW = 64) are loaded once per unit.
+template <int GW>
+__device__ inline void order_unit(const float* x, const BatchJoin& J, int t, const int32_t* piv, int64_t first,
+                                  int lane, int jbits, uint64_t* keys) {
+    constexpr int MC = GW == 64 ? 4 : 1;  // coordinates per lane (GW = 16 / 32 serve d <= GW)
+    const int lg = lane % GW, g = lane / GW;
+    float pv[NPIV][MC];
+#pragma unroll
+    for (int k = 0; k < NPIV; ++k)
+#pragma unroll
+        for (int m = 0; m < MC; ++m) {
+            const int c = lg + GW * m;
+            pv[k][m] = c < J.d ? x[(int64_t)piv[k] * J.d + c] : 0.0f;
+        }
+    for (int sub = 0; sub < SCAN_SUB; ++sub) {
+        const int64_t local = first + sub * (64 / GW) + g;
+        const bool act = local < J.n;
+        const int64_t lc = act ? local : J.n - 1;
+        float xv[MC];
+#pragma unroll
+        for (int m = 0; m < MC; ++m) {
+            const int c = lg + GW * m;
+            xv[m] = c < J.d ? x[lc * J.d + c] : 0.0f;
+        }
+        float best = INFINITY;
+        int bk = 0;
+#pragma unroll
+        for (int k = 0; k < NPIV; ++k) {
+            float acc = 0.0f;
+#pragma unroll
+            for (int m = 0; m < MC; ++m) {
+                const float u = xv[m] - pv[k][m];
+                acc += u * u;
+            }
+            for (int o = GW / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+            if (acc < best) { best = acc; bk = k; }  // NaN rows stay in bucket 0
+        }
+        if (act && lg == 0)
+            keys[J.gbase + lc] = ((uint64_t)t << (PIV_BITS + jbits)) | ((uint64_t)bk << jbits) | (uint64_t)lc;
+    }
+}
+
+__global__ void __launch_bounds__(256) batch_order_kernel(const float* const* __restrict__ xs,
+                                                          const BatchJoin* __restrict__ joins,
+                                                          const int64_t* __restrict__ cstart, int nj,
+                                                          const int32_t* __restrict__ pivots, int jbits,
+                                                          uint64_t* __restrict__ keys) {
+    const int64_t c = (int64_t)blockIdx.x * 4 + tjscreen::uni((int)(threadIdx.x >> 6));
+    if (c >= cstart[nj]) return;
+    const int lane = threadIdx.x & 63;
+    const int t = find_join(cstart, nj, c);
+    const BatchJoin J = joins[t];
+    const int gw = tjmeta::group_width(J.dp);
+    const int64_t first = (c - cstart[t]) * (SCAN_SUB * (64 / gw));
+    const int32_t* piv = pivots + t * NPIV;
+    if (gw == 16) order_unit<16>(xs[t], J, t, piv, first, lane, jbits, keys);
+    else if (gw == 32) order_unit<32>(xs[t], J, t, piv, first, lane, jbits, keys);
+    else order_unit<64>(xs[t], J, t, piv, first, lane, jbits, keys);
+}
+
+// One workgroup per work item (join t, panel p): the panel's bounding ball
+// over its rows in the permuted order.  Centre: binary32 mean (any value is
+// valid); radius: binary64 sum of squares, rounded up (each difference,
+// square and addition loses at most 2^-53 relative: factor 1 + (d+4) 2^-52,
+// then the square root and the binary32 rounding upward).
+__global__ void __launch_bounds__(256) batch_ball_kernel(const float* const* __restrict__ xs,
+                                                         const BatchJoin* __restrict__ joins,
+                                                         const int2* __restrict__ items,
+                                                         const uint64_t* __restrict__ perm, int jbits,
+                                                         float* __restrict__ balls) {
+    __shared__ float spart[4][256];
+    __shared__ float sc[256];
+    __shared__ double sr[4];
+    const int2 it = items[blockIdx.x];
+    const BatchJoin J = joins[it.x];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint64_t jmask = (1ull << jbits) - 1;
+    const int64_t r0 = (int64_t)it.y * J.bm;
+    const int64_t r1 = r0 + J.bm < J.n ? r0 + J.bm : J.n;
+    const float* x = xs[it.x];
+    const int d = J.d;  // <= 256: at most four coordinates per lane
+    // centre: each wave sums every fourth row, lanes across the row
+    float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll 4
+    for (int64_t r = r0 + wave; r < r1; r += 4) {
+        const float* xr = x + perm_of(perm, J.gbase + r, jmask) * d;
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+            if (lane + 64 * m < d) acc[m] += xr[lane + 64 * m];
+    }
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+        if (lane + 64 * m < d) spart[wave][lane + 64 * m] = acc[m];
+    __syncthreads();
+    if (tid < d) sc[tid] = ((spart[0][tid] + spart[1][tid]) + (spart[2][tid] + spart[3][tid])) / (float)(r1 - r0);
+    __syncthreads();
+    float cl[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) cl[m] = lane + 64 * m < d ? sc[lane + 64 * m] : 0.0f;
+    double rmax = 0.0;
+#pragma unroll 4
+    for (int64_t r = r0 + wave; r < r1; r += 4) {
+        const float* xr = x + perm_of(perm, J.gbase + r, jmask) * d;
+        double a = 0.0;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            if (lane + 64 * m < d) {
+                const double u = (double)xr[lane + 64 * m] - (double)cl[m];
+                a += u * u;
+            }
+        }
+        for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o);
+        rmax = fmax(rmax, a);
+    }
+    if (lane == 0) sr[wave] = rmax;
+    __syncthreads();
+    float* ball = balls + J.ball_off + (int64_t)it.y * (d + 1);
+    if (tid < d) ball[tid] = sc[tid];
+    if (tid == 0) {
+        const double r2 = fmax(fmax(sr[0], sr[1]), fmax(sr[2], sr[3])) * (1.0 + (double)(d + 4) * 0x1p-52);
+        ball[d] = f32_up(sqrt(r2) * (1.0 + 0x1p-50));
+    }
+}
+
+// One wave per work item (join t, panel p): the column ranges the panel
+// still has to sweep -- runs of consecutive panels q >= p whose ball bound
+// is not certified outside, at most MAXR runs (past that, a new run is
+// merged into the last one: screening more is always sound) -- and the
+// pairs it skips (TJ_ST_PRUNED, TJ_ST_PRUNED_WORK).  Ranges are local
+// columns at panel boundaries (multiples of 32), the last one may end at n.
+constexpr int MAXR = 4;
+constexpr int RSTRIDE = 1 + 2 * MAXR;  // int32 per item: count, then (begin, end) pairs
+
+__global__ void __launch_bounds__(256) batch_prune_kernel(const BatchJoin* __restrict__ joins,
+                                                          const int2* __restrict__ items, int64_t n_items,
+                                                          const JoinParams* __restrict__ params,
+                                                          const float* __restrict__ balls,
+                                                          int32_t* __restrict__ ranges, uint64_t* __restrict__ stats) {
+    const int64_t w = (int64_t)blockIdx.x * 4 + tjscreen::uni((int)(threadIdx.x >> 6));
+    if (w >= n_items) return;
+    const int lane = threadIdx.x & 63;
+    const int2 it = items[w];
+    const BatchJoin J = joins[it.x];
+    const JoinParams& P = params[it.x];
+    const int d = J.d;
+    const int panels = (J.n + J.bm - 1) / J.bm;
+    const float* bp = balls + J.ball_off + (int64_t)it.y * (d + 1);
+    const double rp = (double)bp[d];
+    const double lim = P.T * (1.0 + 2.0 * P.g64) * (1.0 + 0x1p-50);
+    int nr = 0;
+    int rb[MAXR], re[MAXR];  // runs in panel units [rb, re)
+    for (int q = it.y; q < panels; ++q) {
+        bool keep = q == it.y;
+        if (!keep) {
+            const float* bq = balls + J.ball_off + (int64_t)q * (d + 1);
+            double a = 0.0;
+            for (int k = lane; k < d; k += 64) {
+                const double u = (double)bp[k] - (double)bq[k];
+                a += u * u;
+            }
+            for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o);
+            // |c_p - c_q| from below, then the gap from below
+            const double dist = sqrt(a * (1.0 - (double)(d + 4) * 0x1p-52)) * (1.0 - 0x1p-50);
+            const double rs = rp + (double)bq[d];
+            double gap = dist - rs;
+            gap -= (dist + rs) * 0x1p-50;
+            keep = !(gap > 0.0 && gap * gap * (1.0 - 0x1p-50) > lim);
+        }
+        if (!keep) continue;
+        if (nr > 0 && re[nr - 1] == q) re[nr - 1] = q + 1;   // extends the current run
+        else if (nr < MAXR) { rb[nr] = q; re[nr] = q + 1; ++nr; }
+        else re[nr - 1] = q + 1;                             // merged into the last run
+    }
+    if (lane == 0) {
+        int32_t* out = ranges + w * RSTRIDE;
+        out[0] = nr;
+        int64_t swept = 0;
+        for (int r = 0; r < nr; ++r) {
+            const int64_t c0 = (int64_t)rb[r] * J.bm;
+            const int64_t c1 = (int64_t)re[r] * J.bm < J.n ? (int64_t)re[r] * J.bm : J.n;
+            out[1 + 2 * r] = (int32_t)c0;
+            out[2 + 2 * r] = (int32_t)c1;
+            swept += c1 - c0;
+        }
+        const int64_t r0 = (int64_t)it.y * J.bm;
+        const int64_t rows = r0 + J.bm < J.n ? (int64_t)J.bm : J.n - r0;
+        const int64_t skipped = rows * (J.n - r0 - swept);
+        if (skipped > 0) {
+            atomicAdd((unsigned long long*)&stats[TJ_ST_PRUNED], (unsigned long long)skipped);
+            atomicAdd((unsigned long long*)&stats[TJ_ST_PRUNED_WORK], (unsigned long long)(skipped * d));
+        }
+    }
+}
+
+// Output keys in permuted row numbers -> the caller's row numbers, i < j.
+__global__ void batch_remap_kernel(uint64_t* __restrict__ keys, const uint64_t* __restrict__ count, int64_t cap,
+                                   int jbits, const int32_t* __restrict__ row_join, const BatchJoin* __restrict__ joins,
+                                   const uint64_t* __restrict__ perm) {
+    int64_t cnt = (int64_t)*count;
+    if (cnt > cap) cnt = cap;
+    const uint64_t jmask = (1ull << jbits) - 1;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < cnt; k += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t key = keys[k];
+        const int64_t ig = (int64_t)(key >> jbits);
+        const int64_t gb = joins[row_join[ig]].gbase;
+        int64_t a = perm_of(perm, ig, jmask), b = perm_of(perm, gb + (int64_t)(key & jmask), jmask);
+        if (a > b) { const int64_t u = a; a = b; b = u; }
+        keys[k] = ((uint64_t)(gb + a) << jbits) | (uint64_t)b;
+    }
+}
+
 // ----------------------------------------------------------- row metadata
 // One unit (64 / GW rows of one join) per wave, written into its group's
 // arrays.
 template <int PREC, int GW>
 __device__ inline void rowmeta_unit(const float* xs_t, const BatchJoin& J, const JoinParams& P, int64_t local,
-                                    int lg, uint8_t* q, RowMeta* meta) {
+                                    int lg, uint8_t* q, RowMeta* meta, const uint64_t* perm, uint64_t jmask) {
     const bool act = local < J.n;
     const int64_t lc = act ? local : J.n - 1;  // past the end: shuffle along, store nothing
-    tjmeta::rowmeta_row<PREC, GW>(xs_t + lc * (int64_t)J.d, J.grow + lc, J.d, J.dp, P, lg, q, q, meta, act);
+    const int64_t src = perm ? perm_of(perm, J.gbase + lc, jmask) : lc;
+    tjmeta::rowmeta_row<PREC, GW>(xs_t + src * (int64_t)J.d, J.grow + lc, J.d, J.dp, P, lg, q, q, meta, act);
 }
 
 __global__ void __launch_bounds__(256) batch_rowmeta_kernel(const float* const* __restrict__ xs,
                                                             const BatchJoin* __restrict__ joins,
                                                             const BatchGroup* __restrict__ groups,
                                                             const int64_t* __restrict__ cstart, int nj,
-                                                            const JoinParams* __restrict__ params, uint8_t* ws) {
+                                                            const JoinParams* __restrict__ params, uint8_t* ws,
+                                                            const uint64_t* __restrict__ perm, int jbits) {
     const int64_t c = (int64_t)blockIdx.x * 4 + tjscreen::uni((int)(threadIdx.x >> 6));
     if (c >= cstart[nj]) return;
     const int lane = threadIdx.x & 63;
@@ -182,21 +504,23 @@ __global__ void __launch_bounds__(256) batch_rowmeta_kernel(const float* const* 
     const int64_t local = (c - cstart[t]) * (64 / gw) + lane / gw;
     const int lg = lane % gw;
     const float* x = xs[t];
+    const uint64_t jm = (1ull << jbits) - 1;
     if (J.prec == P_FP8) {
         // FP8 widths are at least 64: one row per wave
-        rowmeta_unit<P_FP8, 64>(x, J, P, local, lg, q, meta);
+        rowmeta_unit<P_FP8, 64>(x, J, P, local, lg, q, meta, perm, jm);
     } else if (gw == 16) {
-        rowmeta_unit<P_FP16, 16>(x, J, P, local, lg, q, meta);
+        rowmeta_unit<P_FP16, 16>(x, J, P, local, lg, q, meta, perm, jm);
     } else if (gw == 32) {
-        rowmeta_unit<P_FP16, 32>(x, J, P, local, lg, q, meta);
+        rowmeta_unit<P_FP16, 32>(x, J, P, local, lg, q, meta, perm, jm);
     } else {
-        rowmeta_unit<P_FP16, 64>(x, J, P, local, lg, q, meta);
+        rowmeta_unit<P_FP16, 64>(x, J, P, local, lg, q, meta, perm, jm);
     }
 }
 
 // ------------------------------------------------------------------ screen
 struct ScreenBatchArgs {
     const int2* items;  // (join, panel), largest first
+    const int32_t* ranges;  // per item: column ranges left after tile pruning (NULL: [panel start, n))
     int64_t n_items;
     const BatchJoin* joins;
     const BatchGroup* groups;
@@ -217,12 +541,13 @@ __global__ void __launch_bounds__(256, 2) batch_screen_kernel(const ScreenBatchA
     __shared__ __attribute__((aligned(16))) float sNp[NSTAGE_MAX * 64];
     __shared__ __attribute__((aligned(16))) float sW[NSTAGE_MAX * 64];
     __shared__ __attribute__((aligned(16))) uint32_t sSc[NSTAGE_MAX * 64];
-    __shared__ uint64_t sCK[4][CCAP];
-    __shared__ float sCP[4][CCAP];
+    __shared__ uint64_t sCK[4][CCAP_BATCH];
+    __shared__ float sCP[4][CCAP_BATCH];
     __shared__ int64_t sItem;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     int nC = 0;
+    uint32_t nAcc = 0;  // fast accepts of this wave (wave-uniform)
     for (;;) {
         if (tid == 0) sItem = (int64_t)atomicAdd((unsigned long long*)A.cursor, 1ull);
         __syncthreads();
@@ -242,11 +567,15 @@ __global__ void __launch_bounds__(256, 2) batch_screen_kernel(const ScreenBatchA
         io.koff_b = -J.grow;           // j -> local row
         io.jbits = A.jbits;
         const int64_t r0 = J.grow + (int64_t)it.y * J.bm;
-        const int64_t ce = J.grow + J.n;
+        const int32_t* rg = A.ranges ? A.ranges + item * RSTRIDE : nullptr;
+        const int nr = rg ? uni(rg[0]) : 1;
+        for (int r = 0; r < nr; ++r) {
+        const int64_t cb = rg ? J.grow + (int64_t)uni(rg[1 + 2 * r]) : r0;
+        const int64_t ce = rg ? J.grow + (int64_t)uni(rg[2 + 2 * r]) : J.grow + (int64_t)J.n;
 #define X(ID, PR, DPV)                                                                                          \
     case ID:                                                                                                    \
-        screen_panel<PR, DPV, rb_for<PR, DPV>(), true>(io, A.co, r0, r0, ce, sB, sNp, sW, sSc, sCK[wave], \
-                                                       sCP[wave], nC, A.fast_accept);                           \
+        screen_panel<PR, DPV, rb_for<PR, DPV>(), true, CCAP_BATCH>(io, A.co, r0, cb, ce, sB, sNp, sW, sSc, sCK[wave], \
+                                                       sCP[wave], nC, nAcc, A.fast_accept);                     \
         break;
         if constexpr (FAM == 0) {
             switch (uni(G.combo)) {
@@ -270,8 +599,9 @@ __global__ void __launch_bounds__(256, 2) batch_screen_kernel(const ScreenBatchA
             }
         }
 #undef X
+        }
     }
-    if (nC) cand_flush(A.co, sCK[wave], sCP[wave], nC, lane);
+    cand_finish(A.co, sCK[wave], sCP[wave], nC, nAcc, lane);
 }
 
 // ----------------------------------------------------------- certificate
@@ -283,9 +613,11 @@ struct BatchSrc {
     const JoinParams* params;
     const uint8_t* ws;
     int jbits;
+    const uint64_t* perm;  // permuted position -> caller's row (NULL: identity)
     __device__ inline void fetch(uint64_t key, tjcert::PairRef& r) const {
+        const uint64_t jmask = (1ull << jbits) - 1;
         const int64_t ig = (int64_t)(key >> jbits);
-        const int64_t jl = (int64_t)(key & ((1ull << jbits) - 1));
+        const int64_t jl = (int64_t)(key & jmask);
         const int t = row_join[ig];
         const BatchJoin J = joins[t];
         const int64_t il = ig - J.gbase;
@@ -293,8 +625,10 @@ struct BatchSrc {
         r.P = params + t;
         r.mi = meta[J.grow + il];
         r.mj = meta[J.grow + jl];
-        r.xi = xs[t] + il * (int64_t)J.d;
-        r.xj = xs[t] + jl * (int64_t)J.d;
+        const int64_t si = perm ? perm_of(perm, ig, jmask) : il;
+        const int64_t sj = perm ? perm_of(perm, J.gbase + jl, jmask) : jl;
+        r.xi = xs[t] + si * (int64_t)J.d;
+        r.xj = xs[t] + sj * (int64_t)J.d;
         r.d = J.d;
     }
 };
@@ -367,8 +701,13 @@ struct tj_batch {
     int64_t* d_cmeta;  // row-metadata units before join t
     int64_t* d_cscan;  // scale-scan units before join t
     int64_t units_meta, units_scan;
+    // tile pruning (off unless prune)
+    bool prune;
+    int order_bits;
+    int64_t ball_floats;
     // workspace layout
     size_t off_maxabs, off_cursor, off_params, off_cand, off_aux, off_out, off_sort, total;
+    size_t off_pivots, off_perm, off_balls, off_cend, off_rows;
     hipEvent_t ev[5];
     bool timing;
     hipGraphExec_t graph;
@@ -413,6 +752,13 @@ static int batch_enqueue(const BatchCall& c, hipStream_t s) {
     if ((rc = launch_fill_u64(stats + TJ_ST_PAIRS, 1, (uint64_t)b->pairs_total, s))) return rc;
     if ((rc = launch_fill_u64((uint64_t*)maxabs, (b->nj + 1) / 2, 0, s))) return rc;
     if ((rc = launch_fill_u64(cursors, N_FAMILIES, 0, s))) return rc;
+    const uint64_t* perm = nullptr;
+    int32_t* ranges = nullptr;
+    if (b->prune) {
+        perm = (const uint64_t*)(ws + b->off_perm);
+        ranges = (int32_t*)(ws + b->off_cend);
+        if ((rc = launch_fill_u64((uint64_t*)(ws + b->off_rows), 1, (uint64_t)b->rows, s))) return rc;
+    }
     TJ_MARK(0);
     if (b->units_scan > 0) {
         int64_t blocks = (b->units_scan + 3) / 4;  // one unit per wave, 4 waves per block
@@ -424,9 +770,30 @@ static int batch_enqueue(const BatchCall& c, hipStream_t s) {
     hipLaunchKernelGGL(batch_params_kernel, dim3((unsigned)((b->nj + 255) / 256)), dim3(256), 0, s, b->d_joins, b->nj,
                        c.T, maxabs, params, stats);
     TJ_HIP(hipGetLastError());
+    if (b->prune && b->units_scan > 0) {
+        // spatial order of each join's rows (sorted order keys = the permutation)
+        int32_t* pivots = (int32_t*)(ws + b->off_pivots);
+        hipLaunchKernelGGL(batch_pivot_kernel, dim3((unsigned)b->nj), dim3(256), 0, s, c.x, b->d_joins, pivots);
+        TJ_HIP(hipGetLastError());
+        hipLaunchKernelGGL(batch_order_kernel, dim3((unsigned)((b->units_scan + 3) / 4)), dim3(256), 0, s, c.x,
+                           b->d_joins, b->d_cscan, b->nj, pivots, b->jbits, (uint64_t*)perm);
+        TJ_HIP(hipGetLastError());
+        if ((rc = launch_sort(ws + b->off_sort, (uint64_t*)perm, (const uint64_t*)(ws + b->off_rows), b->rows,
+                              b->order_bits, s)))
+            return rc;
+    }
     if (b->units_meta > 0) {
         hipLaunchKernelGGL(batch_rowmeta_kernel, dim3((unsigned)((b->units_meta + 3) / 4)), dim3(256), 0, s, c.x,
-                           b->d_joins, b->d_groups, b->d_cmeta, b->nj, params, ws);
+                           b->d_joins, b->d_groups, b->d_cmeta, b->nj, params, ws, perm, b->jbits);
+        TJ_HIP(hipGetLastError());
+    }
+    if (b->prune && b->n_items > 0) {
+        float* balls = (float*)(ws + b->off_balls);
+        hipLaunchKernelGGL(batch_ball_kernel, dim3((unsigned)b->n_items), dim3(256), 0, s, c.x, b->d_joins,
+                           b->d_items, perm, b->jbits, balls);
+        TJ_HIP(hipGetLastError());
+        hipLaunchKernelGGL(batch_prune_kernel, dim3((unsigned)((b->n_items + 3) / 4)), dim3(256), 0, s, b->d_joins,
+                           b->d_items, b->n_items, params, balls, ranges, stats);
         TJ_HIP(hipGetLastError());
     }
     TJ_MARK(1);
@@ -435,6 +802,7 @@ static int batch_enqueue(const BatchCall& c, hipStream_t s) {
         if (n_items == 0) continue;
         ScreenBatchArgs a{};
         a.items = b->d_items + b->fam_begin[f];
+        a.ranges = ranges ? ranges + b->fam_begin[f] * RSTRIDE : nullptr;
         a.n_items = n_items;
         a.joins = b->d_joins;
         a.groups = b->d_groups;
@@ -458,9 +826,16 @@ static int batch_enqueue(const BatchCall& c, hipStream_t s) {
         int64_t blocks = (b->cand_cap + 255) / 256;
         const int64_t maxb = (int64_t)num_cus() * 8;
         if (blocks > maxb) blocks = maxb;
-        const BatchSrc src{c.x, b->d_joins, b->d_groups, b->d_row_join, params, ws, b->jbits};
+        const BatchSrc src{c.x, b->d_joins, b->d_groups, b->d_row_join, params, ws, b->jbits, perm};
         hipLaunchKernelGGL(tjcert::certify_kernel<BatchSrc>, dim3((unsigned)blocks), dim3(256), 0, s, src, cand, aux,
                            b->cand_cap, out_keys, b->out_cap, stats, (b->flags & TJ_FLAG_AUDIT) ? 1 : 0);
+        TJ_HIP(hipGetLastError());
+    }
+    if (b->prune && b->out_cap > 0) {
+        int64_t blocks = (b->out_cap + 255) / 256;
+        if (blocks > 2048) blocks = 2048;
+        hipLaunchKernelGGL(batch_remap_kernel, dim3((unsigned)blocks), dim3(256), 0, s, out_keys, stats + TJ_ST_OUT,
+                           b->out_cap, b->jbits, b->d_row_join, b->d_joins, perm);
         TJ_HIP(hipGetLastError());
     }
     TJ_MARK(3);
@@ -505,6 +880,7 @@ int tj_batch_create(tj_batch** out, int joins, const int64_t* n, const int* d, c
         J[t].prec = precision[t];
         J[t].bm = rows_per_wg(combo);
         J[t].group = combo;  // remapped to a dense group index below
+        J[t].ball_off = 0;
         rows += n[t];
         max_n = std::max<int64_t>(max_n, n[t]);
         pairs += n[t] * (n[t] - 1) / 2;
@@ -548,6 +924,14 @@ int tj_batch_create(tj_batch** out, int joins, const int64_t* n, const int* d, c
     b->ibits = bits_for(rows > 1 ? rows : 2);
     b->jbits = bits_for(max_n > 1 ? max_n : 2);
     b->keybits = b->ibits + b->jbits;
+    b->prune = (flags & TJ_FLAG_PRUNE) != 0;
+    b->order_bits = bits_for(joins > 1 ? joins : 2) + PIV_BITS + b->jbits;
+    b->ball_floats = 0;
+    for (int t = 0; t < joins; ++t) {
+        J[t].ball_off = b->ball_floats;
+        if (J[t].n >= 2) b->ball_floats += (int64_t)((J[t].n + J[t].bm - 1) / J[t].bm) * (J[t].d + 1);
+    }
+    if (b->order_bits > 63) b->prune = false;
     if (b->keybits > 63) {
         delete b;
         set_error("pair keys need more than 63 bits");
@@ -566,7 +950,14 @@ int tj_batch_create(tj_batch** out, int joins, const int64_t* n, const int* d, c
     b->off_cand = bcarve(cur, 8 * (size_t)std::max<int64_t>(cand_capacity, 1));
     b->off_aux = bcarve(cur, 4 * (size_t)std::max<int64_t>(cand_capacity, 1));
     b->off_out = bcarve(cur, 8 * (size_t)std::max<int64_t>(out_capacity, 1));
-    b->off_sort = bcarve(cur, sort_temp_bytes(out_capacity));
+    b->off_sort = bcarve(cur, sort_temp_bytes(b->prune ? std::max<int64_t>(out_capacity, rows) : out_capacity));
+    b->off_pivots = b->off_perm = b->off_balls = b->off_cend = b->off_rows = 0;
+    if (b->prune) {
+        b->off_pivots = bcarve(cur, 4 * (size_t)NPIV * joins);
+        b->off_perm = bcarve(cur, 8 * (size_t)std::max<int64_t>(rows, 1));
+        b->off_balls = bcarve(cur, 4 * (size_t)std::max<int64_t>(b->ball_floats, 1));
+        b->off_rows = bcarve(cur, 8);
+    }
     b->total = (cur + 255) / 256 * 256;
 
     // work list: every panel of every join with at least two rows, largest
@@ -590,6 +981,10 @@ int tj_batch_create(tj_batch** out, int joins, const int64_t* n, const int* d, c
         return a.fam != c.fam ? a.fam < c.fam : a.cost > c.cost;
     });
     b->n_items = (int64_t)items.size();
+    if (b->prune) {
+        b->off_cend = bcarve(cur, 4 * (size_t)RSTRIDE * std::max<int64_t>(b->n_items, 1));
+        b->total = (cur + 255) / 256 * 256;
+    }
     for (int f = 0; f <= N_FAMILIES; ++f) b->fam_begin[f] = 0;
     for (const Item& x : items) b->fam_begin[x.fam + 1]++;
     for (int f = 0; f < N_FAMILIES; ++f) b->fam_begin[f + 1] += b->fam_begin[f];

@@ -207,7 +207,11 @@ struct Geo {
     __device__ static inline int unit(int col, int q) { return col * R16 + (q ^ ((col >> FSH) & FMASK)); }
 };
 
-constexpr int CCAP = 256;  // per-wave staged candidates
+// Per-wave staged candidates: 256 for the single-join screens (a larger
+// buffer slows the FP8 d = 512 body by a fifth), 512 for the batched
+// screens, whose dense-survivor joins flush half as often.
+constexpr int CCAP = 256;
+constexpr int CCAP_BATCH = 512;
 
 // Operands of one panel sweep.  Row indices are positions in the quantized
 // arrays; the emitted key of pair (i, j) is key_pack(i + koff_a, j + koff_b),
@@ -231,9 +235,12 @@ struct CandOut {
 constexpr uint64_t ACCEPT_BIT = 1ull << 63;  // staged-key flag: certified inside
 
 // Route the nC entries staged in this wave's LDS buffers: flagged keys to
-// the output, the others with their s to the band queue.  Two atomics per
-// flush.  Keys use at most 62 bits, so bit 63 is free for the flag.
-__device__ __forceinline__ void cand_flush(const CandOut& co, const uint64_t* sCK, const float* sCP, int& nC, int lane) {
+// the output, the others with their s to the band queue.  At most two
+// atomics per flush (the fast accepts are counted in nAcc and added to
+// cert_counter once, by cand_finish).  Keys use at most 62 bits, so bit 63
+// is free for the flag.
+__device__ __forceinline__ void cand_flush(const CandOut& co, const uint64_t* sCK, const float* sCP, int& nC,
+                                           uint32_t& nAcc, int lane) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     nC = uni(nC);
@@ -249,11 +256,9 @@ __device__ __forceinline__ void cand_flush(const CandOut& co, const uint64_t* sC
     n_acc = uni(n_acc);
     n_band = uni(n_band);
     uint64_t base_o = 0, base_b = 0;
+    nAcc += (uint32_t)n_acc;
     if (lane == 0) {
-        if (n_acc) {
-            base_o = atomicAdd((unsigned long long*)co.out_counter, (unsigned long long)n_acc);
-            atomicAdd((unsigned long long*)co.cert_counter, (unsigned long long)n_acc);
-        }
+        if (n_acc) base_o = atomicAdd((unsigned long long*)co.out_counter, (unsigned long long)n_acc);
         if (n_band) base_b = atomicAdd((unsigned long long*)co.counter, (unsigned long long)n_band);
     }
     base_o = uni64(base_o);
@@ -284,6 +289,13 @@ __device__ __forceinline__ void cand_flush(const CandOut& co, const uint64_t* sC
     nC = 0;
 }
 
+// End of a wave's work: flush what is staged and add its fast accepts.
+__device__ __forceinline__ void cand_finish(const CandOut& co, const uint64_t* sCK, const float* sCP, int& nC,
+                                            uint32_t& nAcc, int lane) {
+    if (nC) cand_flush(co, sCK, sCP, nC, nAcc, lane);
+    if (lane == 0 && nAcc) atomicAdd((unsigned long long*)co.cert_counter, (unsigned long long)nAcc);
+}
+
 // LDS bytes of the S ring of one (precision, width) instantiation.
 template <int PREC, int DP>
 constexpr int ring_bytes() { return Geo<PREC, DP>::NSTAGE * Geo<PREC, DP>::BLKB; }
@@ -293,10 +305,10 @@ constexpr int ring_bytes() { return Geo<PREC, DP>::NSTAGE * Geo<PREC, DP>::BLKB;
 // the workgroup's S ring (NSTAGE slots: quantized columns, reject levels v,
 // accept terms w, FP8 scales), sCK/sCP this wave's staging buffers.  With
 // fast_accept == 0 every survivor goes to the band queue (audit mode).
-template <int PREC, int DP, int RB, bool SELF>
+template <int PREC, int DP, int RB, bool SELF, int CC = CCAP>
 __device__ __forceinline__ void screen_panel(const PanelIO& io, const CandOut& co, int64_t r0, int64_t cb, int64_t ce,
                                              uint8_t* sB, float* sNp, float* sW, uint32_t* sSc, uint64_t* sCK,
-                                             float* sCP, int& nC, int fast_accept) {
+                                             float* sCP, int& nC, uint32_t& nAcc, int fast_accept) {
     using C = Cfg<PREC>;
     using Gm = Geo<PREC, DP>;
     constexpr int NSTAGE = Gm::NSTAGE;
@@ -551,7 +563,7 @@ __device__ __forceinline__ void screen_panel(const PanelIO& io, const CandOut& c
                     const bool mine = (m >> lane) & 1;
                     const int k = __popcll(m);
                     nC = uni(nC);  // wave-uniform: keeps the capacity test a scalar branch
-                    if (__builtin_expect(nC + k > CCAP, 0)) cand_flush(co, sCK, sCP, nC, lane);  // out of line
+                    if (__builtin_expect(nC + k > CC, 0)) cand_flush(co, sCK, sCP, nC, nAcc, lane);  // out of line
                     if (mine) {
                         const int pos = nC + lanes_below(m);
                         uint64_t key = kb + ((uint64_t)off << io.jbits);
@@ -601,6 +613,7 @@ __global__ void __launch_bounds__(256, (min_blocks<PREC, DP, RB>())) screen_kern
     const CandOut co{A.amb_keys, A.amb_aux, A.amb_cap, &A.stats[TJ_ST_CANDIDATES],
                      A.out_keys, A.out_cap, &A.stats[TJ_ST_OUT], &A.stats[TJ_ST_CERT_ACCEPT]};
     int nC = 0;
+    uint32_t nAcc = 0;  // fast accepts of this wave (wave-uniform)
 
     for (;;) {
         if (tid == 0) sItem = atomicAdd((unsigned long long*)&A.stats[TJ_ST_WORK], 1ull);
@@ -615,9 +628,10 @@ __global__ void __launch_bounds__(256, (min_blocks<PREC, DP, RB>())) screen_kern
         if (SELF) { cb = r0; ce = A.nb; }
         else { cb = (int64_t)split * A.cols_per_split; ce = cb + A.cols_per_split; if (ce > A.nb) ce = A.nb; }
         if (cb >= ce) continue;
-        screen_panel<PREC, DP, RB, SELF>(io, co, r0, cb, ce, sB, sNp, sW, sSc, sCK[wave], sCP[wave], nC, !A.audit);
+        screen_panel<PREC, DP, RB, SELF>(io, co, r0, cb, ce, sB, sNp, sW, sSc, sCK[wave], sCP[wave], nC, nAcc,
+                                         !A.audit);
     }
-    if (nC) cand_flush(co, sCK[wave], sCP[wave], nC, lane);
+    cand_finish(co, sCK[wave], sCP[wave], nC, nAcc, lane);
 }
 
 // Rows per wave: two 32-row blocks while their stationary fragments stay

@@ -96,7 +96,7 @@ def test_join_rejects_bad_threshold(L):
 
 def test_stats_decoding():
     import numpy as np
-    raw = np.zeros(20, np.uint64)
+    raw = np.zeros(24, np.uint64)
     raw[0] = 5
     raw[5] = np.array([0.25]).view(np.uint64)[0]
     raw[12] = np.array([-3]).view(np.uint64)[0]

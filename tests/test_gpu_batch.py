@@ -58,11 +58,11 @@ def mixed_sets(seed, sizes, dims):
 SIZES = [0, 1, 2, 3, 31, 64, 65, 127, 200, 257, 511, 700, 1500, 2100]
 
 
-@pytest.mark.parametrize("prec", ["fp16", "fp8"])
-def test_batch_mixed_widths(tj, oracle, prec):
+@pytest.mark.parametrize("prec,prune", [("fp16", False), ("fp8", False), ("fp16", True), ("fp8", True)])
+def test_batch_mixed_widths(tj, oracle, prec, prune):
     sets, Ts = mixed_sets(11, SIZES * 2, [1, 3, 16, 17, 31, 64, 100, 128, 200, 256])
-    say("batch mixed widths", prec, len(sets))
-    got, st = tj.batched_join(sets, T=Ts, precision=prec, audit=True, return_stats=True)
+    say("batch mixed widths", prec, "prune" if prune else "", len(sets))
+    got, st = tj.batched_join(sets, T=Ts, precision=prec, audit=True, return_stats=True, prune=prune)
     say("batch stats", {k: st[k] for k in ("out", "candidates", "ambiguous", "violations")})
     assert st["violations"] == 0
     check_all(oracle, sets, Ts, got, f"mixed {prec}")
@@ -87,7 +87,8 @@ def test_batch_matches_single_join(tj):
         assert np.array_equal(g.astype(np.int64), single)
 
 
-def test_batch_edge_cases(tj, oracle):
+@pytest.mark.parametrize("prune", [False, True])
+def test_batch_edge_cases(tj, oracle, prune):
     rng = np.random.default_rng(14)
     # duplicates with T = 0, all-equal rows, a set of one row, empty sets
     A = mixture(rng, 300, 64)
@@ -99,8 +100,8 @@ def test_batch_edge_cases(tj, oracle):
     F = mixture(rng, 500, 3) * np.float32(1e4)     # large magnitudes, tiny width
     sets = [A, B, C, D, E, F]
     Ts = np.array([0.0, 0.0, 1.0, 1.0, float(eps_for(rng, E)) ** 2, float(eps_for(rng, F)) ** 2])
-    say("batch edge cases")
-    got, st = tj.batched_join(sets, T=Ts, precision="fp16", audit=True, return_stats=True)
+    say("batch edge cases", "prune" if prune else "")
+    got, st = tj.batched_join(sets, T=Ts, precision="fp16", audit=True, return_stats=True, prune=prune)
     assert st["violations"] == 0
     check_all(oracle, sets, Ts, got, "edge")
     assert got[1].shape[0] == 40 * 39 // 2 and got[3].shape == (0, 2) and got[2].shape == (0, 2)
@@ -132,7 +133,7 @@ def test_batch_uneven_workload_reduced(tj, oracle):
     sets = [w.R for w in ws]
     Ts = np.array([w.T for w in ws])
     say("batched_uneven reduced", sum(x.shape[0] for x in sets))
-    got, st = tj.batched_join(sets, T=Ts, precision="fp16", audit=True, return_stats=True)
+    got, st = tj.batched_join(sets, T=Ts, precision="fp16", audit=True, return_stats=True, prune=True)
     say("stats", {k: st[k] for k in ("out", "candidates", "ambiguous", "violations", "max_err")})
     assert st["violations"] == 0
     check_all(oracle, sets, Ts, got, "batched_uneven")
@@ -191,3 +192,66 @@ def test_plan_graph_replay(tj):
         n = int(plan._stats[0].item())
         assert n == eager.shape[0]
         assert np.array_equal(pairs[:n].cpu().numpy(), eager)
+
+
+def separated(rng, n, d, k, gap):
+    """k Gaussian clusters (unit spread) whose centres are ~gap apart."""
+    c = rng.standard_normal((k, d)) * gap / np.sqrt(2.0 * d)
+    return (c[rng.integers(0, k, n)] + rng.standard_normal((n, d))).astype(np.float32)
+
+
+@pytest.mark.parametrize("prec", ["fp16", "fp8"])
+def test_tile_pruning_matches_unpruned_and_oracle(tj, oracle, prec):
+    """Far-apart clusters: most panel pairs are certified apart and skipped;
+    results equal the unpruned batch and the oracle, join by join."""
+    rng = np.random.default_rng(21)
+    sets, Ts = [], []
+    for n, d in [(3000, 16), (2600, 64), (2200, 256), (1800, 32), (700, 128), (300, 16)]:
+        if prec == "fp8" and d < 64:
+            d = 64
+        X = separated(rng, n, d, 4, 12.0 * np.sqrt(d))
+        sets.append(X)
+        Ts.append(float(eps_for(rng, X, 0.02)) ** 2)
+    Ts = np.asarray(Ts)
+    say("tile pruning", prec, [x.shape for x in sets])
+    got, st = tj.batched_join(sets, T=Ts, precision=prec, return_stats=True, prune=True)
+    ref, st0 = tj.batched_join(sets, T=Ts, precision=prec, return_stats=True)
+    say("pruned pairs", st["pruned"], "of", st["pairs"], "candidates", st["candidates"], st0["candidates"])
+    assert st["pruned"] > st["pairs"] // 4 and st0["pruned"] == 0
+    for g, r in zip(got, ref):
+        assert np.array_equal(g, r)
+    check_all(oracle, sets, Ts, got, f"pruned {prec}")
+
+
+def test_tile_pruning_keeps_cross_cluster_pairs(tj, oracle):
+    """T large enough that pairs across neighbouring clusters are inside:
+    the ball bound must keep every such panel pair."""
+    rng = np.random.default_rng(22)
+    sets = [separated(rng, 2500, 16, 6, 12.0), separated(rng, 1500, 64, 3, 20.0)]
+    Ts = []
+    for X in sets:
+        M = oracle.d2_matrix(X[:400], X[:400])
+        Ts.append(float(np.quantile(M[M > 0], 0.4)))  # reaches into other clusters
+    Ts = np.asarray(Ts)
+    say("tile pruning, wide T")
+    got, st = tj.batched_join(sets, T=Ts, precision="fp16", audit=True, return_stats=True, prune=True)
+    assert st["violations"] == 0
+    check_all(oracle, sets, Ts, got, "wide T")
+
+
+def test_tile_pruning_degenerate_rows(tj, oracle):
+    """All-equal rows, duplicated rows at T = 0, and huge / tiny magnitudes:
+    the balls degenerate (radius 0) or overflow, pruning stays sound."""
+    rng = np.random.default_rng(23)
+    A = np.ones((1200, 16), np.float32)
+    B = separated(rng, 1000, 64, 4, 96.0)
+    B[500:] = B[:500]
+    C = separated(rng, 900, 32, 4, 68.0) * np.float32(1e30)
+    D = separated(rng, 900, 32, 4, 68.0) * np.float32(1e-30)
+    sets = [A, B, C, D]
+    Ts = np.array([0.0, 0.0, float(eps_for(rng, C, 0.02)) ** 2, float(eps_for(rng, D, 0.02)) ** 2])
+    say("tile pruning, degenerate rows")
+    got, st = tj.batched_join(sets, T=Ts, precision="fp16", audit=True, return_stats=True, prune=True)
+    assert st["violations"] == 0
+    check_all(oracle, sets, Ts, got, "degenerate")
+    assert got[0].shape[0] == 1200 * 1199 // 2
