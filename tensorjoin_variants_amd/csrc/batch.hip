@@ -84,11 +84,12 @@ int rows_per_wg(int combo) {
     }
 }
 
-// Work units of the two per-row passes.  A group of GW = group_width(dp)
-// lanes handles one row, so a wave holds 64 / GW rows of one join; the
-// row-metadata pass takes one such wave-row-set per unit, the scale scan
-// SCAN_SUB of them (one atomic per unit).  chunk prefix tables: units of
-// joins 0..t-1 (device int64 [joins + 1]).
+// Work units of the per-row passes.  A group of GW = group_width(dp) lanes
+// handles one row, so a wave holds 64 / GW rows of one join at a time; the
+// scale scan (and the spatial order) take SCAN_SUB such row sets per unit
+// (one atomic per unit), the row-metadata pass 64 rows per unit (lane L
+// finishes row L).  Chunk prefix tables: units of joins 0..t-1 (device
+// int64 [joins + 1]).
 constexpr int SCAN_SUB = 8;
 
 __device__ inline int find_join(const int64_t* __restrict__ cs, int nj, int64_t c) {
@@ -476,15 +477,6 @@ __global__ void batch_remap_kernel(uint64_t* __restrict__ keys, const uint64_t* 
 // ----------------------------------------------------------- row metadata
 // One unit (64 / GW rows of one join) per wave, written into its group's
 // arrays.
-template <int PREC, int GW>
-__device__ inline void rowmeta_unit(const float* xs_t, const BatchJoin& J, const JoinParams& P, int64_t local,
-                                    int lg, uint8_t* q, RowMeta* meta, const uint64_t* perm, uint64_t jmask) {
-    const bool act = local < J.n;
-    const int64_t lc = act ? local : J.n - 1;  // past the end: shuffle along, store nothing
-    const int64_t src = perm ? perm_of(perm, J.gbase + lc, jmask) : lc;
-    tjmeta::rowmeta_row<PREC, GW>(xs_t + src * (int64_t)J.d, J.grow + lc, J.d, J.dp, P, lg, q, q, meta, act);
-}
-
 __global__ void __launch_bounds__(256) batch_rowmeta_kernel(const float* const* __restrict__ xs,
                                                             const BatchJoin* __restrict__ joins,
                                                             const BatchGroup* __restrict__ groups,
@@ -501,19 +493,23 @@ __global__ void __launch_bounds__(256) batch_rowmeta_kernel(const float* const* 
     uint8_t* q = ws + G.q_off;
     RowMeta* meta = (RowMeta*)(ws + G.meta_off);
     const int gw = tjmeta::group_width(J.dp);
-    const int64_t local = (c - cstart[t]) * (64 / gw) + lane / gw;
-    const int lg = lane % gw;
+    const int64_t r0 = (c - cstart[t]) * 64;
     const float* x = xs[t];
     const uint64_t jm = (1ull << jbits) - 1;
+    const int64_t gb = J.gbase;
+    const int d = J.d;
+    // the caller's row behind position r (rows are read through the spatial
+    // order when there is one)
+    auto src = [=](int64_t r) { return x + (perm ? perm_of(perm, gb + r, jm) : r) * (int64_t)d; };
     if (J.prec == P_FP8) {
-        // FP8 widths are at least 64: one row per wave
-        rowmeta_unit<P_FP8, 64>(x, J, P, local, lg, q, meta, perm, jm);
+        // FP8 widths are at least 64: one row per step
+        tjmeta::rowmeta_wave<P_FP8, 64>(src, J.n, r0, d, J.dp, P, lane, q, q, meta, J.grow);
     } else if (gw == 16) {
-        rowmeta_unit<P_FP16, 16>(x, J, P, local, lg, q, meta, perm, jm);
+        tjmeta::rowmeta_wave<P_FP16, 16>(src, J.n, r0, d, J.dp, P, lane, q, q, meta, J.grow);
     } else if (gw == 32) {
-        rowmeta_unit<P_FP16, 32>(x, J, P, local, lg, q, meta, perm, jm);
+        tjmeta::rowmeta_wave<P_FP16, 32>(src, J.n, r0, d, J.dp, P, lane, q, q, meta, J.grow);
     } else {
-        rowmeta_unit<P_FP16, 64>(x, J, P, local, lg, q, meta, perm, jm);
+        tjmeta::rowmeta_wave<P_FP16, 64>(src, J.n, r0, d, J.dp, P, lane, q, q, meta, J.grow);
     }
 }
 
@@ -996,7 +992,7 @@ int tj_batch_create(tj_batch** out, int joins, const int64_t* n, const int* d, c
     std::vector<int64_t> cmeta(joins + 1, 0), cscan(joins + 1, 0);
     for (int t = 0; t < joins; ++t) {
         const int rpw = 64 / tjmeta::group_width(J[t].dp);
-        cmeta[t + 1] = cmeta[t] + (J[t].n + rpw - 1) / rpw;
+        cmeta[t + 1] = cmeta[t] + (J[t].n + 63) / 64;
         cscan[t + 1] = cscan[t] + (J[t].n + (int64_t)SCAN_SUB * rpw - 1) / ((int64_t)SCAN_SUB * rpw);
     }
     b->units_meta = cmeta[joins];

@@ -48,13 +48,13 @@ __global__ void __launch_bounds__(256) rowmeta_kernel(const float* __restrict__ 
                                                       int dp, const JoinParams* __restrict__ pp,
                                                       uint8_t* __restrict__ qhi, uint8_t* __restrict__ qlo,
                                                       RowMeta* __restrict__ meta) {
-    // 64 / GW rows per wave, 4 waves per block
+    // 64 rows per wave, 4 waves per block
     const int lane = threadIdx.x & 63;
-    const int64_t row = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * (64 / GW) + lane / GW;
-    const bool act = row < n;
+    const int64_t r0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 64;
+    if (r0 >= n) return;  // wave-uniform
     const JoinParams P = *pp;
-    const int64_t rc = act ? row : n - 1;
-    tjmeta::rowmeta_row<PREC, GW>(x + rc * (int64_t)d, rc, d, dp, P, lane % GW, qhi, qlo, meta, act);
+    tjmeta::rowmeta_wave<PREC, GW>([=](int64_t r) { return x + r * (int64_t)d; }, n, r0, d, dp, P, lane, qhi, qlo,
+                                   meta, 0);
 }
 
 __global__ void fill_u64_kernel(uint64_t* p, int64_t n, uint64_t v) {
@@ -86,7 +86,7 @@ int launch_rowmeta(const float* x, int64_t n, int d, int dp, int prec, const Joi
     (void)stats;
     if (n == 0) return TJ_OK;
     const int gw = tjmeta::group_width(dp);
-    const int64_t rows_per_block = 4 * (64 / gw);
+    const int64_t rows_per_block = 4 * 64;
     dim3 grid((unsigned)((n + rows_per_block - 1) / rows_per_block)), block(256);
     uint8_t* h = (uint8_t*)qhi;
     uint8_t* l = (uint8_t*)qlo;

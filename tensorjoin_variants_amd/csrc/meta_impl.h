@@ -76,16 +76,33 @@ __device__ inline double group_max(double x) {
 // Lanes per row for a padded width: no idle lanes for narrow rows.
 __host__ __device__ inline int group_width(int dp) { return dp <= 16 ? 16 : (dp <= 32 ? 32 : 64); }
 
-// act == false: the group takes part in the shuffles but stores nothing
-// (a group past the last row of its join).
+// The per-row sums of one row (binary64), held by every lane of its group.
+struct RowSums {
+    double nx, nh, nl, nr;  // |x|^2, |hi|^2, |lo|^2, |residual|^2 (join units)
+    int e_hi, e_lo;         // FP8 per-row scale exponents
+};
+
+__device__ inline RowSums shfl_sums(const RowSums& s, int src) {
+    RowSums t;
+    t.nx = __shfl(s.nx, src);
+    t.nh = __shfl(s.nh, src);
+    t.nl = __shfl(s.nl, src);
+    t.nr = __shfl(s.nr, src);
+    t.e_hi = __shfl(s.e_hi, src);
+    t.e_lo = __shfl(s.e_lo, src);
+    return t;
+}
+
+// Quantize one row (its GW lanes; lane t of the group handles coordinates
+// t, t+GW, ... of the padded row), write its codes at index `row`, and return
+// its sums.  act == false: the group takes part in the shuffles but stores
+// nothing (a group past the last row).
 template <int PREC, int GW = 64>
-__device__ inline void rowmeta_row(const float* __restrict__ xr, int64_t row, int d, int dp, const JoinParams& P, int lane,
-                            uint8_t* __restrict__ qhi, uint8_t* __restrict__ qlo, RowMeta* __restrict__ meta,
-                            bool act = true) {
+__device__ inline RowSums rowmeta_sums(const float* __restrict__ xr, int64_t row, int d, int dp, const JoinParams& P,
+                                       int lane, uint8_t* __restrict__ qhi, uint8_t* __restrict__ qlo, bool act) {
     constexpr int TERMS = (PREC == P_FP16X3 || PREC == P_BF16X3 || PREC == P_FP8X3) ? 3 : 1;
     constexpr bool FP8 = (PREC == P_FP8 || PREC == P_FP8X3);
     constexpr bool BF16 = (PREC == P_BF16 || PREC == P_BF16X3);
-    constexpr int ES = FP8 ? 1 : 2;
     const double sc = ldexp(1.0, -P.G);
 
     // FP8: the per-row scale needs the row max first.
@@ -136,12 +153,21 @@ __device__ inline void rowmeta_row(const float* __restrict__ xr, int64_t row, in
         nl += bl * bl;
         nr += r * r;
     }
-    nx = group_sum<GW>(nx);
-    nh = group_sum<GW>(nh);
-    nl = group_sum<GW>(nl);
-    nr = group_sum<GW>(nr);
-    if (lane != 0 || !act) return;
-    (void)ES;
+    RowSums out;
+    out.nx = group_sum<GW>(nx);
+    out.nh = group_sum<GW>(nh);
+    out.nl = group_sum<GW>(nl);
+    out.nr = group_sum<GW>(nr);
+    out.e_hi = e_hi;
+    out.e_lo = e_lo;
+    return out;
+}
+
+// A row's certificate terms from its sums (one lane per row).
+template <int PREC>
+__device__ inline RowMeta rowmeta_terms(const RowSums& S, int dp, const JoinParams& P) {
+    constexpr int TERMS = (PREC == P_FP16X3 || PREC == P_BF16X3 || PREC == P_FP8X3) ? 3 : 1;
+    const double nx = S.nx, nh = S.nh, nl = S.nl, nr = S.nr;
     const double rel = (double)(dp + 4) * 0x1p-52;  // sum of dp non-negative terms, each squared once
     const double root_pad = 1.0 + 0x1p-48;           // sqrt rounding
     const float nlo = f32_down(nx * (1.0 - rel));
@@ -186,10 +212,37 @@ __device__ inline void rowmeta_row(const float* __restrict__ xr, int64_t row, in
     w += fabs(w) * 0x1p-50 + 0x1p-1000;
     RowMeta m;
     m.m0 = make_float4(nlo, nhi, H, E);
-    uint32_t packed = (uint32_t)(e_hi + 32768) | ((uint32_t)(e_lo + 32768) << 16);
+    uint32_t packed = (uint32_t)(S.e_hi + 32768) | ((uint32_t)(S.e_lo + 32768) << 16);
     m.m1 = make_float4(L, t32, v32, __builtin_bit_cast(float, packed));
     m.m2 = make_float4(f32_up(w), f32_up(a), 0.0f, 0.0f);
-    meta[row] = m;
+    return m;
+}
+
+// One wave, 64 consecutive rows [r0, r0 + 64) of a range of n rows: the
+// rows are quantized GW lanes each (64 / GW at a time), lane L collects row
+// r0 + L's sums, and then every lane derives its own row's terms -- the
+// binary64 tail runs on 64 lanes at once instead of one lane per group.
+// src(r): the input row r of the range; row r's codes and metadata go to
+// index base + r.
+template <int PREC, int GW, class Src>
+__device__ inline void rowmeta_wave(Src src, int64_t n, int64_t r0, int d, int dp, const JoinParams& P, int lane,
+                                    uint8_t* __restrict__ qhi, uint8_t* __restrict__ qlo, RowMeta* __restrict__ meta,
+                                    int64_t base) {
+    constexpr int RPW = 64 / GW;
+    const int g = lane / GW, lg = lane % GW;
+    RowSums mine{};
+    for (int step = 0; step < GW; ++step) {
+        const int64_t first = r0 + (int64_t)step * RPW;
+        if (first >= n) break;  // uniform
+        const int64_t r = first + g;
+        const bool act = r < n;
+        const int64_t rc = act ? r : n - 1;
+        const RowSums s = rowmeta_sums<PREC, GW>(src(rc), base + rc, d, dp, P, lg, qhi, qlo, act);
+        const RowSums t = shfl_sums(s, (lane % RPW) * GW);  // lane L <- the group of row r0 + L
+        if (lane / RPW == step) mine = t;
+    }
+    const int64_t my = r0 + lane;
+    if (my < n) meta[base + my] = rowmeta_terms<PREC>(mine, dp, P);
 }
 
 }  // namespace tjmeta
