@@ -560,13 +560,20 @@ __device__ __forceinline__ void screen_panel(const PanelIO& io, const CandOut& c
                     const float sv = acc[rb][r];
                     const uint64_t m = ms[r];
                     if (m == 0) continue;
-                    const bool mine = (m >> lane) & 1;
+                    // the lane's own survivor test: the compare itself off the
+                    // diagonal (one VALU op), the staged mask on it
+                    const bool mine = DIAG ? ((m >> lane) & 1) != 0 : sv >= nj;
                     const int k = __popcll(m);
                     nC = uni(nC);  // wave-uniform: keeps the capacity test a scalar branch
                     if (__builtin_expect(nC + k > CC, 0)) cand_flush(co, sCK, sCP, nC, nAcc, lane);  // out of line
                     if (mine) {
                         const int pos = nC + lanes_below(m);
-                        uint64_t key = kb + ((uint64_t)off << io.jbits);
+                        // the row offset's key increment, formed here by one
+                        // scalar shift: hoisted out of the column loop, the 16
+                        // increments would be spilled to VGPR lanes
+                        int js = io.jbits;
+                        __asm__ volatile("" : "+s"(js));
+                        uint64_t key = kb + ((uint64_t)off << js);
                         if (sv >= uj) key |= ACCEPT_BIT;
                         sCK[pos] = key;
                         sCP[pos] = sv;
