@@ -262,42 +262,34 @@ __global__ void __launch_bounds__(256) batch_pivot_kernel(const float* const* __
 // jbits) | local row.  Sorted, position k of the batch row space holds the
 // key of the row placed there (the joins stay contiguous and in order).
 // One scan unit (SCAN_SUB groups of 64 / GW rows of one join) per wave: the
-// pivots' coordinates of a lane (at most four: d <= 256 and GW >= d / 4 or
-// GW = 64) are loaded once per unit.
-template <int GW>
+// pivots' coordinates of a lane are loaded once per unit.  The nearest pivot
+// is taken on the first ORDER_DIMS coordinates only (the order only steers
+// how much the tile bounds can skip; any order is correct), so a row costs
+// one 128-byte read however wide it is.
+constexpr int ORDER_DIMS = 32;
+// OGW = min(GW, 32) lanes per row, one coordinate each; `rows` rows from
+// `first`.
+template <int OGW>
 __device__ inline void order_unit(const float* x, const BatchJoin& J, int t, const int32_t* piv, int64_t first,
-                                  int lane, int jbits, uint64_t* keys) {
-    constexpr int MC = GW == 64 ? 4 : 1;  // coordinates per lane (GW = 16 / 32 serve d <= GW)
-    const int lg = lane % GW, g = lane / GW;
-    float pv[NPIV][MC];
+                                  int rows, int lane, int jbits, uint64_t* keys) {
+    constexpr int RPS = 64 / OGW;  // rows per step
+    const int dd = J.d < ORDER_DIMS ? J.d : ORDER_DIMS;
+    const int lg = lane % OGW, g = lane / OGW;
+    float pv[NPIV];
 #pragma unroll
-    for (int k = 0; k < NPIV; ++k)
-#pragma unroll
-        for (int m = 0; m < MC; ++m) {
-            const int c = lg + GW * m;
-            pv[k][m] = c < J.d ? x[(int64_t)piv[k] * J.d + c] : 0.0f;
-        }
-    for (int sub = 0; sub < SCAN_SUB; ++sub) {
-        const int64_t local = first + sub * (64 / GW) + g;
-        const bool act = local < J.n;
+    for (int k = 0; k < NPIV; ++k) pv[k] = lg < dd ? x[(int64_t)piv[k] * J.d + lg] : 0.0f;
+    for (int step = 0; step * RPS < rows; ++step) {
+        const int64_t local = first + step * RPS + g;
+        const bool act = step * RPS + g < rows && local < J.n;
         const int64_t lc = act ? local : J.n - 1;
-        float xv[MC];
-#pragma unroll
-        for (int m = 0; m < MC; ++m) {
-            const int c = lg + GW * m;
-            xv[m] = c < J.d ? x[lc * J.d + c] : 0.0f;
-        }
+        const float xv = lg < dd ? x[lc * J.d + lg] : 0.0f;
         float best = INFINITY;
         int bk = 0;
 #pragma unroll
         for (int k = 0; k < NPIV; ++k) {
-            float acc = 0.0f;
-#pragma unroll
-            for (int m = 0; m < MC; ++m) {
-                const float u = xv[m] - pv[k][m];
-                acc += u * u;
-            }
-            for (int o = GW / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+            const float u = xv - pv[k];
+            float acc = u * u;
+            for (int o = OGW / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
             if (acc < best) { best = acc; bk = k; }  // NaN rows stay in bucket 0
         }
         if (act && lg == 0)
@@ -316,11 +308,11 @@ __global__ void __launch_bounds__(256) batch_order_kernel(const float* const* __
     const int t = find_join(cstart, nj, c);
     const BatchJoin J = joins[t];
     const int gw = tjmeta::group_width(J.dp);
-    const int64_t first = (c - cstart[t]) * (SCAN_SUB * (64 / gw));
+    const int rows = SCAN_SUB * (64 / gw);  // the scan unit's rows
+    const int64_t first = (c - cstart[t]) * rows;
     const int32_t* piv = pivots + t * NPIV;
-    if (gw == 16) order_unit<16>(xs[t], J, t, piv, first, lane, jbits, keys);
-    else if (gw == 32) order_unit<32>(xs[t], J, t, piv, first, lane, jbits, keys);
-    else order_unit<64>(xs[t], J, t, piv, first, lane, jbits, keys);
+    if (gw == 16) order_unit<16>(xs[t], J, t, piv, first, rows, lane, jbits, keys);
+    else order_unit<32>(xs[t], J, t, piv, first, rows, lane, jbits, keys);
 }
 
 // One workgroup per work item (join t, panel p): the panel's bounding ball
@@ -344,10 +336,13 @@ __global__ void __launch_bounds__(256) batch_ball_kernel(const float* const* __r
     const int64_t r1 = r0 + J.bm < J.n ? r0 + J.bm : J.n;
     const float* x = xs[it.x];
     const int d = J.d;  // <= 256: at most four coordinates per lane
-    // centre: each wave sums every fourth row, lanes across the row
+    // centre: the mean of up to 32 rows spread over the panel (any centre is
+    // valid; the radius below is taken over every row), 8 per wave
+    const int64_t nr = r1 - r0;
+    const int ns = nr < 32 ? (int)nr : 32;
     float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll 4
-    for (int64_t r = r0 + wave; r < r1; r += 4) {
+    for (int sidx = wave; sidx < ns; sidx += 4) {
+        const int64_t r = r0 + (int64_t)sidx * nr / ns;
         const float* xr = x + perm_of(perm, J.gbase + r, jmask) * d;
 #pragma unroll
         for (int m = 0; m < 4; ++m)
@@ -357,7 +352,7 @@ __global__ void __launch_bounds__(256) batch_ball_kernel(const float* const* __r
     for (int m = 0; m < 4; ++m)
         if (lane + 64 * m < d) spart[wave][lane + 64 * m] = acc[m];
     __syncthreads();
-    if (tid < d) sc[tid] = ((spart[0][tid] + spart[1][tid]) + (spart[2][tid] + spart[3][tid])) / (float)(r1 - r0);
+    if (tid < d) sc[tid] = ((spart[0][tid] + spart[1][tid]) + (spart[2][tid] + spart[3][tid])) / (float)ns;
     __syncthreads();
     float cl[4];
 #pragma unroll
