@@ -265,10 +265,9 @@ __global__ void __launch_bounds__(256) batch_pivot_kernel(const float* const* __
 // pivots' coordinates of a lane are loaded once per unit.  The nearest pivot
 // is taken on the first ORDER_DIMS coordinates only (the order only steers
 // how much the tile bounds can skip; any order is correct), so a row costs
-// one 128-byte read however wide it is.
-constexpr int ORDER_DIMS = 32;
-// OGW = min(GW, 32) lanes per row, one coordinate each; `rows` rows from
-// `first`.
+// one 64-byte read however wide it is.
+constexpr int ORDER_DIMS = 16;
+// OGW = 16 lanes per row, one coordinate each; `rows` rows from `first`.
 template <int OGW>
 __device__ inline void order_unit(const float* x, const BatchJoin& J, int t, const int32_t* piv, int64_t first,
                                   int rows, int lane, int jbits, uint64_t* keys) {
@@ -311,8 +310,7 @@ __global__ void __launch_bounds__(256) batch_order_kernel(const float* const* __
     const int rows = SCAN_SUB * (64 / gw);  // the scan unit's rows
     const int64_t first = (c - cstart[t]) * rows;
     const int32_t* piv = pivots + t * NPIV;
-    if (gw == 16) order_unit<16>(xs[t], J, t, piv, first, rows, lane, jbits, keys);
-    else order_unit<32>(xs[t], J, t, piv, first, rows, lane, jbits, keys);
+    order_unit<ORDER_DIMS>(xs[t], J, t, piv, first, rows, lane, jbits, keys);
 }
 
 // One workgroup per work item (join t, panel p): the panel's bounding ball
@@ -354,30 +352,43 @@ __global__ void __launch_bounds__(256) batch_ball_kernel(const float* const* __r
     __syncthreads();
     if (tid < d) sc[tid] = ((spart[0][tid] + spart[1][tid]) + (spart[2][tid] + spart[3][tid])) / (float)ns;
     __syncthreads();
-    float cl[4];
-#pragma unroll
-    for (int m = 0; m < 4; ++m) cl[m] = lane + 64 * m < d ? sc[lane + 64 * m] : 0.0f;
-    double rmax = 0.0;
-#pragma unroll 4
-    for (int64_t r = r0 + wave; r < r1; r += 4) {
-        const float* xr = x + perm_of(perm, J.gbase + r, jmask) * d;
-        double a = 0.0;
-#pragma unroll
-        for (int m = 0; m < 4; ++m) {
-            if (lane + 64 * m < d) {
-                const double u = (double)xr[lane + 64 * m] - (double)cl[m];
-                a += u * u;
-            }
+    // radius: 16 lanes per row (4 rows per wave at a time), binary32 sums.
+    // Every difference and square of binary32 values is rounded once and
+    // all terms are non-negative, so the exact |x - c|^2 <= (S + (d+2)
+    // 2^-125) / (1 - (d+2) 2^-24) whatever the summation order.  A NaN or
+    // infinite sum (centre overflow) gives an infinite radius: no pruning.
+    const int g16 = lane >> 4, l16 = lane & 15;
+    float rmax = 0.0f;
+    bool bad = false;
+    for (int64_t rb = r0 + wave * 4; rb < r1; rb += 16) {
+        const int64_t r = rb + g16;
+        const bool act = r < r1;
+        const float* xr = x + perm_of(perm, J.gbase + (act ? r : r0), jmask) * d;
+        float a = 0.0f;
+        for (int k = l16; k < d; k += 16) {
+            const float u = xr[k] - sc[k];
+            a += u * u;
         }
-        for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o);
-        rmax = fmax(rmax, a);
+        a += __shfl_xor(a, 8);
+        a += __shfl_xor(a, 4);
+        a += __shfl_xor(a, 2);
+        a += __shfl_xor(a, 1);
+        if (act) {
+            bad |= !(a < INFINITY);
+            rmax = fmaxf(rmax, a);
+        }
     }
-    if (lane == 0) sr[wave] = rmax;
+    bad = __any(bad);
+    for (int o = 32; o > 0; o >>= 1) rmax = fmaxf(rmax, __shfl_xor(rmax, o));
+    if (lane == 0) sr[wave] = bad ? (double)INFINITY : (double)rmax;
     __syncthreads();
     float* ball = balls + J.ball_off + (int64_t)it.y * (d + 1);
     if (tid < d) ball[tid] = sc[tid];
     if (tid == 0) {
-        const double r2 = fmax(fmax(sr[0], sr[1]), fmax(sr[2], sr[3])) * (1.0 + (double)(d + 4) * 0x1p-52);
+        const double S = fmax(fmax(sr[0], sr[1]), fmax(sr[2], sr[3]));  // no NaN here
+        const double n = (double)d + 2.0;
+        double r2 = (S + n * 0x1p-125) / (1.0 - n * 0x1p-24);
+        r2 += r2 * 0x1p-50;
         ball[d] = f32_up(sqrt(r2) * (1.0 + 0x1p-50));
     }
 }

@@ -248,8 +248,12 @@ def test_tile_pruning_degenerate_rows(tj, oracle):
     B[500:] = B[:500]
     C = separated(rng, 900, 32, 4, 68.0) * np.float32(1e30)
     D = separated(rng, 900, 32, 4, 68.0) * np.float32(1e-30)
-    sets = [A, B, C, D]
-    Ts = np.array([0.0, 0.0, float(eps_for(rng, C, 0.02)) ** 2, float(eps_for(rng, D, 0.02)) ** 2])
+    # +-3e38 coordinates: panel centres overflow (inf or NaN); no pruning there
+    E = (rng.choice([-3.0e38, 3.0e38], size=(700, 16)) * rng.choice([1.0, 0.5], size=(700, 16))).astype(np.float32)
+    E[350:] = E[:350]
+    sets = [A, B, C, D, E]
+    Ts = np.array([0.0, 0.0, float(eps_for(rng, C, 0.02)) ** 2, float(eps_for(rng, D, 0.02)) ** 2,
+                   float(np.float64(3.0e38) ** 2)])
     say("tile pruning, degenerate rows")
     got, st = tj.batched_join(sets, T=Ts, precision="fp16", audit=True, return_stats=True, prune=True)
     assert st["violations"] == 0
