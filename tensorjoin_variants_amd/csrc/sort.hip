@@ -103,6 +103,7 @@ __global__ void __launch_bounds__(SB) scatter_kernel(const uint64_t* __restrict_
     __shared__ uint32_t run[256];         // next global position per digit
     __shared__ uint32_t tstart[256];      // first tile position per digit
     __shared__ uint32_t ttot[256];        // keys per digit in the tile
+    __shared__ uint32_t wsum[4];          // per-wave sums of the digit-total scan
     __shared__ uint32_t wcnt[KPT][4][256];  // per (slot, wave) digit counts, then offsets
     __shared__ uint64_t sorted[TILE];
     const int tid = threadIdx.x, wave = tid >> 6;
@@ -138,7 +139,9 @@ __global__ void __launch_bounds__(SB) scatter_kernel(const uint64_t* __restrict_
             if (valid[k] && rank[k] == 0) wcnt[k][wave][dg[k]] = (uint32_t)__popcll(peers);
         }
         __syncthreads();
-        // per digit: exclusive offsets over (slot, wave) in tile order
+        // per digit: exclusive offsets over (slot, wave) in tile order, then
+        // the exclusive scan of the tile's digit totals: each wave scans its
+        // 64 digits in registers, one barrier joins the four wave sums
         {
             uint32_t acc = 0;
 #pragma unroll
@@ -150,17 +153,19 @@ __global__ void __launch_bounds__(SB) scatter_kernel(const uint64_t* __restrict_
                     acc += c;
                 }
             ttot[tid] = acc;
-            tstart[tid] = acc;
-        }
-        __syncthreads();
-        // exclusive scan of the tile's digit totals (Hillis-Steele on tstart)
-        for (int o = 1; o < 256; o <<= 1) {
-            const uint32_t v = tid >= o ? tstart[tid - o] : 0;
+            uint32_t v = acc;
+            const int lane = tid & 63;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t u = __shfl_up(v, o);
+                if (lane >= o) v += u;
+            }
+            if (lane == 63) wsum[wave] = v;
             __syncthreads();
-            tstart[tid] += v;
-            __syncthreads();
+            uint32_t add = 0;
+            for (int w = 0; w < wave; ++w) add += wsum[w];
+            tstart[tid] = add + v - ttot[tid];
         }
-        tstart[tid] -= ttot[tid];
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < KPT; ++k)
