@@ -392,8 +392,9 @@ static int join_enqueue(const JoinCall& c, hipStream_t s) {
         TJ_MARK(2);
         TJ_MARK(3);
     }
-    if ((rc = launch_sort(ws + p->off_sort, out_keys, stats + TJ_ST_OUT, p->out_cap, p->keybits, s))) return rc;
-    if ((rc = launch_unpack(out_keys, stats + TJ_ST_OUT, p->out_cap, p->jbits, pairs, out_format, s))) return rc;
+    uint64_t* sorted = out_keys;
+    if ((rc = launch_sort(ws + p->off_sort, out_keys, stats + TJ_ST_OUT, p->out_cap, p->keybits, s, &sorted))) return rc;
+    if ((rc = launch_unpack(sorted, stats + TJ_ST_OUT, p->out_cap, p->jbits, pairs, out_format, s))) return rc;
     TJ_MARK(4);
 #undef TJ_MARK
     return TJ_OK;

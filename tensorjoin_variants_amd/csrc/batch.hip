@@ -841,12 +841,13 @@ static int batch_enqueue(const BatchCall& c, hipStream_t s) {
         TJ_HIP(hipGetLastError());
     }
     TJ_MARK(3);
-    if ((rc = launch_sort(ws + b->off_sort, out_keys, stats + TJ_ST_OUT, b->out_cap, b->keybits, s))) return rc;
+    uint64_t* sorted = out_keys;
+    if ((rc = launch_sort(ws + b->off_sort, out_keys, stats + TJ_ST_OUT, b->out_cap, b->keybits, s, &sorted))) return rc;
     {
         int64_t work = std::max<int64_t>(b->out_cap, b->nj + 1);
         int64_t blocks = (work + 255) / 256;
         if (blocks > 2048) blocks = 2048;
-        hipLaunchKernelGGL(batch_unpack_kernel, dim3((unsigned)blocks), dim3(256), 0, s, out_keys, stats + TJ_ST_OUT,
+        hipLaunchKernelGGL(batch_unpack_kernel, dim3((unsigned)blocks), dim3(256), 0, s, sorted, stats + TJ_ST_OUT,
                            b->out_cap, b->jbits, b->d_row_join, b->d_joins, b->nj, c.pairs, c.offsets);
         TJ_HIP(hipGetLastError());
     }

@@ -229,7 +229,9 @@ size_t sort_temp_bytes(int64_t capacity) {
     return a + (size_t)256 * nblocks_for(capacity) * 4 + 512 * 4 + 256;
 }
 
-int launch_sort(void* temp, uint64_t* keys, const uint64_t* count, int64_t capacity, int bits, hipStream_t s) {
+int launch_sort(void* temp, uint64_t* keys, const uint64_t* count, int64_t capacity, int bits, hipStream_t s,
+                uint64_t** result) {
+    if (result) *result = keys;
     if (capacity <= 0 || bits <= 0) return TJ_OK;
     const int nb = nblocks_for(capacity);
     uint64_t* alt = (uint64_t*)temp;
@@ -245,7 +247,9 @@ int launch_sort(void* temp, uint64_t* keys, const uint64_t* count, int64_t capac
         hipLaunchKernelGGL(scatter_kernel, dim3(nb), dim3(SB), 0, s, src, dst, count, capacity, 8 * p, hist, tot + 256, nb);
         uint64_t* t = src; src = dst; dst = t;
     }
-    if (src != keys) {
+    if (result) {
+        *result = src;
+    } else if (src != keys) {
         int64_t blocks = (capacity + 255) / 256;
         if (blocks > 2048) blocks = 2048;
         hipLaunchKernelGGL(copy_kernel, dim3((unsigned)blocks), dim3(256), 0, s, src, keys, count, capacity);

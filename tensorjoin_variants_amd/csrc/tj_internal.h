@@ -242,7 +242,11 @@ int launch_certify(const float* R, const float* S, int d, int jbits, const uint6
 int launch_plain(const float* R, int64_t nR, const float* S, int64_t nS, int d, int jbits, double T,
                  bool self_join, uint64_t* out_keys, int64_t out_cap, uint64_t* stats, hipStream_t s);
 size_t sort_temp_bytes(int64_t capacity);
-int launch_sort(void* temp, uint64_t* keys, const uint64_t* count, int64_t capacity, int bits, hipStream_t s);
+// result == NULL: the sorted keys end in `keys`; otherwise *result names the
+// buffer holding them (keys or the ping-pong half of temp), saving the copy
+// back after an odd number of passes.
+int launch_sort(void* temp, uint64_t* keys, const uint64_t* count, int64_t capacity, int bits, hipStream_t s,
+                uint64_t** result = nullptr);
 int launch_unpack(const uint64_t* keys, const uint64_t* count, int64_t cap, int jbits, void* pairs,
                   int out_format, hipStream_t s);
 int launch_fill_u64(uint64_t* p, int64_t n, uint64_t v, hipStream_t s);
