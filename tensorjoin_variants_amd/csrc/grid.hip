@@ -11,9 +11,11 @@
 //    (memory bounded by 2|S| slots however sparse the grid is);
 //  * queries run in cell order (R is sorted by cell too), so the 64 lanes of
 //    a wave visit the same neighbour cells and the same S rows;
-//  * output without a global sort: a count pass, an exclusive scan, a write
-//    pass at exact per-row offsets, and a per-row sort of the j's, so the
-//    (i, j) rows come out in order;
+//  * output: a count pass, an exclusive scan, a write pass of pair keys at
+//    exact per-row offsets (rows come out in i order), and one radix sort of
+//    the keys for the j's within each row (a per-row insertion sort was
+//    quadratic in the longest row: the dense blobs made it the costliest
+//    kernel);
 //  * the predicate is the sequential binary64 difference form of the CPU
 //    oracle (d2 <= T; GDS used sqrt(d2) <= eps, GDS/kernel.cu:368-371).
 // The cell side is sqrt(T) * (1 + 2^-10) (or larger when the cell ids would
@@ -190,14 +192,14 @@ __device__ inline double d2_row(const float* q, const float4& y, int d) {
 }
 
 // One query per lane, queries in cell order.  mode 0: count into cnt[i];
-// mode 1: write the j's at out_j[off[i] + k].
+// mode 1: write the pair keys (i << jbits) | j at out_keys[off[i] + k].
 template <int MODE>
 __global__ void __launch_bounds__(256) probe_kernel(const float* __restrict__ R, int64_t nR, const GridParams* __restrict__ gp,
                                                     const uint64_t* __restrict__ rkeys, const float4* __restrict__ srow,
                                                     const uint64_t* __restrict__ hkey, const uint2* __restrict__ hval,
                                                     uint64_t hmask, double T, int self_join, uint32_t* __restrict__ cnt,
-                                                    const uint64_t* __restrict__ off, uint32_t* __restrict__ out_j,
-                                                    int64_t out_cap, uint64_t* __restrict__ stats) {
+                                                    const uint64_t* __restrict__ off, uint64_t* __restrict__ out_keys,
+                                                    int jbits, int64_t out_cap, uint64_t* __restrict__ stats) {
     const GridParams g = *gp;
     const uint64_t imask = (1ull << g.idxbits) - 1;
     uint64_t cand = 0;
@@ -223,7 +225,7 @@ __global__ void __launch_bounds__(256) probe_kernel(const float* __restrict__ R,
                         if (self_join && j <= i) continue;
                         ++cand;
                         if (d2_row(q, y, g.d) <= T) {
-                            if (MODE == 1 && base + n < (uint64_t)out_cap) out_j[base + n] = j;
+                            if (MODE == 1 && base + n < (uint64_t)out_cap) out_keys[base + n] = (i << jbits) | j;
                             ++n;
                         }
                     }
@@ -300,28 +302,6 @@ __global__ void __launch_bounds__(256) scan_apply(const uint32_t* __restrict__ i
     }
 }
 
-// Per row: sort its j's (insertion sort; rows are short) and write (i, j).
-__global__ void __launch_bounds__(256) finish_kernel(int64_t nR, const uint32_t* __restrict__ cnt, const uint64_t* __restrict__ off,
-                                                     uint32_t* __restrict__ out_j, int64_t out_cap, void* __restrict__ pairs,
-                                                     int fmt) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nR; i += (int64_t)gridDim.x * blockDim.x) {
-        const uint64_t b = off[i];
-        const uint32_t n = cnt[i];
-        if (b + n > (uint64_t)out_cap) continue;
-        uint32_t* seg = out_j + b;
-        for (uint32_t a = 1; a < n; ++a) {
-            const uint32_t v = seg[a];
-            uint32_t k = a;
-            while (k > 0 && seg[k - 1] > v) { seg[k] = seg[k - 1]; --k; }
-            seg[k] = v;
-        }
-        for (uint32_t a = 0; a < n; ++a) {
-            if (fmt == TJ_OUT_I32) ((int2*)pairs)[b + a] = make_int2((int)i, (int)seg[a]);
-            else ((longlong2*)pairs)[b + a] = make_longlong2((long long)i, (long long)seg[a]);
-        }
-    }
-}
-
 }  // namespace
 
 namespace tj {
@@ -347,14 +327,14 @@ GridLayout grid_layout(int64_t nR, int64_t nS, int64_t out_cap) {
     L.count_ns = carve(8);
     L.rkeys = carve(8 * (size_t)(nR > 0 ? nR : 1));
     L.skeys = carve(8 * (size_t)(nS > 0 ? nS : 1));
-    L.sort_tmp = carve(sort_temp_bytes(nmax > 0 ? nmax : 1));
+    L.sort_tmp = carve(sort_temp_bytes(std::max<int64_t>(std::max<int64_t>(nmax, out_cap), 1)));
     L.srow = carve(16 * (size_t)(nS > 0 ? nS : 1));
     L.hkey = carve(8 * hs);
     L.hval = carve(8 * hs);
     L.cnt = carve(4 * (size_t)(nR > 0 ? nR : 1));
     L.off = carve(8 * (size_t)(nR > 0 ? nR : 1));
     L.bsum = carve(8 * (size_t)((nR + SCAN_B - 1) / SCAN_B + 1));
-    L.outj = carve(4 * (size_t)(out_cap > 0 ? out_cap : 1));
+    L.outj = carve(8 * (size_t)(out_cap > 0 ? out_cap : 1));  // pair keys
     L.total = (cur + 255) / 256 * 256;
     return L;
 }
@@ -383,7 +363,8 @@ int launch_grid_join(const float* R, int64_t nR, const float* S, int64_t nS, int
     uint32_t* cnt = (uint32_t*)(ws + L.cnt);
     uint64_t* off = (uint64_t*)(ws + L.off);
     uint64_t* bsum = (uint64_t*)(ws + L.bsum);
-    uint32_t* outj = (uint32_t*)(ws + L.outj);
+    uint64_t* okeys = (uint64_t*)(ws + L.outj);
+    const int jbits = bits_for(nS > 1 ? nS : 2);
     uint64_t* cR = (uint64_t*)(ws + L.count_nr);
     uint64_t* cS = (uint64_t*)(ws + L.count_ns);
     int rc;
@@ -405,17 +386,20 @@ int launch_grid_join(const float* R, int64_t nR, const float* S, int64_t nS, int
     // count -> scan -> write -> per-row sort
     const unsigned pb = blocks_for(nR, (int64_t)num_cus() * 16);
     hipLaunchKernelGGL(probe_kernel<0>, dim3(pb), dim3(256), 0, s, R, nR, gp, rkeys, srow, hkey, hval,
-                       (uint64_t)(L.hslots - 1), T, (int)self_join, cnt, off, outj, out_cap, stats);
+                       (uint64_t)(L.hslots - 1), T, (int)self_join, cnt, off, okeys, jbits, out_cap, stats);
     const int64_t nb = (nR + SCAN_B - 1) / SCAN_B;
     hipLaunchKernelGGL(scan_block_sums, dim3((unsigned)nb), dim3(256), 0, s, cnt, nR, bsum);
     hipLaunchKernelGGL(scan_sums, dim3(1), dim3(1024), 0, s, bsum, nb, stats);
     hipLaunchKernelGGL(scan_apply, dim3((unsigned)nb), dim3(256), 0, s, cnt, nR, bsum, off);
     hipLaunchKernelGGL(probe_kernel<1>, dim3(pb), dim3(256), 0, s, R, nR, gp, rkeys, srow, hkey, hval,
-                       (uint64_t)(L.hslots - 1), T, (int)self_join, cnt, off, outj, out_cap, stats);
-    hipLaunchKernelGGL(finish_kernel, dim3(blocks_for(nR, gb)), dim3(256), 0, s, nR, cnt, off, outj, out_cap, pairs,
-                       out_format);
+                       (uint64_t)(L.hslots - 1), T, (int)self_join, cnt, off, okeys, jbits, out_cap, stats);
     TJ_HIP(hipGetLastError());
-    return TJ_OK;
+    // rows are already in order (the scan runs over i); one radix sort of the
+    // keys orders each row's j's, then the keys unpack to pairs
+    uint64_t* sorted = okeys;
+    const int keybits = bits_for(nR > 1 ? nR : 2) + jbits;
+    if ((rc = launch_sort(ws + L.sort_tmp, okeys, stats + TJ_ST_OUT, out_cap, keybits, s, &sorted))) return rc;
+    return launch_unpack(sorted, stats + TJ_ST_OUT, out_cap, jbits, pairs, out_format, s);
 }
 
 }  // namespace tj
