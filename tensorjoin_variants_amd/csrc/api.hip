@@ -24,11 +24,6 @@ int screen_fp8(int dp, bool self_join, const ScreenArgs& a, int grid, hipStream_
 int screen_fp8x3(int dp, bool self_join, const ScreenArgs& a, int grid, hipStream_t s, int query);
 int launch_mirror(const int64_t* pairs, const uint64_t* count, int64_t cap, int64_t n, uint64_t* ids, uint64_t* total,
                   hipStream_t s);
-struct GridLayout {
-    size_t bounds, params, rkeys, skeys, sort_tmp, srow, hkey, hval, cnt, off, bsum, outj, count_nr, count_ns, total;
-    uint64_t hslots;
-    int idxbits;
-};
 GridLayout grid_layout(int64_t nR, int64_t nS, int64_t out_cap);
 int launch_grid_join(const float* R, int64_t nR, const float* S, int64_t nS, int d, double T, bool self_join,
                      uint8_t* ws, const GridLayout& L, int64_t out_cap, void* pairs, int out_format, uint64_t* stats,

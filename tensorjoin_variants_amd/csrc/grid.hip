@@ -306,11 +306,6 @@ __global__ void __launch_bounds__(256) scan_apply(const uint32_t* __restrict__ i
 
 namespace tj {
 
-struct GridLayout {
-    size_t bounds, params, rkeys, skeys, sort_tmp, srow, hkey, hval, cnt, off, bsum, outj, count_nr, count_ns, total;
-    uint64_t hslots;
-    int idxbits;
-};
 
 GridLayout grid_layout(int64_t nR, int64_t nS, int64_t out_cap) {
     GridLayout L{};

@@ -213,6 +213,12 @@ inline int bits_for(int64_t n) { int b = 0; while (((int64_t)1 << b) < n) ++b; r
 
 // ------------------------------------------------------------- launchers
 namespace tj {
+// Workspace layout of the grid join (grid.hip), kept in the plan (api.hip).
+struct GridLayout {
+    size_t bounds, params, rkeys, skeys, sort_tmp, srow, hkey, hval, cnt, off, bsum, outj, count_nr, count_ns, total;
+    uint64_t hslots;
+    int idxbits;
+};
 struct ScreenArgs {
     const void* qa_hi; const void* qa_lo; const RowMeta* ma; int64_t na;
     const void* qb_hi; const void* qb_lo; const RowMeta* mb; int64_t nb;
