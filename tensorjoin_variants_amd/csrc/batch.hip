@@ -498,7 +498,7 @@ __global__ void __launch_bounds__(256) batch_rowmeta_kernel(const float* const* 
     const JoinParams P = params[t];
     uint8_t* q = ws + G.q_off;
     RowMeta* meta = (RowMeta*)(ws + G.meta_off);
-    const int gw = tjmeta::group_width(J.dp);
+    const int gw = tjmeta::meta_width(J.dp);
     const int64_t r0 = (c - cstart[t]) * 64;
     const float* x = xs[t];
     const uint64_t jm = (1ull << jbits) - 1;
@@ -508,8 +508,9 @@ __global__ void __launch_bounds__(256) batch_rowmeta_kernel(const float* const* 
     // order when there is one)
     auto src = [=](int64_t r) { return x + (perm ? perm_of(perm, gb + r, jm) : r) * (int64_t)d; };
     if (J.prec == P_FP8) {
-        // FP8 widths are at least 64: one row per step
-        tjmeta::rowmeta_wave<P_FP8, 64>(src, J.n, r0, d, J.dp, P, lane, q, q, meta, J.grow);
+        if (gw == 16) tjmeta::rowmeta_wave<P_FP8, 16>(src, J.n, r0, d, J.dp, P, lane, q, q, meta, J.grow);
+        else if (gw == 32) tjmeta::rowmeta_wave<P_FP8, 32>(src, J.n, r0, d, J.dp, P, lane, q, q, meta, J.grow);
+        else tjmeta::rowmeta_wave<P_FP8, 64>(src, J.n, r0, d, J.dp, P, lane, q, q, meta, J.grow);
     } else if (gw == 16) {
         tjmeta::rowmeta_wave<P_FP16, 16>(src, J.n, r0, d, J.dp, P, lane, q, q, meta, J.grow);
     } else if (gw == 32) {

@@ -85,7 +85,7 @@ int launch_rowmeta(const float* x, int64_t n, int d, int dp, int prec, const Joi
                    void* qlo, RowMeta* meta, uint64_t* stats, hipStream_t s) {
     (void)stats;
     if (n == 0) return TJ_OK;
-    const int gw = tjmeta::group_width(dp);
+    const int gw = tjmeta::meta_width(dp);
     const int64_t rows_per_block = 4 * 64;
     dim3 grid((unsigned)((n + rows_per_block - 1) / rows_per_block)), block(256);
     uint8_t* h = (uint8_t*)qhi;

@@ -75,6 +75,10 @@ __device__ inline double group_max(double x) {
 
 // Lanes per row for a padded width: no idle lanes for narrow rows.
 __host__ __device__ inline int group_width(int dp) { return dp <= 16 ? 16 : (dp <= 32 ? 32 : 64); }
+// Lanes per row of the row-metadata pass: at least four coordinates per
+// lane, so a 64-row wave takes 64 / width rows per step (more rows in
+// flight for mid widths).
+__host__ __device__ inline int meta_width(int dp) { return dp <= 64 ? 16 : (dp <= 128 ? 32 : 64); }
 
 // The per-row sums of one row (binary64), held by every lane of its group.
 struct RowSums {
