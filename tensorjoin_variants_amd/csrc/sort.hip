@@ -104,7 +104,9 @@ __global__ void __launch_bounds__(SB) scatter_kernel(const uint64_t* __restrict_
     __shared__ uint32_t tstart[256];      // first tile position per digit
     __shared__ uint32_t ttot[256];        // keys per digit in the tile
     __shared__ uint32_t wsum[4];          // per-wave sums of the digit-total scan
-    __shared__ uint32_t wcnt[KPT][4][256];  // per (slot, wave) digit counts, then offsets
+    // per (slot, wave) digit counts, then offsets within the tile (<= TILE:
+    // 16 bits; the smaller LDS footprint keeps four blocks per CU resident)
+    __shared__ uint16_t wcnt[KPT][4][256];
     __shared__ uint64_t sorted[TILE];
     const int tid = threadIdx.x, wave = tid >> 6;
     const int64_t n = clamp_count(count, cap);
@@ -113,7 +115,7 @@ __global__ void __launch_bounds__(SB) scatter_kernel(const uint64_t* __restrict_
     const int64_t b1 = b0 + chunk < n ? b0 + chunk : n;
     run[tid] = bases[tid] + offs[(int64_t)tid * nb + blockIdx.x];
     for (int64_t base = b0; base < b1; base += TILE) {
-        for (int e = tid; e < KPT * 4 * 256; e += SB) (&wcnt[0][0][0])[e] = 0;
+        for (int e = tid; e < KPT * 4 * 256 / 2; e += SB) ((uint32_t*)&wcnt[0][0][0])[e] = 0;
         __syncthreads();
         uint64_t key[KPT];
         uint32_t dg[KPT];
@@ -136,7 +138,7 @@ __global__ void __launch_bounds__(SB) scatter_kernel(const uint64_t* __restrict_
                 peers &= bit ? bm : ~bm;
             }
             rank[k] = lanes_below(peers);
-            if (valid[k] && rank[k] == 0) wcnt[k][wave][dg[k]] = (uint32_t)__popcll(peers);
+            if (valid[k] && rank[k] == 0) wcnt[k][wave][dg[k]] = (uint16_t)__popcll(peers);
         }
         __syncthreads();
         // per digit: exclusive offsets over (slot, wave) in tile order, then
@@ -149,7 +151,7 @@ __global__ void __launch_bounds__(SB) scatter_kernel(const uint64_t* __restrict_
 #pragma unroll
                 for (int w = 0; w < 4; ++w) {
                     const uint32_t c = wcnt[k][w][tid];
-                    wcnt[k][w][tid] = acc;
+                    wcnt[k][w][tid] = (uint16_t)acc;
                     acc += c;
                 }
             ttot[tid] = acc;
