@@ -148,8 +148,8 @@ def main() -> int:
         print(json.dumps({"copy_bytes": n, "copies": 3}), flush=True)
         return 0
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the multi-GPU sharding is "
-              "described in DESIGN.md and not run", file=sys.stderr)
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}): this build runs the join on a "
+              "single GPU; sharding over several GPUs is a design, not a run", file=sys.stderr)
         return 2
 
     if args.workload == "batched_uneven":

@@ -27,6 +27,7 @@
 // then runs again (the retry protocol of CEJ/src/run_g2a_tensorjoin.py:329-378).
 #include <hip/hip_runtime.h>
 #include <stdlib.h>
+#include <sys/mman.h>
 #include <string.h>
 #include <algorithm>
 #include <atomic>
@@ -285,6 +286,9 @@ int download(tj_host* h, void* dst, const void* src, size_t bytes, hipEvent_t af
     return TJ_OK;
 }
 
+// The host result.  Large buffers are 2 MiB aligned and marked for huge
+// pages, so first-touch faults of a fresh result cost one fault per 2 MiB
+// rather than per 4 KiB (where the kernel allows transparent huge pages).
 struct Out {
     uint8_t* buf = nullptr;
     size_t bytes = 0, cap = 0;
@@ -294,9 +298,18 @@ struct Out {
     Out& operator=(const Out&) = delete;
     int reserve(size_t need) {
         if (need <= cap) return TJ_OK;
-        size_t c = std::max(need, cap + cap / 2);
-        void* nb = realloc(buf, c);
+        const size_t c = std::max(need, cap + cap / 2);
+        void* nb = nullptr;
+        if (c >= (8u << 20)) {
+            const size_t al = (c + (2u << 20) - 1) & ~(size_t)((2u << 20) - 1);
+            if (posix_memalign(&nb, 2u << 20, al) != 0) nb = nullptr;
+            if (nb) (void)madvise(nb, al, MADV_HUGEPAGE);
+        } else {
+            nb = malloc(c);
+        }
         TJ_CHECK(nb != nullptr, TJ_ERR_INVALID, "out of host memory for %zu result bytes", c);
+        if (bytes) memcpy(nb, buf, bytes);
+        free(buf);
         buf = (uint8_t*)nb;
         cap = c;
         return TJ_OK;
@@ -534,7 +547,10 @@ int tj_host_join(tj_host* h, const float* R, const float* S, double T, int out_f
             *count = (int64_t)ids;
         } else {
             const size_t bytes = (size_t)need_out * 2 * elem;
-            if ((rc = out.reserve(out.bytes + bytes))) return rc;
+            // the first chunk sizes the whole result (chunks are alike), later
+            // ones grow it only if they are denser
+            const size_t want = c == 0 ? bytes * (size_t)h->nchunks + bytes / 8 : out.bytes + bytes;
+            if ((rc = out.reserve(std::max(want, out.bytes + bytes)))) return rc;
             if ((rc = download(h, out.buf + out.bytes, h->dpairs[c & 1], bytes, h->ev_done[c & 1], elem,
                                c * h->chunk_rows)))
                 return rc;
