@@ -279,3 +279,31 @@ def test_certified_column_splits(tj, oracle, prec):
     say("column splits", prec, desc["col_splits"], desc["cols_per_split"])
     assert desc["col_splits"] >= 2
     run_check(tj, oracle, R, S, T, prec, name="column splits")
+
+
+@pytest.mark.parametrize("d", [16, 100, 256])
+def test_fp32_filter_near_ties(tj, oracle, d):
+    """Pairs whose squared distance sits within a few binary32 ulps of T:
+    the certified FP32 filter must hand the undecidable ones to the
+    sequential FP64 form (0 misses either way), and decide the rest."""
+    rng = np.random.default_rng(300 + d)
+    n = 3000
+    R = rng.standard_normal((n, d)).astype(np.float32)
+    step = rng.standard_normal((n, d)).astype(np.float32)
+    step /= np.linalg.norm(step, axis=1, keepdims=True)
+    # partner j = i at distance 0.5 (1 +- 1e-3): most partners are inside
+    # the FP16 band but far enough from T for binary32, a few are not
+    scale = 0.5 * (1.0 + 2e-3 * (rng.random(n) - 0.5))
+    S = (R + scale[:, None].astype(np.float32) * step).astype(np.float32)
+    dd = oracle.d2_pairs(R, S, np.stack([np.arange(n), np.arange(n)], 1))
+    T = float(np.sort(dd)[n // 2])
+    # T is one partner's exact sequential d2: a tie (kept, d2 <= T), and
+    # near-ties around it
+    say("fp32 filter near ties", d, T)
+    for mode in (False, True):
+        got, st = tj.join(R, S, T=T, method="certified", precision="fp16", out_dtype=np.int64, audit=mode,
+                          return_stats=True)
+        want = oracle.join(R, S, T)
+        assert np.array_equal(got, want), (d, mode, got.shape, want.shape)
+        assert st["violations"] == 0, st
+        assert st["fp32_accept"] + st["fp32_reject"] > 0 and st["fp64_refined"] > 0, st
