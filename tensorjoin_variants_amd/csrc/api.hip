@@ -66,9 +66,9 @@ static int screen_call(int prec, int dp, bool self_join, const ScreenArgs& a, in
 int launch_screen(int prec, int dp, bool self_join, const ScreenArgs& a, int grid, hipStream_t s) {
     return screen_call(prec, dp, self_join, a, grid, s, 0);
 }
-int screen_rows_per_wg(int prec, int dp) {
+int screen_rows_per_wg(int prec, int dp, bool self_join) {
     ScreenArgs dummy{};
-    return screen_call(prec, dp, false, dummy, 0, nullptr, 2);
+    return screen_call(prec, dp, self_join, dummy, 0, nullptr, 2);
 }
 int screen_max_grid(int prec, int dp, bool self_join) {
     ScreenArgs dummy{};
@@ -205,7 +205,7 @@ int tj_plan_create(tj_plan** out, int64_t nR, int64_t nS, int d, int method, int
         }
         p->off_amb = carve(cur, 8 * (size_t)(p->amb_cap > 0 ? p->amb_cap : 1));
         p->off_aux = carve(cur, 4 * (size_t)(p->amb_cap > 0 ? p->amb_cap : 1));
-        p->bm = screen_rows_per_wg(p->prec, p->dp);
+        p->bm = screen_rows_per_wg(p->prec, p->dp, p->self_join);
         if (p->bm <= 0) { delete p; return TJ_ERR_UNSUPPORTED; }
         p->panels = (int)((nR + p->bm - 1) / p->bm);
         const int maxg = screen_max_grid(p->prec, p->dp, p->self_join);

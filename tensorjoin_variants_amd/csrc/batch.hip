@@ -77,7 +77,7 @@ int combo_of(int prec, int dp) {
 int rows_per_wg(int combo) {
     switch (combo) {
 #define X(ID, PR, DPV) \
-    case ID: return 4 * tjscreen::rb_for<PR, DPV>() * 32;
+    case ID: return 4 * tjscreen::rb_batch<PR, DPV>() * 32;
         TJ_BATCH_COMBOS(X)
 #undef X
         default: return 0;
@@ -546,6 +546,7 @@ __global__ void __launch_bounds__(256, 2) batch_screen_kernel(const ScreenBatchA
     __shared__ __attribute__((aligned(16))) uint32_t sSc[NSTAGE_MAX * 64];
     __shared__ uint64_t sCK[4][CCAP_BATCH];
     __shared__ float sCP[4][CCAP_BATCH];
+    __shared__ __attribute__((aligned(16))) float sT[4][4];  // row levels of unseeded bodies (none here)
     __shared__ int64_t sItem;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
@@ -577,8 +578,8 @@ __global__ void __launch_bounds__(256, 2) batch_screen_kernel(const ScreenBatchA
         const int64_t ce = rg ? J.grow + (int64_t)uni(rg[2 + 2 * r]) : J.grow + (int64_t)J.n;
 #define X(ID, PR, DPV)                                                                                          \
     case ID:                                                                                                    \
-        screen_panel<PR, DPV, rb_for<PR, DPV>(), true, CCAP_BATCH>(io, A.co, r0, cb, ce, sB, sNp, sW, sSc, sCK[wave], \
-                                                       sCP[wave], nC, nAcc, A.fast_accept);                     \
+        screen_panel<PR, DPV, rb_batch<PR, DPV>(), true, CCAP_BATCH>(io, A.co, r0, cb, ce, sB, sNp, sW, sSc, sCK[wave], \
+                                                       sCP[wave], sT[wave], nC, nAcc, A.fast_accept);                     \
         break;
         if constexpr (FAM == 0) {
             switch (uni(G.combo)) {

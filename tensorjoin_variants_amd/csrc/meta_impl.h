@@ -183,6 +183,10 @@ __device__ inline RowMeta rowmeta_terms(const RowSums& S, int dp, const JoinPara
     // Fast-path terms.  The screen seeds the MFMA accumulator of row i with
     // t_i, so one chain yields s_ij = t_i + p_ij (p the dot of the quantized
     // rows) with |s - (t_i + sum a b)| <= gamma (|t_i| + sum |a b|) + alpha.
+    // (Its unseeded bodies compare fl(t_i + p) instead, p accumulated from
+    // zero: by the monotonicity of rounding, fl(t_i + p) < v_j implies
+    // t_i + p < v_j, which the bounds below reject without the gamma |t_i|
+    // slack; the accept side is covered by the screen's margin on u_j.)
     // With Z = H + L the residual part of the exact dot is bounded by
     //   L_i L_j + Z_i E_j + E_i Z_j + E_i E_j,  sum |a b| <= H_i H_j [+ H_i L_j + L_i H_j],
     // and 2ab <= rho a^2 + b^2/rho splits everything except gamma |t_i| into

@@ -10,7 +10,7 @@
 //
 // MI355X design:
 //  * One persistent work-queue launch.  A work item is a panel of
-//    BM = 4 waves x RB x 32 stationary rows (R) against a range of streamed
+//    BM = 4 waves x RB x 32 stationary rows (R; RB = 1, 2 or 4) against a range of streamed
 //    columns (S).  Items are taken from an atomic cursor, largest first
 //    (triangular panels of a self-join shrink with the panel index); for an
 //    R x S join consecutive items share the same column range, so the
@@ -22,10 +22,13 @@
 //  * v_mfma_f32_32x32x16_{f16,bf16} / v_mfma_scale_f32_32x32x64_f8f6f4 with
 //    FP32 accumulation; the k order inside a fragment is any fixed
 //    permutation shared by A and B (a dot product does not depend on it).
-//  * Fast path per pair: w = fma(-2, p, nprime_j) and one compare against the
-//    row threshold thr_i (both precomputed by the row-metadata pass, see
-//    meta.hip).  w > thr_i is a rigorous reject.  The rest (accepted pairs
-//    and the ambiguous band) take the exact per-pair interval in binary64.
+//  * Fast path per pair: s = t_i + p against the column level v_j (row and
+//    column terms from the row-metadata pass, meta_impl.h); s < v_j is a
+//    rigorous reject, s >= a_i + w_j a rigorous accept.  Bodies of up to two
+//    32-row blocks per wave seed the MFMA accumulator with t_i; four-block
+//    bodies (no VGPRs left for seeds) accumulate from zero and add t_i in
+//    binary32 afterwards.  The band between takes the exact per-pair
+//    interval in binary64.
 //  * Compaction: wave ballot + mbcnt prefix into a per-wave LDS staging
 //    buffer, flushed to HBM with one atomic per 256 pairs.
 #pragma once
@@ -94,10 +97,12 @@ __device__ inline int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 // Exact per-pair interval in binary64 (join units) and its class:
 // 1 = certified inside, 0 = certified outside, 2 = ambiguous.
-// s is the screen's MFMA output t_i + p (t_i the accumulator seed of row i):
-//   p = s - t_i,  |p - x_i.x_j| <= B_ij,
+// s is the screen's t_i + p, either the MFMA output of a chain seeded with
+// t_i or fl(t_i + p) after an unseeded chain; both satisfy
+//   |s - t_i - x_i.x_j| <= B_ij,
 //   B_ij = gamma (|t_i| + H_i H_j [+ H_i L_j + L_i H_j]) + L_i L_j + Z_i E_j + E_i Z_j + E_i E_j + alpha
-//   d2 in [nlo_i + nlo_j - 2p - 2B, nhi_i + nhi_j - 2p + 2B]
+//          + 2^-24 |s| + 2^-149   (the rounding of the binary32 sum, if any)
+//   d2 in [nlo_i + nlo_j - 2p - 2B, nhi_i + nhi_j - 2p + 2B],  p = s - t_i,
 // widened by the binary64 rounding of this evaluation; the oracle's own
 // gap g is charged in Tj_acc / Tj_rej.
 __device__ inline int classify(float s, const RowMeta& mi, const RowMeta& mj, const JoinParams& P,
@@ -113,6 +118,7 @@ __device__ inline int classify(float s, const RowMeta& mi, const RowMeta& mj, co
         const double Zi = Hi + Li, Zj = Hj + Lj;
         B = P.gamma_acc * (fabs(ti) + Hi * Hj + Hi * Lj + Li * Hj) + Li * Lj + Zi * Ej + Ei * Zj + Ei * Ej + P.alpha;
     }
+    B += fabs((double)s) * 0x1p-24 + 0x1p-149;
     B *= 1.0 + 0x1p-44;  // a dozen binary64 roundings, each <= 2^-53 relative
     const double slo = (double)mi.m0.x + (double)mj.m0.x;  // exact
     const double shi = (double)mi.m0.y + (double)mj.m0.y;  // exact
@@ -303,15 +309,24 @@ constexpr int ring_bytes() { return Geo<PREC, DP>::NSTAGE * Geo<PREC, DP>::BLKB;
 // One work item: the panel of BM = 4 x RB x 32 stationary rows starting at r0
 // against columns [cb, ce), all 4 waves of the workgroup.  sB/sNp/sW/sSc are
 // the workgroup's S ring (NSTAGE slots: quantized columns, reject levels v,
-// accept terms w, FP8 scales), sCK/sCP this wave's staging buffers.  With
-// fast_accept == 0 every survivor goes to the band queue (audit mode).
+// accept terms w, FP8 scales), sCK/sCP this wave's staging buffers, sT this
+// wave's RB x 32 row levels t (16-byte aligned; used by unseeded bodies).
+// With fast_accept == 0 every survivor goes to the band queue (audit mode).
+//
+// Unseeded bodies (RB = 4): p is the MFMA dot accumulated from zero and the
+// pair's level is s = fl(p + t_i).  Rounding to nearest is monotone and v_j
+// is a binary32 number, so p + t_i >= v_j implies s >= v_j: s < v_j rejects
+// as rigorously as the seeded chain's output does.  Per 32 x 32 tile a lane
+// first tests fl(max_r p_r + max_r t_r) >= v_j (p_r + t_r is at most
+// max p + max t), and only tiles that pass form the 16 sums.
 template <int PREC, int DP, int RB, bool SELF, int CC = CCAP>
 __device__ __forceinline__ void screen_panel(const PanelIO& io, const CandOut& co, int64_t r0, int64_t cb, int64_t ce,
                                              uint8_t* sB, float* sNp, float* sW, uint32_t* sSc, uint64_t* sCK,
-                                             float* sCP, int& nC, uint32_t& nAcc, int fast_accept) {
+                                             float* sCP, float* sT, int& nC, uint32_t& nAcc, int fast_accept) {
     using C = Cfg<PREC>;
     using Gm = Geo<PREC, DP>;
     constexpr int NSTAGE = Gm::NSTAGE;
+    constexpr bool SEEDED = RB <= 2;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int hh = lane >> 5, c32 = lane & 31;
@@ -319,10 +334,18 @@ __device__ __forceinline__ void screen_panel(const PanelIO& io, const CandOut& c
     const int64_t rw0 = r0 + (int64_t)uni(wave) * RB * 32;  // uniform: tile tests stay scalar
     const int nblk = (int)((ce - cb + 31) / 32);
 
-    // ---- stationary fragments, thresholds and scales (VGPR resident)
+    // ---- stationary fragments, row levels and scales (VGPR resident)
     Frag<C::CH> fa[C::NT][RB][Gm::KS];
-    f32x16 seed[RB];
+    f32x16 seed[SEEDED ? RB : 1];
+    float tmax[RB];  // unseeded: largest t over the lane's 16 rows of each block (-inf: none valid)
     int sa[C::NT][RB];
+    if constexpr (!SEEDED) {
+        // t of the wave's rows; -inf past na, so such rows never survive
+        for (int k = lane; k < RB * 32; k += 64) {
+            const int64_t row = rw0 + k;
+            sT[k] = row < na ? io.ma[row].m1.y : -INFINITY;
+        }
+    }
 #pragma unroll
     for (int rb = 0; rb < RB; ++rb) {
         const int64_t row = rw0 + rb * 32 + c32;
@@ -339,10 +362,21 @@ __device__ __forceinline__ void screen_panel(const PanelIO& io, const CandOut& c
 #pragma unroll
             for (int a = 0; a < C::NT; ++a) sa[a][rb] = 127;
         }
+        if constexpr (SEEDED) {
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            const int64_t ri = rw0 + rb * 32 + 8 * (r >> 2) + 4 * hh + (r & 3);
-            seed[rb][r] = ri < na ? io.ma[ri].m1.y : -INFINITY;
+            for (int r = 0; r < 16; ++r) {
+                const int64_t ri = rw0 + rb * 32 + 8 * (r >> 2) + 4 * hh + (r & 3);
+                seed[rb][r] = ri < na ? io.ma[ri].m1.y : -INFINITY;
+            }
+        } else {
+            // register r of a lane holds row 8 (r / 4) + 4 hh + r % 4 of the block
+            float tm = -INFINITY;
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                const float4 tv = *(const float4*)(sT + rb * 32 + 8 * g + 4 * hh);
+                tm = fmaxf(tm, fmaxf(fmaxf(tv.x, tv.y), fmaxf(tv.z, tv.w)));
+            }
+            tmax[rb] = tm;
         }
     }
     // largest accept term a_i over this wave's valid rows: s >= w_j + a_w
@@ -374,8 +408,12 @@ __device__ __forceinline__ void screen_panel(const PanelIO& io, const CandOut& c
                     __asm__ volatile("" ::"v"(q1.x), "v"(q1.y), "v"(q1.z), "v"(q1.w));
                 }
             }
+        if constexpr (SEEDED) {
 #pragma unroll
-        for (int r = 0; r < 16; ++r) __asm__ volatile("" ::"v"(seed[rb][r]));
+            for (int r = 0; r < 16; ++r) __asm__ volatile("" ::"v"(seed[rb][r]));
+        } else {
+            __asm__ volatile("" ::"v"(tmax[rb]));
+        }
     }
     __asm__ volatile("" ::"v"(a_w));
 
@@ -498,7 +536,7 @@ __device__ __forceinline__ void screen_panel(const PanelIO& io, const CandOut& c
             if constexpr (C::NT == 1) {
 #pragma unroll
                 for (int rb = 0; rb < RB; ++rb) {
-                    if (t == 0) acc[rb] = mma_c<PREC>(fa[0][rb][t], fb, seed[rb], sa[0][rb], sbh);
+                    if (t == 0) acc[rb] = mma_c<PREC>(fa[0][rb][t], fb, SEEDED ? seed[rb] : f32x16{}, sa[0][rb], sbh);
                     else mma<PREC>(acc[rb], fa[0][rb][t], fb, sa[0][rb], sbh);
                 }
             } else {
@@ -512,7 +550,7 @@ __device__ __forceinline__ void screen_panel(const PanelIO& io, const CandOut& c
                 }
 #pragma unroll
                 for (int rb = 0; rb < RB; ++rb) {
-                    if (t == 0) acc[rb] = mma_c<PREC>(fa[0][rb][t], fb, seed[rb], sa[0][rb], sbh);
+                    if (t == 0) acc[rb] = mma_c<PREC>(fa[0][rb][t], fb, SEEDED ? seed[rb] : f32x16{}, sa[0][rb], sbh);
                     else mma<PREC>(acc[rb], fa[0][rb][t], fb, sa[0][rb], sbh);
                     mma<PREC>(acc[rb], fa[0][rb][t], fl, sa[0][rb], sbl);
                     mma<PREC>(acc[rb], fa[1][rb][t], fb, sa[1][rb], sbh);
@@ -520,10 +558,12 @@ __device__ __forceinline__ void screen_panel(const PanelIO& io, const CandOut& c
             }
         }
         // ---- fast reject: s = t_i + p_ij < v_j rejects rigorously; one
-        // max over the 16 rows of a lane, one compare, one ballot per block
+        // max over the 16 rows of a lane (unseeded: plus the largest t of
+        // those rows), one compare, one ballot per block
 #pragma unroll
         for (int rb = 0; rb < RB; ++rb) {
-            const float mx = max16(acc[rb]);
+            float mx = max16(acc[rb]);
+            if constexpr (!SEEDED) mx += tmax[rb];
             if (__ballot(mx >= nj) == 0) continue;
             // ---- survivors: register r of this lane is row
             // ibase + off(r), off(r) = 8 (r / 4) + r % 4, column jcol.  Each
@@ -534,7 +574,17 @@ __device__ __forceinline__ void screen_panel(const PanelIO& io, const CandOut& c
             const uint64_t kb = ((uint64_t)(ibase + io.koff_a) << io.jbits) | (uint64_t)(jcol + io.koff_b);
             // (columns past nb have nj = +inf: no survivor reads uj there)
             const float u = wj + a_w;
-            const float uj = u + (fabsf(u) * 0x1p-22f + 0x1p-126f);  // >= w_j + a_w despite the rounding of the sum
+            // >= w_j + a_w despite the rounding of the sum (and, unseeded, s
+            // >= uj still gives t_i + p >= w_j + a_w despite that of s)
+            const float uj = u + (fabsf(u) * 0x1p-22f + 0x1p-126f);
+            float tr[16];  // unseeded: t of the lane's 16 rows
+            if constexpr (!SEEDED) {
+#pragma unroll
+                for (int g = 0; g < 4; ++g) {
+                    const float4 tv = *(const float4*)(sT + rb * 32 + 8 * g + 4 * hh);
+                    tr[4 * g] = tv.x; tr[4 * g + 1] = tv.y; tr[4 * g + 2] = tv.z; tr[4 * g + 3] = tv.w;
+                }
+            }
             // Stage register r's survivors; DIAG adds the self-join's i < j:
             // off(r) < e <=> ibase + off(r) < jcol.  Ballots of single
             // compares are the compare masks themselves.
@@ -551,13 +601,13 @@ __device__ __forceinline__ void screen_panel(const PanelIO& io, const CandOut& c
                 uint64_t ms[16];
 #pragma unroll
                 for (int r = 0; r < 16; ++r) {
-                    ms[r] = __builtin_amdgcn_ballot_w64(acc[rb][r] >= nj);
+                    ms[r] = __builtin_amdgcn_ballot_w64((SEEDED ? acc[rb][r] : acc[rb][r] + tr[r]) >= nj);
                     if constexpr (DIAG) ms[r] &= __builtin_amdgcn_ballot_w64(8 * (r >> 2) + (r & 3) < e);
                 }
 #pragma unroll
                 for (int r = 0; r < 16; ++r) {
                     const int off = 8 * (r >> 2) + (r & 3);
-                    const float sv = acc[rb][r];
+                    const float sv = SEEDED ? acc[rb][r] : acc[rb][r] + tr[r];
                     const uint64_t m = ms[r];
                     if (m == 0) continue;
                     // the lane's own survivor test: the compare itself off the
@@ -588,13 +638,14 @@ __device__ __forceinline__ void screen_panel(const PanelIO& io, const CandOut& c
 }
 
 // Workgroups per CU the compiler should plan registers for: two; one when
-// the stationary fragments alone take 128 VGPRs (up to 512 each); three for
-// single-tile waves with at most 64 fragment VGPRs (up to 168 each).
+// the stationary fragments of a one- or two-block wave take 128 VGPRs (up to
+// 512 each); three for single-tile waves with at most 64 fragment VGPRs (up
+// to 168 each).  Four-block waves are unseeded and stay at two.
 template <int PREC, int DP, int RB>
 constexpr int min_blocks() {
     using C = Cfg<PREC>;
     constexpr int frag = C::NT * RB * (DP / C::KSTEP) * (C::CH / 4);
-    return frag >= 128 ? 1 : (RB == 1 && frag <= 64 ? 3 : 2);
+    return RB == 4 ? 2 : (frag >= 128 ? 1 : (RB == 1 && frag <= 64 ? 3 : 2));
 }
 
 template <int PREC, int DP, int RB, bool SELF>
@@ -607,6 +658,7 @@ __global__ void __launch_bounds__(256, (min_blocks<PREC, DP, RB>())) screen_kern
     __shared__ __attribute__((aligned(16))) uint32_t sSc[Gm::NSTAGE * 64];
     __shared__ uint64_t sCK[4][CCAP];
     __shared__ float sCP[4][CCAP];
+    __shared__ __attribute__((aligned(16))) float sT[4][RB <= 2 ? 4 : RB * 32];
     __shared__ int sItem;
 
     const int tid = threadIdx.x;
@@ -635,31 +687,40 @@ __global__ void __launch_bounds__(256, (min_blocks<PREC, DP, RB>())) screen_kern
         if (SELF) { cb = r0; ce = A.nb; }
         else { cb = (int64_t)split * A.cols_per_split; ce = cb + A.cols_per_split; if (ce > A.nb) ce = A.nb; }
         if (cb >= ce) continue;
-        screen_panel<PREC, DP, RB, SELF>(io, co, r0, cb, ce, sB, sNp, sW, sSc, sCK[wave], sCP[wave], nC, nAcc,
-                                         !A.audit);
+        screen_panel<PREC, DP, RB, SELF>(io, co, r0, cb, ce, sB, sNp, sW, sSc, sCK[wave], sCP[wave], sT[wave], nC,
+                                         nAcc, !A.audit);
     }
     cand_finish(co, sCK[wave], sCP[wave], nC, nAcc, lane);
 }
 
-// Rows per wave: two 32-row blocks while their stationary fragments stay
-// within 64 VGPRs (256 bytes per lane) together, else one.
+// Rows per wave for batched joins: two 32-row blocks while their stationary
+// fragments stay within 64 VGPRs (256 bytes per lane) together, else one.
 template <int PREC, int DP>
-constexpr int rb_for() {
+constexpr int rb_batch() {
     using C = Cfg<PREC>;
     constexpr int bytes_per_block = C::NT * DP * C::ES / 2;  // per lane, one 32-row block (64 lanes x 2 halves)
     return bytes_per_block * 2 <= 256 ? 2 : 1;
 }
+// Single joins: R x S with single-term FP16 / BF16 codes of width 64 or 128
+// take four blocks per wave (each LDS fragment feeds four MFMAs, the block
+// overheads are spread over twice the MFMA work; unseeded, as the 64 seed
+// registers would not fit); everything else as batched joins.
+template <int PREC, int DP, bool SELF>
+constexpr int rb_for() {
+    using C = Cfg<PREC>;
+    return (!SELF && C::NT == 1 && !C::FP8 && DP >= 64 && DP <= 128) ? 4 : rb_batch<PREC, DP>();
+}
 
 template <int PREC, int DP, bool SELF>
 int launch_one(const tj::ScreenArgs& a, int grid, hipStream_t s) {
-    constexpr int RB = rb_for<PREC, DP>();
+    constexpr int RB = rb_for<PREC, DP, SELF>();
     hipLaunchKernelGGL((screen_kernel<PREC, DP, RB, SELF>), dim3(grid), dim3(256), 0, s, a);
     return hipGetLastError() == hipSuccess ? TJ_OK : TJ_ERR_HIP;
 }
 
 template <int PREC, int DP, bool SELF>
 int occupancy_one() {
-    constexpr int RB = rb_for<PREC, DP>();
+    constexpr int RB = rb_for<PREC, DP, SELF>();
     int blocks = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, screen_kernel<PREC, DP, RB, SELF>, 256, 0) != hipSuccess) return 1;
     return blocks > 0 ? blocks : 1;
@@ -681,5 +742,6 @@ int occupancy_one() {
     case DPV:                                                                                                 \
         if (query == 1) return self_join ? tjscreen::occupancy_one<PREC, DPV, true>()                        \
                                          : tjscreen::occupancy_one<PREC, DPV, false>();                      \
-        if (query == 2) return 4 * tjscreen::rb_for<PREC, DPV>() * 32;                                        \
+        if (query == 2) return 4 * (self_join ? tjscreen::rb_for<PREC, DPV, true>()                          \
+                                              : tjscreen::rb_for<PREC, DPV, false>()) * 32;                  \
         return self_join ? tjscreen::launch_one<PREC, DPV, true>(a, grid, s) : tjscreen::launch_one<PREC, DPV, false>(a, grid, s);

@@ -231,7 +231,7 @@ struct ScreenArgs {
     int audit;
 };
 int launch_screen(int prec, int dp, bool self_join, const ScreenArgs& a, int grid, hipStream_t s);
-int screen_rows_per_wg(int prec, int dp);
+int screen_rows_per_wg(int prec, int dp, bool self_join);
 int screen_max_grid(int prec, int dp, bool self_join);
 
 int launch_scale_scan(const float* x, int64_t n, int d, uint32_t* maxabs, uint64_t* stats, hipStream_t s);
