@@ -447,7 +447,14 @@ __device__ __forceinline__ void screen_panel(const PanelIO& io, const CandOut& c
     const uint64_t mbb = uni64((uint64_t)(uintptr_t)io.mb);
     const uint32_t ldsB = uni((int)lds_addr(sB)), ldsNp = uni((int)lds_addr(sNp)), ldsW = uni((int)lds_addr(sW)),
                    ldsSc = uni((int)lds_addr(sSc));
-    const int mcol = 8 * wv + (lane & 7);  // this lane's metadata column within a block (lanes 0-7)
+    // this lane's metadata column within a block (lanes 0-7), formed anew
+    // per block: kept live across the loop it is spilled when registers are
+    // tight, and the reload's vmcnt(0) would drain the prefetch ring
+    auto meta_col = [&]() {
+        int l = lane;
+        __asm__ volatile("" : "+v"(l));
+        return 8 * wv + (l & 7);
+    };
     auto issue = [&](int blk, int slot) {
         const int64_t c0 = cb + (int64_t)blk * 32;
         const uint32_t dst = ldsB + (uint32_t)(slot * Gm::BLKB);
@@ -465,7 +472,7 @@ __device__ __forceinline__ void screen_panel(const PanelIO& io, const CandOut& c
                 }
             }
             if (lane < 8) {
-                const uint32_t off = (uint32_t)mcol * (uint32_t)sizeof(RowMeta);
+                const uint32_t off = (uint32_t)meta_col() * (uint32_t)sizeof(RowMeta);
                 dma4s(mbase, off + (uint32_t)offsetof(RowMeta, m1) + 8, ldsNp + mo);
                 dma4s(mbase, off + (uint32_t)offsetof(RowMeta, m2), ldsW + mo);
                 if constexpr (C::FP8) dma4s(mbase, off + (uint32_t)offsetof(RowMeta, m1) + 12, ldsSc + mo);
@@ -486,6 +493,7 @@ __device__ __forceinline__ void screen_panel(const PanelIO& io, const CandOut& c
                 }
             }
             if (lane < 8) {
+                const int mcol = meta_col();
                 const uint32_t off = (uint32_t)(mcol > last ? last : mcol) * (uint32_t)sizeof(RowMeta);
                 dma4s(mbase, off + (uint32_t)offsetof(RowMeta, m1) + 8, ldsNp + mo);
                 dma4s(mbase, off + (uint32_t)offsetof(RowMeta, m2), ldsW + mo);
