@@ -9,6 +9,11 @@
 //    atomics), cell keys, my radix sort by cell, S gathered into cell order
 //    as 16-byte rows, and an open-addressing hash of the non-empty cells
 //    (memory bounded by 2|S| slots however sparse the grid is);
+//  * a bitmap of the non-empty cells (when the grid has at most 64 |S|
+//    cells) answers "is this neighbour cell empty" for the three x-adjacent
+//    cells of a row with one or two 8-byte reads: most neighbour cells of a
+//    sparse region are empty, and each empty one would otherwise cost a
+//    hash probe sequence of random 64-byte lines;
 //  * queries run in cell order (R is sorted by cell too), so the 64 lanes of
 //    a wave visit the same neighbour cells and the same S rows;
 //  * output: a count pass, an exclusive scan, a write pass of pair keys at
@@ -30,8 +35,9 @@ struct GridParams {
     double inv_side;  // 1 / side
     double side;
     uint64_t ncell[3];
-    uint64_t mul[3];  // linear id = sum c_k * mul[k]
-    int keybits, idxbits, d, pad;
+    uint64_t mul[3];  // linear id = sum c_k * mul[k] (mul[0] = 1: x-adjacent cells are adjacent ids)
+    uint64_t cells;   // product of ncell
+    int keybits, idxbits, d, use_bm;  // use_bm: cells fit the occupancy bitmap
 };
 
 __device__ inline uint32_t f2ord(float f) {
@@ -69,7 +75,8 @@ __global__ void init_bounds_kernel(uint32_t* bounds) {
     if (threadIdx.x < 3) { bounds[threadIdx.x] = 0xFFFFFFFFu; bounds[3 + threadIdx.x] = 0; }
 }
 
-__global__ void grid_params_kernel(const uint32_t* bounds, double T, int d, int idxbits, GridParams* gp) {
+__global__ void grid_params_kernel(const uint32_t* bounds, double T, int d, int idxbits, uint64_t bm_bits,
+                                   GridParams* gp) {
     if (threadIdx.x != 0) return;
     GridParams g;
     double side = sqrt(T) * (1.0 + 0x1p-10);
@@ -111,8 +118,17 @@ __global__ void grid_params_kernel(const uint32_t* bounds, double T, int d, int 
     g.keybits = bits;
     g.idxbits = idxbits;
     g.d = d;
-    g.pad = 0;
+    g.cells = mul;
+    g.use_bm = mul <= bm_bits ? 1 : 0;
     *gp = g;
+}
+
+// Clear the words of the occupancy bitmap the grid uses.
+__global__ void __launch_bounds__(256) clear_bitmap_kernel(const GridParams* __restrict__ gp, uint64_t* __restrict__ bm) {
+    const GridParams g = *gp;
+    const int64_t words = g.use_bm ? (int64_t)((g.cells + 63) / 64) : 0;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < words; t += (int64_t)gridDim.x * blockDim.x)
+        bm[t] = 0;
 }
 
 __device__ inline void cell_of(const float* x, const GridParams& g, int64_t c[3]) {
@@ -146,7 +162,8 @@ constexpr uint64_t EMPTY = ~0ull;
 // of each non-empty cell inserted into the hash (slot: key, start, end).
 __global__ void __launch_bounds__(256) gather_kernel(const float* __restrict__ S, int64_t n, const GridParams* __restrict__ gp,
                                                      const uint64_t* __restrict__ keys, float4* __restrict__ srow,
-                                                     uint64_t* __restrict__ hkey, uint2* __restrict__ hval, uint64_t hmask) {
+                                                     uint64_t* __restrict__ hkey, uint2* __restrict__ hval, uint64_t hmask,
+                                                     uint64_t* __restrict__ bm) {
     const GridParams g = *gp;
     const uint64_t imask = (1ull << g.idxbits) - 1;
     for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
@@ -169,6 +186,7 @@ __global__ void __launch_bounds__(256) gather_kernel(const float* __restrict__ S
                 h = (h + 1) & hmask;
             }
             hval[h] = make_uint2((uint32_t)p, (uint32_t)e);
+            if (g.use_bm) atomicOr((unsigned long long*)&bm[cell >> 6], 1ull << (cell & 63));
         }
     }
 }
@@ -197,7 +215,8 @@ template <int MODE>
 __global__ void __launch_bounds__(256) probe_kernel(const float* __restrict__ R, int64_t nR, const GridParams* __restrict__ gp,
                                                     const uint64_t* __restrict__ rkeys, const float4* __restrict__ srow,
                                                     const uint64_t* __restrict__ hkey, const uint2* __restrict__ hval,
-                                                    uint64_t hmask, double T, int self_join, uint32_t* __restrict__ cnt,
+                                                    uint64_t hmask, const uint64_t* __restrict__ bm, double T,
+                                                    int self_join, uint32_t* __restrict__ cnt,
                                                     const uint64_t* __restrict__ off, uint64_t* __restrict__ out_keys,
                                                     int jbits, int64_t out_cap, uint64_t* __restrict__ stats) {
     const GridParams g = *gp;
@@ -214,10 +233,22 @@ __global__ void __launch_bounds__(256) probe_kernel(const float* __restrict__ R,
         if (MODE == 1) base = off[i];
         const int r1 = g.d > 1 ? 1 : 0, r2 = g.d > 2 ? 1 : 0;
         for (int dz = -r2; dz <= r2; ++dz)
-            for (int dy = -r1; dy <= r1; ++dy)
+            for (int dy = -r1; dy <= r1; ++dy) {
+                // ids of the x-adjacent cells: mid - 1, mid, mid + 1 (cell_of
+                // keeps c in [1, ncell - 2], so all three exist)
+                const uint64_t mid = (uint64_t)c[0] + (uint64_t)(c[1] + dy) * g.mul[1] + (uint64_t)(c[2] + dz) * g.mul[2];
+                uint32_t occ = 7;  // bit dx + 1: that cell may hold S points
+                if (g.use_bm) {
+                    const uint64_t b0 = mid - 1, b2 = mid + 1;
+                    const uint64_t w0 = bm[b0 >> 6];
+                    const uint64_t w2 = (b2 >> 6) == (b0 >> 6) ? w0 : bm[b2 >> 6];
+                    const uint32_t sh = (uint32_t)(b0 & 63);
+                    occ = (uint32_t)((w0 >> sh) & 7);
+                    if (sh > 61) occ |= (uint32_t)(w2 << (64 - sh)) & 7;
+                }
                 for (int dx = -1; dx <= 1; ++dx) {
-                    const uint64_t cell = (uint64_t)(c[0] + dx) * g.mul[0] + (uint64_t)(c[1] + dy) * g.mul[1] +
-                                          (uint64_t)(c[2] + dz) * g.mul[2];
+                    if (!((occ >> (dx + 1)) & 1)) continue;
+                    const uint64_t cell = mid + (uint64_t)(int64_t)dx;
                     const uint2 run = lookup(hkey, hval, hmask, cell);
                     for (uint32_t p = run.x; p < run.y; ++p) {
                         const float4 y = srow[p];
@@ -230,6 +261,7 @@ __global__ void __launch_bounds__(256) probe_kernel(const float* __restrict__ R,
                         }
                     }
                 }
+            }
         if (MODE == 0) cnt[i] = n;
     }
     if (MODE == 0) {
@@ -330,6 +362,10 @@ GridLayout grid_layout(int64_t nR, int64_t nS, int64_t out_cap) {
     L.off = carve(8 * (size_t)(nR > 0 ? nR : 1));
     L.bsum = carve(8 * (size_t)((nR + SCAN_B - 1) / SCAN_B + 1));
     L.outj = carve(8 * (size_t)(out_cap > 0 ? out_cap : 1));  // pair keys
+    // occupancy bitmap: 64 bits per S point (at least 2^20); sparser grids
+    // go without it
+    L.bm_bits = std::max<uint64_t>(1ull << 20, 64ull * (uint64_t)(nS > 0 ? nS : 1));
+    L.bitmap = carve(L.bm_bits / 8);
     L.total = (cur + 255) / 256 * 256;
     return L;
 }
@@ -367,7 +403,9 @@ int launch_grid_join(const float* R, int64_t nR, const float* S, int64_t nS, int
     hipLaunchKernelGGL(init_bounds_kernel, dim3(1), dim3(64), 0, s, bounds);
     hipLaunchKernelGGL(bounds_kernel, dim3(blocks_for(nR, gb)), dim3(256), 0, s, R, nR, d, bounds);
     if (!self_join) hipLaunchKernelGGL(bounds_kernel, dim3(blocks_for(nS, gb)), dim3(256), 0, s, S, nS, d, bounds);
-    hipLaunchKernelGGL(grid_params_kernel, dim3(1), dim3(64), 0, s, bounds, T, d, L.idxbits, gp);
+    hipLaunchKernelGGL(grid_params_kernel, dim3(1), dim3(64), 0, s, bounds, T, d, L.idxbits, L.bm_bits, gp);
+    uint64_t* bm = (uint64_t*)(ws + L.bitmap);
+    hipLaunchKernelGGL(clear_bitmap_kernel, dim3(blocks_for((int64_t)(L.bm_bits / 64), gb)), dim3(256), 0, s, gp, bm);
     if ((rc = launch_fill_u64(cR, 1, (uint64_t)nR, s))) return rc;
     if ((rc = launch_fill_u64(cS, 1, (uint64_t)nS, s))) return rc;
     if ((rc = launch_fill_u64(hkey, (int64_t)L.hslots, EMPTY, s))) return rc;
@@ -377,17 +415,17 @@ int launch_grid_join(const float* R, int64_t nR, const float* S, int64_t nS, int
     hipLaunchKernelGGL(keys_kernel, dim3(blocks_for(nS, gb)), dim3(256), 0, s, S, nS, gp, skeys);
     if ((rc = launch_sort(ws + L.sort_tmp, skeys, cS, nS, 63, s))) return rc;
     hipLaunchKernelGGL(gather_kernel, dim3(blocks_for(nS, gb)), dim3(256), 0, s, S, nS, gp, skeys, srow, hkey, hval,
-                       (uint64_t)(L.hslots - 1));
+                       (uint64_t)(L.hslots - 1), bm);
     // count -> scan -> write -> per-row sort
     const unsigned pb = blocks_for(nR, (int64_t)num_cus() * 16);
     hipLaunchKernelGGL(probe_kernel<0>, dim3(pb), dim3(256), 0, s, R, nR, gp, rkeys, srow, hkey, hval,
-                       (uint64_t)(L.hslots - 1), T, (int)self_join, cnt, off, okeys, jbits, out_cap, stats);
+                       (uint64_t)(L.hslots - 1), bm, T, (int)self_join, cnt, off, okeys, jbits, out_cap, stats);
     const int64_t nb = (nR + SCAN_B - 1) / SCAN_B;
     hipLaunchKernelGGL(scan_block_sums, dim3((unsigned)nb), dim3(256), 0, s, cnt, nR, bsum);
     hipLaunchKernelGGL(scan_sums, dim3(1), dim3(1024), 0, s, bsum, nb, stats);
     hipLaunchKernelGGL(scan_apply, dim3((unsigned)nb), dim3(256), 0, s, cnt, nR, bsum, off);
     hipLaunchKernelGGL(probe_kernel<1>, dim3(pb), dim3(256), 0, s, R, nR, gp, rkeys, srow, hkey, hval,
-                       (uint64_t)(L.hslots - 1), T, (int)self_join, cnt, off, okeys, jbits, out_cap, stats);
+                       (uint64_t)(L.hslots - 1), bm, T, (int)self_join, cnt, off, okeys, jbits, out_cap, stats);
     TJ_HIP(hipGetLastError());
     // rows are already in order (the scan runs over i); one radix sort of the
     // keys orders each row's j's, then the keys unpack to pairs

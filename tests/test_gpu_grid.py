@@ -58,6 +58,12 @@ def test_grid_boundaries(tj, oracle):
     W = rng.random((2000, 3)).astype(np.float32)
     W[:, 1] = 0.5
     check(tj, oracle, W, None, 0.02 ** 2, "flat dim")
+    # a grid far sparser than 64 cells per point (no occupancy bitmap):
+    # 8e9 cells for 3,000 points, pairs only between near-copies
+    V = (rng.random((1500, 3)) * 1000).astype(np.float32)
+    V = np.concatenate([V, V + rng.uniform(-0.2, 0.2, V.shape).astype(np.float32)]).astype(np.float32)
+    check(tj, oracle, V, None, 0.25, "sparse grid self")
+    check(tj, oracle, V[:1500], V[1500:], 0.25, "sparse grid rxs")
 
 
 def test_grid_edge_sizes(tj, oracle):
