@@ -58,15 +58,28 @@ __global__ void __launch_bounds__(256) bounds_kernel(const float* __restrict__ x
             mn[k] = o < mn[k] ? o : mn[k];
             mx[k] = o > mx[k] ? o : mx[k];
         }
+    // wave, then block reduction; one atomic per block and bound (per-wave
+    // atomics on six addresses serialised into most of the kernel's time)
+    __shared__ uint32_t smn[3][4], smx[3][4];
+    const int wave = threadIdx.x >> 6;
     for (int k = 0; k < d; ++k) {
         for (int s = 32; s > 0; s >>= 1) {
             const uint32_t a = __shfl_xor(mn[k], s), b = __shfl_xor(mx[k], s);
             mn[k] = a < mn[k] ? a : mn[k];
             mx[k] = b > mx[k] ? b : mx[k];
         }
-        if ((threadIdx.x & 63) == 0) {
-            atomicMin(&bounds[k], mn[k]);
-            atomicMax(&bounds[3 + k], mx[k]);
+        if ((threadIdx.x & 63) == 0) { smn[k][wave] = mn[k]; smx[k][wave] = mx[k]; }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < d; ++k) {
+            uint32_t a = smn[k][0], b = smx[k][0];
+            for (int w = 1; w < (int)(blockDim.x >> 6); ++w) {
+                a = smn[k][w] < a ? smn[k][w] : a;
+                b = smx[k][w] > b ? smx[k][w] : b;
+            }
+            atomicMin(&bounds[k], a);
+            atomicMax(&bounds[3 + k], b);
         }
     }
 }
@@ -401,8 +414,9 @@ int launch_grid_join(const float* R, int64_t nR, const float* S, int64_t nS, int
     int rc;
     const unsigned gb = (unsigned)(num_cus() * 4);
     hipLaunchKernelGGL(init_bounds_kernel, dim3(1), dim3(64), 0, s, bounds);
-    hipLaunchKernelGGL(bounds_kernel, dim3(blocks_for(nR, gb)), dim3(256), 0, s, R, nR, d, bounds);
-    if (!self_join) hipLaunchKernelGGL(bounds_kernel, dim3(blocks_for(nS, gb)), dim3(256), 0, s, S, nS, d, bounds);
+    const unsigned bb = (unsigned)(num_cus() * 2);
+    hipLaunchKernelGGL(bounds_kernel, dim3(blocks_for(nR, bb)), dim3(256), 0, s, R, nR, d, bounds);
+    if (!self_join) hipLaunchKernelGGL(bounds_kernel, dim3(blocks_for(nS, bb)), dim3(256), 0, s, S, nS, d, bounds);
     hipLaunchKernelGGL(grid_params_kernel, dim3(1), dim3(64), 0, s, bounds, T, d, L.idxbits, L.bm_bits, gp);
     uint64_t* bm = (uint64_t*)(ws + L.bitmap);
     hipLaunchKernelGGL(clear_bitmap_kernel, dim3(blocks_for((int64_t)(L.bm_bits / 64), gb)), dim3(256), 0, s, gp, bm);

@@ -14,22 +14,41 @@ namespace {
 // ---------------------------------------------------------------- scale scan
 // max |x| over the finite coordinates (binary32 bits; non-negative floats
 // order like their bit patterns) and a count of non-finite coordinates.
+// 16-byte loads over the aligned body (scalar head and tail), a block
+// reduction in LDS, and one atomic per block: per-wave atomics on the one
+// address serialised into the kernel's largest cost.
 __global__ void __launch_bounds__(256) scale_scan_kernel(const float* __restrict__ x, int64_t count,
                                                          uint32_t* maxabs, uint64_t* stats) {
     uint32_t m = 0;
     uint32_t bad = 0;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < count; t += stride) {
-        uint32_t u = __builtin_bit_cast(uint32_t, x[t]) & 0x7FFFFFFFu;
+    auto take = [&](float f) {
+        const uint32_t u = __builtin_bit_cast(uint32_t, f) & 0x7FFFFFFFu;
         if (u >= 0x7F800000u) bad++;
         else m = u > m ? u : m;
+    };
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t head = (int64_t)(((16 - ((uintptr_t)x & 15)) & 15) / 4);  // floats before the first 16-byte boundary
+    const int64_t h = head < count ? head : count;
+    const int64_t nv = (count - h) / 4;
+    const float4* xv = (const float4*)(x + h);
+    for (int64_t t = tid; t < nv; t += stride) {
+        const float4 v = xv[t];
+        take(v.x); take(v.y); take(v.z); take(v.w);
     }
+    for (int64_t t = tid; t < h; t += stride) take(x[t]);
+    for (int64_t t = h + 4 * nv + tid; t < count; t += stride) take(x[t]);
     for (int o = 32; o > 0; o >>= 1) {
         uint32_t om = __shfl_xor(m, o);
         m = om > m ? om : m;
         bad += __shfl_xor(bad, o);
     }
-    if ((threadIdx.x & 63) == 0) {
+    __shared__ uint32_t sm[4], sb[4];
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { sm[wave] = m; sb[wave] = bad; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) { m = sm[w] > m ? sm[w] : m; bad += sb[w]; }
         if (m) atomicMax(maxabs, m);
         if (bad) atomicAdd((unsigned long long*)&stats[TJ_ST_INVALID], (unsigned long long)bad);
     }
@@ -68,8 +87,9 @@ namespace tj {
 int launch_scale_scan(const float* x, int64_t n, int d, uint32_t* maxabs, uint64_t* stats, hipStream_t s) {
     int64_t count = n * (int64_t)d;
     if (count == 0) return TJ_OK;
-    int64_t blocks = (count + 255) / 256;
-    if (blocks > 4096) blocks = 4096;
+    int64_t blocks = (count + 1023) / 1024;
+    const int64_t cap = 2 * (int64_t)num_cus();
+    if (blocks > cap) blocks = cap;
     hipLaunchKernelGGL(scale_scan_kernel, dim3((unsigned)blocks), dim3(256), 0, s, x, count, maxabs, stats);
     TJ_HIP(hipGetLastError());
     return TJ_OK;
