@@ -9,11 +9,14 @@
 //    atomics), cell keys, my radix sort by cell, S gathered into cell order
 //    as 16-byte rows, and an open-addressing hash of the non-empty cells
 //    (memory bounded by 2|S| slots however sparse the grid is);
-//  * a bitmap of the non-empty cells (when the grid has at most 64 |S|
-//    cells) answers "is this neighbour cell empty" for the three x-adjacent
-//    cells of a row with one or two 8-byte reads: most neighbour cells of a
-//    sparse region are empty, and each empty one would otherwise cost a
-//    hash probe sequence of random 64-byte lines;
+//  * when the grid has at most 64 |S| cells, no hash at all: a bitmap of
+//    the non-empty cells with a prefix count per 64-bit word ranks every
+//    cell, and runs[rank] is where its S rows start.  The three x-adjacent
+//    cells of a neighbour row are consecutive ids, so their S rows are one
+//    contiguous range [runs[rank(mid - 1)], runs[rank(mid - 1) + occupied]):
+//    a query reads nine bitmap words (shared by the cell-ordered lanes of a
+//    wave), nine prefix counts and 18 run bounds, where hashing cost a probe
+//    sequence of random 64-byte lines per neighbour cell, empty or not;
 //  * queries run in cell order (R is sorted by cell too), so the 64 lanes of
 //    a wave visit the same neighbour cells and the same S rows;
 //  * output: a count pass, an exclusive scan, a write pass of pair keys at
@@ -189,8 +192,19 @@ __global__ void __launch_bounds__(256) gather_kernel(const float* __restrict__ S
         if (g.d > 2) r.z = y[2];
         srow[p] = r;
         if (p == 0 || (keys[p - 1] >> g.idxbits) != cell) {
-            int64_t e = p + 1;
-            while (e < n && (keys[e] >> g.idxbits) == cell) ++e;
+            if (g.use_bm) {
+                atomicOr((unsigned long long*)&bm[cell >> 6], 1ull << (cell & 63));
+                continue;
+            }
+            // end of the run: galloping search over the sorted keys
+            int64_t lo = p, step = 1;  // keys[lo] is in the cell
+            while (lo + step < n && (keys[lo + step] >> g.idxbits) == cell) { lo += step; step <<= 1; }
+            int64_t hi = lo + step < n ? lo + step : n;  // keys[hi] is past the cell (or hi = n)
+            while (hi - lo > 1) {
+                const int64_t mid = lo + (hi - lo) / 2;
+                if ((keys[mid] >> g.idxbits) == cell) lo = mid; else hi = mid;
+            }
+            const int64_t e = hi;
             uint64_t h = hmix(cell) & hmask;
             for (;;) {
                 const unsigned long long prev = atomicCAS((unsigned long long*)&hkey[h], (unsigned long long)EMPTY,
@@ -199,8 +213,93 @@ __global__ void __launch_bounds__(256) gather_kernel(const float* __restrict__ S
                 h = (h + 1) & hmask;
             }
             hval[h] = make_uint2((uint32_t)p, (uint32_t)e);
-            if (g.use_bm) atomicOr((unsigned long long*)&bm[cell >> 6], 1ull << (cell & 63));
         }
+    }
+}
+
+// Exclusive prefix of the bitmap words' popcounts: bmpre[w] = occupied cells
+// below word w.  Three passes (block sums of 1024 words, one block over the
+// sums, apply); the word count comes from the device-side grid parameters.
+constexpr int BMB = 1024;
+__device__ inline int64_t bm_words(const GridParams& g) { return g.use_bm ? (int64_t)((g.cells + 63) / 64) : 0; }
+__global__ void __launch_bounds__(256) bm_block_sums(const GridParams* __restrict__ gp, const uint64_t* __restrict__ bm,
+                                                     uint32_t* __restrict__ bsum) {
+    const int64_t W = bm_words(*gp);
+    const int64_t b0 = (int64_t)blockIdx.x * BMB;
+    if (b0 >= W) return;
+    uint32_t s = 0;
+    for (int k = threadIdx.x; k < BMB; k += 256) if (b0 + k < W) s += (uint32_t)__popcll(bm[b0 + k]);
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    __shared__ uint32_t part[4];
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) bsum[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
+}
+__global__ void __launch_bounds__(1024) bm_top_scan(const GridParams* __restrict__ gp, uint32_t* __restrict__ bsum) {
+    const int64_t nb = (bm_words(*gp) + BMB - 1) / BMB;
+    __shared__ uint32_t part[1024];
+    const int64_t per = (nb + 1023) / 1024;
+    const int64_t a = threadIdx.x * per, b = a + per < nb ? a + per : nb;
+    uint32_t s = 0;
+    for (int64_t t = a; t < b; ++t) s += bsum[t];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        const uint32_t v = threadIdx.x >= (unsigned)o ? part[threadIdx.x - o] : 0;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = threadIdx.x ? part[threadIdx.x - 1] : 0;
+    for (int64_t t = a; t < b; ++t) {
+        const uint32_t v = bsum[t];
+        bsum[t] = run;
+        run += v;
+    }
+}
+__global__ void __launch_bounds__(256) bm_apply(const GridParams* __restrict__ gp, const uint64_t* __restrict__ bm,
+                                                const uint32_t* __restrict__ bsum, uint32_t* __restrict__ bmpre) {
+    const int64_t W = bm_words(*gp);
+    const int64_t b0 = (int64_t)blockIdx.x * BMB;
+    if (b0 >= W) return;
+    uint32_t v[4], s = 0;  // each thread owns 4 consecutive words
+    for (int u = 0; u < 4; ++u) {
+        const int64_t t = b0 + threadIdx.x * 4 + u;
+        v[u] = t < W ? (uint32_t)__popcll(bm[t]) : 0;
+        s += v[u];
+    }
+    __shared__ uint32_t part[256];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 1; o < 256; o <<= 1) {
+        const uint32_t w = threadIdx.x >= (unsigned)o ? part[threadIdx.x - o] : 0;
+        __syncthreads();
+        part[threadIdx.x] += w;
+        __syncthreads();
+    }
+    uint32_t run = bsum[blockIdx.x] + (threadIdx.x ? part[threadIdx.x - 1] : 0);
+    for (int u = 0; u < 4; ++u) {
+        const int64_t t = b0 + threadIdx.x * 4 + u;
+        if (t < W) bmpre[t] = run;
+        run += v[u];
+    }
+}
+__device__ inline uint32_t cell_rank(const uint64_t* __restrict__ bm, const uint32_t* __restrict__ bmpre, uint64_t cell) {
+    const uint64_t w = bm[cell >> 6];
+    const uint32_t b = (uint32_t)(cell & 63);
+    return bmpre[cell >> 6] + (uint32_t)__popcll(b ? (w & ((1ull << b) - 1)) : 0ull);
+}
+// runs[rank(cell)] = first sorted S position of each non-empty cell; the
+// entry after the last run holds n.
+__global__ void __launch_bounds__(256) runs_kernel(int64_t n, const GridParams* __restrict__ gp,
+                                                   const uint64_t* __restrict__ keys, const uint64_t* __restrict__ bm,
+                                                   const uint32_t* __restrict__ bmpre, uint32_t* __restrict__ runs) {
+    const GridParams g = *gp;
+    if (!g.use_bm) return;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t cell = keys[p] >> g.idxbits;
+        if (p == 0 || (keys[p - 1] >> g.idxbits) != cell) runs[cell_rank(bm, bmpre, cell)] = (uint32_t)p;
+        if (p == n - 1) runs[cell_rank(bm, bmpre, cell) + 1] = (uint32_t)n;
     }
 }
 
@@ -228,8 +327,9 @@ template <int MODE>
 __global__ void __launch_bounds__(256) probe_kernel(const float* __restrict__ R, int64_t nR, const GridParams* __restrict__ gp,
                                                     const uint64_t* __restrict__ rkeys, const float4* __restrict__ srow,
                                                     const uint64_t* __restrict__ hkey, const uint2* __restrict__ hval,
-                                                    uint64_t hmask, const uint64_t* __restrict__ bm, double T,
-                                                    int self_join, uint32_t* __restrict__ cnt,
+                                                    uint64_t hmask, const uint64_t* __restrict__ bm,
+                                                    const uint32_t* __restrict__ bmpre, const uint32_t* __restrict__ runs,
+                                                    double T, int self_join, uint32_t* __restrict__ cnt,
                                                     const uint64_t* __restrict__ off, uint64_t* __restrict__ out_keys,
                                                     int jbits, int64_t out_cap, uint64_t* __restrict__ stats) {
     const GridParams g = *gp;
@@ -250,19 +350,27 @@ __global__ void __launch_bounds__(256) probe_kernel(const float* __restrict__ R,
                 // ids of the x-adjacent cells: mid - 1, mid, mid + 1 (cell_of
                 // keeps c in [1, ncell - 2], so all three exist)
                 const uint64_t mid = (uint64_t)c[0] + (uint64_t)(c[1] + dy) * g.mul[1] + (uint64_t)(c[2] + dz) * g.mul[2];
-                uint32_t occ = 7;  // bit dx + 1: that cell may hold S points
+                // the S rows of the row's cells: one range (bitmap) or one
+                // hash run per cell
+                uint2 rng[3];
+                int nrng = 0;
                 if (g.use_bm) {
                     const uint64_t b0 = mid - 1, b2 = mid + 1;
                     const uint64_t w0 = bm[b0 >> 6];
                     const uint64_t w2 = (b2 >> 6) == (b0 >> 6) ? w0 : bm[b2 >> 6];
                     const uint32_t sh = (uint32_t)(b0 & 63);
-                    occ = (uint32_t)((w0 >> sh) & 7);
+                    uint32_t occ = (uint32_t)((w0 >> sh) & 7);
                     if (sh > 61) occ |= (uint32_t)(w2 << (64 - sh)) & 7;
+                    if (occ) {
+                        const uint32_t r0 = bmpre[b0 >> 6] + (uint32_t)__popcll(sh ? (w0 & ((1ull << sh) - 1)) : 0ull);
+                        rng[0] = make_uint2(runs[r0], runs[r0 + (uint32_t)__popc(occ)]);
+                        nrng = 1;
+                    }
+                } else {
+                    for (int dx = -1; dx <= 1; ++dx) rng[nrng++] = lookup(hkey, hval, hmask, mid + (uint64_t)(int64_t)dx);
                 }
-                for (int dx = -1; dx <= 1; ++dx) {
-                    if (!((occ >> (dx + 1)) & 1)) continue;
-                    const uint64_t cell = mid + (uint64_t)(int64_t)dx;
-                    const uint2 run = lookup(hkey, hval, hmask, cell);
+                for (int ri = 0; ri < nrng; ++ri) {
+                    const uint2 run = rng[ri];
                     for (uint32_t p = run.x; p < run.y; ++p) {
                         const float4 y = srow[p];
                         const uint32_t j = __builtin_bit_cast(uint32_t, y.w);
@@ -379,6 +487,9 @@ GridLayout grid_layout(int64_t nR, int64_t nS, int64_t out_cap) {
     // go without it
     L.bm_bits = std::max<uint64_t>(1ull << 20, 64ull * (uint64_t)(nS > 0 ? nS : 1));
     L.bitmap = carve(L.bm_bits / 8);
+    L.bmpre = carve(4 * (L.bm_bits / 64));
+    L.bmsum = carve(4 * (L.bm_bits / 64 / BMB + 1));
+    L.runs = carve(4 * (size_t)((nS > 0 ? nS : 1) + 1));
     L.total = (cur + 255) / 256 * 256;
     return L;
 }
@@ -430,16 +541,27 @@ int launch_grid_join(const float* R, int64_t nR, const float* S, int64_t nS, int
     if ((rc = launch_sort(ws + L.sort_tmp, skeys, cS, nS, 63, s))) return rc;
     hipLaunchKernelGGL(gather_kernel, dim3(blocks_for(nS, gb)), dim3(256), 0, s, S, nS, gp, skeys, srow, hkey, hval,
                        (uint64_t)(L.hslots - 1), bm);
+    // rank structure of the bitmap (its kernels return at once without one)
+    uint32_t* bmpre = (uint32_t*)(ws + L.bmpre);
+    uint32_t* bmsum = (uint32_t*)(ws + L.bmsum);
+    uint32_t* runs = (uint32_t*)(ws + L.runs);
+    const unsigned nbm = (unsigned)((L.bm_bits / 64 + BMB - 1) / BMB);
+    hipLaunchKernelGGL(bm_block_sums, dim3(nbm), dim3(256), 0, s, gp, bm, bmsum);
+    hipLaunchKernelGGL(bm_top_scan, dim3(1), dim3(1024), 0, s, gp, bmsum);
+    hipLaunchKernelGGL(bm_apply, dim3(nbm), dim3(256), 0, s, gp, bm, bmsum, bmpre);
+    hipLaunchKernelGGL(runs_kernel, dim3(blocks_for(nS, gb)), dim3(256), 0, s, nS, gp, skeys, bm, bmpre, runs);
     // count -> scan -> write -> per-row sort
     const unsigned pb = blocks_for(nR, (int64_t)num_cus() * 16);
     hipLaunchKernelGGL(probe_kernel<0>, dim3(pb), dim3(256), 0, s, R, nR, gp, rkeys, srow, hkey, hval,
-                       (uint64_t)(L.hslots - 1), bm, T, (int)self_join, cnt, off, okeys, jbits, out_cap, stats);
+                       (uint64_t)(L.hslots - 1), bm, bmpre, runs, T, (int)self_join, cnt, off, okeys, jbits, out_cap,
+                       stats);
     const int64_t nb = (nR + SCAN_B - 1) / SCAN_B;
     hipLaunchKernelGGL(scan_block_sums, dim3((unsigned)nb), dim3(256), 0, s, cnt, nR, bsum);
     hipLaunchKernelGGL(scan_sums, dim3(1), dim3(1024), 0, s, bsum, nb, stats);
     hipLaunchKernelGGL(scan_apply, dim3((unsigned)nb), dim3(256), 0, s, cnt, nR, bsum, off);
     hipLaunchKernelGGL(probe_kernel<1>, dim3(pb), dim3(256), 0, s, R, nR, gp, rkeys, srow, hkey, hval,
-                       (uint64_t)(L.hslots - 1), bm, T, (int)self_join, cnt, off, okeys, jbits, out_cap, stats);
+                       (uint64_t)(L.hslots - 1), bm, bmpre, runs, T, (int)self_join, cnt, off, okeys, jbits, out_cap,
+                       stats);
     TJ_HIP(hipGetLastError());
     // rows are already in order (the scan runs over i); one radix sort of the
     // keys orders each row's j's, then the keys unpack to pairs

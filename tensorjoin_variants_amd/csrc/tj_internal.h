@@ -216,7 +216,7 @@ namespace tj {
 // Workspace layout of the grid join (grid.hip), kept in the plan (api.hip).
 struct GridLayout {
     size_t bounds, params, rkeys, skeys, sort_tmp, srow, hkey, hval, cnt, off, bsum, outj, count_nr, count_ns, bitmap,
-        total;
+        bmpre, bmsum, runs, total;
     uint64_t hslots;
     uint64_t bm_bits;  // capacity of the cell-occupancy bitmap
     int idxbits;
