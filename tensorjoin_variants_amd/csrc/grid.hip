@@ -17,13 +17,14 @@
 //    a query reads nine bitmap words (shared by the cell-ordered lanes of a
 //    wave), nine prefix counts and 18 run bounds, where hashing cost a probe
 //    sequence of random 64-byte lines per neighbour cell, empty or not;
-//  * queries run in cell order (R is sorted by cell too), so the 64 lanes of
-//    a wave visit the same neighbour cells and the same S rows;
+//  * queries run in cell order (R is gathered into cell order as 16-byte
+//    rows too; a self-join reuses S's), so the 64 lanes of a wave visit the
+//    same neighbour cells and the same S rows, and read their queries,
+//    counts and offsets contiguously;
 //  * output: a count pass, an exclusive scan, a write pass of pair keys at
-//    exact per-row offsets (rows come out in i order), and one radix sort of
-//    the keys for the j's within each row (a per-row insertion sort was
-//    quadratic in the longest row: the dense blobs made it the costliest
-//    kernel);
+//    exact per-query offsets, and one radix sort of the keys over all their
+//    bits (a per-row insertion sort was quadratic in the longest row: the
+//    dense blobs made it the costliest kernel);
 //  * the predicate is the sequential binary64 difference form of the CPU
 //    oracle (d2 <= T; GDS used sqrt(d2) <= eps, GDS/kernel.cu:368-371).
 // The cell side is sqrt(T) * (1 + 2^-10) (or larger when the cell ids would
@@ -174,6 +175,23 @@ __device__ inline uint64_t hmix(uint64_t k) {
 }
 constexpr uint64_t EMPTY = ~0ull;
 
+// R in cell order: 16-byte rows (x, y, z, original index as bits), so the
+// probes read their queries and counts contiguously.
+__global__ void __launch_bounds__(256) rows_kernel(const float* __restrict__ x, int64_t n, const GridParams* __restrict__ gp,
+                                                   const uint64_t* __restrict__ keys, float4* __restrict__ rows) {
+    const GridParams g = *gp;
+    const uint64_t imask = (1ull << g.idxbits) - 1;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t j = keys[p] & imask;
+        float4 r = make_float4(0.f, 0.f, 0.f, __builtin_bit_cast(float, (uint32_t)j));
+        const float* y = x + j * g.d;
+        r.x = y[0];
+        if (g.d > 1) r.y = y[1];
+        if (g.d > 2) r.z = y[2];
+        rows[p] = r;
+    }
+}
+
 // S in cell order: 16-byte rows (x, y, z, original index as bits); run starts
 // of each non-empty cell inserted into the hash (slot: key, start, end).
 __global__ void __launch_bounds__(256) gather_kernel(const float* __restrict__ S, int64_t n, const GridParams* __restrict__ gp,
@@ -322,10 +340,11 @@ __device__ inline double d2_row(const float* q, const float4& y, int d) {
 }
 
 // One query per lane, queries in cell order.  mode 0: count into cnt[i];
-// mode 1: write the pair keys (i << jbits) | j at out_keys[off[i] + k].
+// mode 1: write the pair keys (i << jbits) | j at out_keys[off[t] + k].
+// Query t is row t of R in cell order (rrow: x, y, z, original index).
 template <int MODE>
-__global__ void __launch_bounds__(256) probe_kernel(const float* __restrict__ R, int64_t nR, const GridParams* __restrict__ gp,
-                                                    const uint64_t* __restrict__ rkeys, const float4* __restrict__ srow,
+__global__ void __launch_bounds__(256) probe_kernel(const float4* __restrict__ rrow, int64_t nR, const GridParams* __restrict__ gp,
+                                                    const float4* __restrict__ srow,
                                                     const uint64_t* __restrict__ hkey, const uint2* __restrict__ hval,
                                                     uint64_t hmask, const uint64_t* __restrict__ bm,
                                                     const uint32_t* __restrict__ bmpre, const uint32_t* __restrict__ runs,
@@ -333,17 +352,16 @@ __global__ void __launch_bounds__(256) probe_kernel(const float* __restrict__ R,
                                                     const uint64_t* __restrict__ off, uint64_t* __restrict__ out_keys,
                                                     int jbits, int64_t out_cap, uint64_t* __restrict__ stats) {
     const GridParams g = *gp;
-    const uint64_t imask = (1ull << g.idxbits) - 1;
     uint64_t cand = 0;
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nR; t += (int64_t)gridDim.x * blockDim.x) {
-        const uint64_t i = rkeys[t] & imask;
-        float q[3] = {0.f, 0.f, 0.f};
-        for (int k = 0; k < g.d; ++k) q[k] = R[i * g.d + k];
+        const float4 qr = rrow[t];
+        const uint64_t i = __builtin_bit_cast(uint32_t, qr.w);
+        const float q[3] = {qr.x, qr.y, qr.z};
         int64_t c[3];
         cell_of(q, g, c);
         uint32_t n = 0;
         uint64_t base = 0;
-        if (MODE == 1) base = off[i];
+        if (MODE == 1) base = off[t];
         const int r1 = g.d > 1 ? 1 : 0, r2 = g.d > 2 ? 1 : 0;
         for (int dz = -r2; dz <= r2; ++dz)
             for (int dy = -r1; dy <= r1; ++dy) {
@@ -369,21 +387,32 @@ __global__ void __launch_bounds__(256) probe_kernel(const float* __restrict__ R,
                 } else {
                     for (int dx = -1; dx <= 1; ++dx) rng[nrng++] = lookup(hkey, hval, hmask, mid + (uint64_t)(int64_t)dx);
                 }
+                auto visit = [&](const float4& y) {
+                    const uint32_t j = __builtin_bit_cast(uint32_t, y.w);
+                    if (self_join && j <= i) return;
+                    ++cand;
+                    if (d2_row(q, y, g.d) <= T) {
+                        if (MODE == 1 && base + n < (uint64_t)out_cap) out_keys[base + n] = (i << jbits) | j;
+                        ++n;
+                    }
+                };
                 for (int ri = 0; ri < nrng; ++ri) {
                     const uint2 run = rng[ri];
-                    for (uint32_t p = run.x; p < run.y; ++p) {
-                        const float4 y = srow[p];
-                        const uint32_t j = __builtin_bit_cast(uint32_t, y.w);
-                        if (self_join && j <= i) continue;
-                        ++cand;
-                        if (d2_row(q, y, g.d) <= T) {
-                            if (MODE == 1 && base + n < (uint64_t)out_cap) out_keys[base + n] = (i << jbits) | j;
-                            ++n;
-                        }
+                    // eight rows in flight at a time: near a dense blob a
+                    // lane's range holds hundreds of rows, and one load per
+                    // iteration made those few waves the whole kernel's tail
+                    uint32_t p = run.x;
+                    for (; p + 8 <= run.y; p += 8) {
+                        float4 y[8];
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) y[u] = srow[p + u];
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) visit(y[u]);
                     }
+                    for (; p < run.y; ++p) visit(srow[p]);
                 }
             }
-        if (MODE == 0) cnt[i] = n;
+        if (MODE == 0) cnt[t] = n;
     }
     if (MODE == 0) {
         for (int s = 32; s > 0; s >>= 1) cand += __shfl_xor(cand, s);
@@ -477,6 +506,7 @@ GridLayout grid_layout(int64_t nR, int64_t nS, int64_t out_cap) {
     L.skeys = carve(8 * (size_t)(nS > 0 ? nS : 1));
     L.sort_tmp = carve(sort_temp_bytes(std::max<int64_t>(std::max<int64_t>(nmax, out_cap), 1)));
     L.srow = carve(16 * (size_t)(nS > 0 ? nS : 1));
+    L.rrow = carve(16 * (size_t)(nR > 0 ? nR : 1));
     L.hkey = carve(8 * hs);
     L.hval = carve(8 * hs);
     L.cnt = carve(4 * (size_t)(nR > 0 ? nR : 1));
@@ -534,9 +564,14 @@ int launch_grid_join(const float* R, int64_t nR, const float* S, int64_t nS, int
     if ((rc = launch_fill_u64(cR, 1, (uint64_t)nR, s))) return rc;
     if ((rc = launch_fill_u64(cS, 1, (uint64_t)nS, s))) return rc;
     if ((rc = launch_fill_u64(hkey, (int64_t)L.hslots, EMPTY, s))) return rc;
-    // R and S sorted by cell (key = cell id << idxbits | index)
-    hipLaunchKernelGGL(keys_kernel, dim3(blocks_for(nR, gb)), dim3(256), 0, s, R, nR, gp, rkeys);
-    if ((rc = launch_sort(ws + L.sort_tmp, rkeys, cR, nR, 63, s))) return rc;
+    // R and S sorted by cell (key = cell id << idxbits | index); a self-join
+    // sorts once and reads its queries from S's rows
+    float4* rrow = self_join ? srow : (float4*)(ws + L.rrow);
+    if (!self_join) {
+        hipLaunchKernelGGL(keys_kernel, dim3(blocks_for(nR, gb)), dim3(256), 0, s, R, nR, gp, rkeys);
+        if ((rc = launch_sort(ws + L.sort_tmp, rkeys, cR, nR, 63, s))) return rc;
+        hipLaunchKernelGGL(rows_kernel, dim3(blocks_for(nR, gb)), dim3(256), 0, s, R, nR, gp, rkeys, rrow);
+    }
     hipLaunchKernelGGL(keys_kernel, dim3(blocks_for(nS, gb)), dim3(256), 0, s, S, nS, gp, skeys);
     if ((rc = launch_sort(ws + L.sort_tmp, skeys, cS, nS, 63, s))) return rc;
     hipLaunchKernelGGL(gather_kernel, dim3(blocks_for(nS, gb)), dim3(256), 0, s, S, nS, gp, skeys, srow, hkey, hval,
@@ -550,21 +585,21 @@ int launch_grid_join(const float* R, int64_t nR, const float* S, int64_t nS, int
     hipLaunchKernelGGL(bm_top_scan, dim3(1), dim3(1024), 0, s, gp, bmsum);
     hipLaunchKernelGGL(bm_apply, dim3(nbm), dim3(256), 0, s, gp, bm, bmsum, bmpre);
     hipLaunchKernelGGL(runs_kernel, dim3(blocks_for(nS, gb)), dim3(256), 0, s, nS, gp, skeys, bm, bmpre, runs);
-    // count -> scan -> write -> per-row sort
+    // count -> scan -> write -> key sort
     const unsigned pb = blocks_for(nR, (int64_t)num_cus() * 16);
-    hipLaunchKernelGGL(probe_kernel<0>, dim3(pb), dim3(256), 0, s, R, nR, gp, rkeys, srow, hkey, hval,
+    hipLaunchKernelGGL(probe_kernel<0>, dim3(pb), dim3(256), 0, s, rrow, nR, gp, srow, hkey, hval,
                        (uint64_t)(L.hslots - 1), bm, bmpre, runs, T, (int)self_join, cnt, off, okeys, jbits, out_cap,
                        stats);
     const int64_t nb = (nR + SCAN_B - 1) / SCAN_B;
     hipLaunchKernelGGL(scan_block_sums, dim3((unsigned)nb), dim3(256), 0, s, cnt, nR, bsum);
     hipLaunchKernelGGL(scan_sums, dim3(1), dim3(1024), 0, s, bsum, nb, stats);
     hipLaunchKernelGGL(scan_apply, dim3((unsigned)nb), dim3(256), 0, s, cnt, nR, bsum, off);
-    hipLaunchKernelGGL(probe_kernel<1>, dim3(pb), dim3(256), 0, s, R, nR, gp, rkeys, srow, hkey, hval,
+    hipLaunchKernelGGL(probe_kernel<1>, dim3(pb), dim3(256), 0, s, rrow, nR, gp, srow, hkey, hval,
                        (uint64_t)(L.hslots - 1), bm, bmpre, runs, T, (int)self_join, cnt, off, okeys, jbits, out_cap,
                        stats);
     TJ_HIP(hipGetLastError());
-    // rows are already in order (the scan runs over i); one radix sort of the
-    // keys orders each row's j's, then the keys unpack to pairs
+    // the keys were written in query (cell) order; one radix sort over all
+    // their bits orders them by (i, j), then they unpack to pairs
     uint64_t* sorted = okeys;
     const int keybits = bits_for(nR > 1 ? nR : 2) + jbits;
     if ((rc = launch_sort(ws + L.sort_tmp, okeys, stats + TJ_ST_OUT, out_cap, keybits, s, &sorted))) return rc;
