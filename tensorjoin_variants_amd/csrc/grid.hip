@@ -565,15 +565,17 @@ int launch_grid_join(const float* R, int64_t nR, const float* S, int64_t nS, int
     if ((rc = launch_fill_u64(cS, 1, (uint64_t)nS, s))) return rc;
     if ((rc = launch_fill_u64(hkey, (int64_t)L.hslots, EMPTY, s))) return rc;
     // R and S sorted by cell (key = cell id << idxbits | index); a self-join
-    // sorts once and reads its queries from S's rows
+    // sorts once and reads its queries from S's rows.  The keys are written
+    // in index order, so the digits wholly inside the index need no pass.
+    const int lo_digit = L.idxbits / 8 * 8;
     float4* rrow = self_join ? srow : (float4*)(ws + L.rrow);
     if (!self_join) {
         hipLaunchKernelGGL(keys_kernel, dim3(blocks_for(nR, gb)), dim3(256), 0, s, R, nR, gp, rkeys);
-        if ((rc = launch_sort(ws + L.sort_tmp, rkeys, cR, nR, 63, s))) return rc;
+        if ((rc = launch_sort(ws + L.sort_tmp, rkeys, cR, nR, 63, s, nullptr, lo_digit))) return rc;
         hipLaunchKernelGGL(rows_kernel, dim3(blocks_for(nR, gb)), dim3(256), 0, s, R, nR, gp, rkeys, rrow);
     }
     hipLaunchKernelGGL(keys_kernel, dim3(blocks_for(nS, gb)), dim3(256), 0, s, S, nS, gp, skeys);
-    if ((rc = launch_sort(ws + L.sort_tmp, skeys, cS, nS, 63, s))) return rc;
+    if ((rc = launch_sort(ws + L.sort_tmp, skeys, cS, nS, 63, s, nullptr, lo_digit))) return rc;
     hipLaunchKernelGGL(gather_kernel, dim3(blocks_for(nS, gb)), dim3(256), 0, s, S, nS, gp, skeys, srow, hkey, hval,
                        (uint64_t)(L.hslots - 1), bm);
     // rank structure of the bitmap (its kernels return at once without one)

@@ -232,7 +232,7 @@ size_t sort_temp_bytes(int64_t capacity) {
 }
 
 int launch_sort(void* temp, uint64_t* keys, const uint64_t* count, int64_t capacity, int bits, hipStream_t s,
-                uint64_t** result) {
+                uint64_t** result, int lo_bit) {
     if (result) *result = keys;
     if (capacity <= 0 || bits <= 0) return TJ_OK;
     const int nb = nblocks_for(capacity);
@@ -242,7 +242,7 @@ int launch_sort(void* temp, uint64_t* keys, const uint64_t* count, int64_t capac
     const int passes = (bits + 7) / 8;
     uint64_t* src = keys;
     uint64_t* dst = alt;
-    for (int p = 0; p < passes; ++p) {
+    for (int p = lo_bit / 8; p < passes; ++p) {
         hipLaunchKernelGGL(hist_kernel, dim3(nb), dim3(SB), 0, s, src, count, capacity, 8 * p, hist, nb);
         hipLaunchKernelGGL(scan_rows_kernel, dim3(256), dim3(256), 0, s, hist, nb, tot);
         hipLaunchKernelGGL(scan_totals_kernel, dim3(1), dim3(256), 0, s, tot);

@@ -252,9 +252,12 @@ int launch_plain(const float* R, int64_t nR, const float* S, int64_t nS, int d, 
 size_t sort_temp_bytes(int64_t capacity);
 // result == NULL: the sorted keys end in `keys`; otherwise *result names the
 // buffer holding them (keys or the ping-pong half of temp), saving the copy
-// back after an odd number of passes.
+// back after an odd number of passes.  lo_bit (a multiple of 8) skips the
+// digits below it: keys already in order of those bits among equal higher
+// bits (e.g. written in index order with the index in the low bits) come
+// out as from the full sort.
 int launch_sort(void* temp, uint64_t* keys, const uint64_t* count, int64_t capacity, int bits, hipStream_t s,
-                uint64_t** result = nullptr);
+                uint64_t** result = nullptr, int lo_bit = 0);
 int launch_unpack(const uint64_t* keys, const uint64_t* count, int64_t cap, int jbits, void* pairs,
                   int out_format, hipStream_t s);
 int launch_fill_u64(uint64_t* p, int64_t n, uint64_t v, hipStream_t s);
