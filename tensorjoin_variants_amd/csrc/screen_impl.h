@@ -585,13 +585,15 @@ __device__ __forceinline__ void screen_panel(const PanelIO& io, const CandOut& c
             // >= w_j + a_w despite the rounding of the sum (and, unseeded, s
             // >= uj still gives t_i + p >= w_j + a_w despite that of s)
             const float uj = u + (fabsf(u) * 0x1p-22f + 0x1p-126f);
-            float tr[16];  // unseeded: t of the lane's 16 rows
+            // unseeded: a per-lane level thr <= v_j - max_r t_r (the binary32
+            // difference, lowered by 2^-22 of itself: more than its rounding
+            // and that of the margin) screens the 16 registers with one
+            // compare each; p_r + t_r >= v_j implies p_r >= thr, so only the
+            // registers that pass form s_r = fl(p_r + t_r) with their own t
+            float thr = 0.0f;
             if constexpr (!SEEDED) {
-#pragma unroll
-                for (int g = 0; g < 4; ++g) {
-                    const float4 tv = *(const float4*)(sT + rb * 32 + 8 * g + 4 * hh);
-                    tr[4 * g] = tv.x; tr[4 * g + 1] = tv.y; tr[4 * g + 2] = tv.z; tr[4 * g + 3] = tv.w;
-                }
+                const float dl = nj - tmax[rb];
+                thr = dl - (fabsf(dl) * 0x1p-22f + 0x1p-126f);
             }
             // Stage register r's survivors; DIAG adds the self-join's i < j:
             // off(r) < e <=> ibase + off(r) < jcol.  Ballots of single
@@ -609,18 +611,24 @@ __device__ __forceinline__ void screen_panel(const PanelIO& io, const CandOut& c
                 uint64_t ms[16];
 #pragma unroll
                 for (int r = 0; r < 16; ++r) {
-                    ms[r] = __builtin_amdgcn_ballot_w64((SEEDED ? acc[rb][r] : acc[rb][r] + tr[r]) >= nj);
+                    ms[r] = __builtin_amdgcn_ballot_w64(acc[rb][r] >= (SEEDED ? nj : thr));
                     if constexpr (DIAG) ms[r] &= __builtin_amdgcn_ballot_w64(8 * (r >> 2) + (r & 3) < e);
                 }
 #pragma unroll
                 for (int r = 0; r < 16; ++r) {
                     const int off = 8 * (r >> 2) + (r & 3);
-                    const float sv = SEEDED ? acc[rb][r] : acc[rb][r] + tr[r];
-                    const uint64_t m = ms[r];
+                    float sv = acc[rb][r];
+                    uint64_t m = ms[r];
                     if (m == 0) continue;
-                    // the lane's own survivor test: the compare itself off the
-                    // diagonal (one VALU op), the staged mask on it
-                    const bool mine = DIAG ? ((m >> lane) & 1) != 0 : sv >= nj;
+                    if constexpr (!SEEDED) {
+                        sv += sT[rb * 32 + off + 4 * hh];
+                        m &= __builtin_amdgcn_ballot_w64(sv >= nj);
+                        if (m == 0) continue;
+                    }
+                    // the lane's own survivor test: the compare itself for a
+                    // seeded body off the diagonal (one VALU op), else its
+                    // bit of the mask (two tests made it)
+                    const bool mine = (DIAG || !SEEDED) ? ((m >> lane) & 1) != 0 : sv >= nj;
                     const int k = __popcll(m);
                     nC = uni(nC);  // wave-uniform: keeps the capacity test a scalar branch
                     if (__builtin_expect(nC + k > CC, 0)) cand_flush(co, sCK, sCP, nC, nAcc, lane);  // out of line
