@@ -210,8 +210,16 @@ int tj_plan_create(tj_plan** out, int64_t nR, int64_t nS, int d, int method, int
         p->panels = (int)((nR + p->bm - 1) / p->bm);
         const int maxg = screen_max_grid(p->prec, p->dp, p->self_join);
         if (p->self_join) {
-            p->col_splits = 1;
-            p->cols_per_split = (int)std::min<int64_t>(p->nS, 1ll << 30);
+            // Column splits of the triangle's rows (panel p sweeps columns
+            // from its first row on, split by split; splits past nS are empty
+            // items): enough items to balance the persistent grid when a
+            // small join has few panels, none narrower than 1024 columns.
+            const int64_t want = (8ll * maxg + std::max(p->panels, 1) - 1) / std::max(p->panels, 1);
+            const int64_t by_width = (p->nS + 1023) / 1024;
+            const int64_t cs = std::max<int64_t>(1, std::min(want, by_width));
+            const int64_t per = ((p->nS + cs - 1) / cs + 31) / 32 * 32;
+            p->cols_per_split = (int)std::max<int64_t>(per, 32);
+            p->col_splits = (int)((p->nS + p->cols_per_split - 1) / p->cols_per_split);
         } else {
             // Column splits of about 4 MiB of quantized S (one XCD's L2),
             // items ordered split-major: the workgroups running at any time

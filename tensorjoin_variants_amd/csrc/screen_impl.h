@@ -700,8 +700,9 @@ __global__ void __launch_bounds__(256, (min_blocks<PREC, DP, RB>())) screen_kern
         const int split = item / A.panels;
         const int64_t r0 = (int64_t)panel * BM;
         int64_t cb, ce;
-        if (SELF) { cb = r0; ce = A.nb; }
-        else { cb = (int64_t)split * A.cols_per_split; ce = cb + A.cols_per_split; if (ce > A.nb) ce = A.nb; }
+        cb = (SELF ? r0 : 0) + (int64_t)split * A.cols_per_split;  // a self-join's panel starts at its diagonal
+        ce = cb + A.cols_per_split;
+        if (ce > A.nb) ce = A.nb;
         if (cb >= ce) continue;
         screen_panel<PREC, DP, RB, SELF>(io, co, r0, cb, ce, sB, sNp, sW, sSc, sCK[wave], sCP[wave], sT[wave], nC,
                                          nAcc, !A.audit);

@@ -54,7 +54,11 @@ def test_self_join_plan(L):
     buf = C.create_string_buffer(1024)
     L.tj_plan_describe(p, buf, 1024)
     desc = json.loads(buf.value.decode())
-    assert desc["self"] == 1 and desc["nS"] == 8192 and desc["col_splits"] == 1
+    assert desc["self"] == 1 and desc["nS"] == 8192
+    # the triangle's columns are split for balance: whole 32-column blocks,
+    # at least 1024 columns a split, covering every column
+    cps, cs = desc["cols_per_split"], desc["col_splits"]
+    assert cps % 32 == 0 and cps >= 1024 and cs * cps >= 8192 and (cs - 1) * cps < 8192
     L.tj_plan_destroy(p)
 
 
