@@ -352,27 +352,59 @@ __global__ void __launch_bounds__(256) probe_kernel(const float4* __restrict__ r
                                                     const uint64_t* __restrict__ off, uint64_t* __restrict__ out_keys,
                                                     int jbits, int64_t out_cap, uint64_t* __restrict__ stats) {
     const GridParams g = *gp;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __shared__ uint32_t sCnt[4][64];  // per-query hits of a wave's shared sweep
     uint64_t cand = 0;
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nR; t += (int64_t)gridDim.x * blockDim.x) {
-        const float4 qr = rrow[t];
+    // whole waves step together (tb is the wave's first query), so the
+    // shared sweep below always has all 64 lanes
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t tb = (int64_t)blockIdx.x * blockDim.x + wave * 64; tb < nR; tb += stride) {
+        const int64_t t = tb + lane;
+        const bool valid = t < nR;
+        const float4 qr = valid ? rrow[t] : make_float4(0.f, 0.f, 0.f, 0.f);
         const uint64_t i = __builtin_bit_cast(uint32_t, qr.w);
         const float q[3] = {qr.x, qr.y, qr.z};
         int64_t c[3];
         cell_of(q, g, c);
         uint32_t n = 0;
         uint64_t base = 0;
-        if (MODE == 1) base = off[t];
+        if (MODE == 1 && valid) base = off[t];
+        auto visit = [&](const float4& y) {
+            const uint32_t j = __builtin_bit_cast(uint32_t, y.w);
+            if (self_join && j <= i) return;
+            ++cand;
+            if (d2_row(q, y, g.d) <= T) {
+                if (MODE == 1 && base + n < (uint64_t)out_cap) out_keys[base + n] = (i << jbits) | j;
+                ++n;
+            }
+        };
+        auto sweep = [&](uint2 run) {
+            // eight rows in flight at a time
+            uint32_t p = run.x;
+            for (; p + 8 <= run.y; p += 8) {
+                float4 y[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) y[u] = srow[p + u];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) visit(y[u]);
+            }
+            for (; p < run.y; ++p) visit(srow[p]);
+        };
         const int r1 = g.d > 1 ? 1 : 0, r2 = g.d > 2 ? 1 : 0;
         for (int dz = -r2; dz <= r2; ++dz)
             for (int dy = -r1; dy <= r1; ++dy) {
                 // ids of the x-adjacent cells: mid - 1, mid, mid + 1 (cell_of
                 // keeps c in [1, ncell - 2], so all three exist)
                 const uint64_t mid = (uint64_t)c[0] + (uint64_t)(c[1] + dy) * g.mul[1] + (uint64_t)(c[2] + dz) * g.mul[2];
-                // the S rows of the row's cells: one range (bitmap) or one
-                // hash run per cell
-                uint2 rng[3];
-                int nrng = 0;
-                if (g.use_bm) {
+                if (!g.use_bm) {
+                    // one hash run per cell
+                    if (valid)
+                        for (int dx = -1; dx <= 1; ++dx) sweep(lookup(hkey, hval, hmask, mid + (uint64_t)(int64_t)dx));
+                    continue;
+                }
+                // the S rows of the row's three cells: one range
+                uint2 run = make_uint2(0, 0);
+                if (valid) {
                     const uint64_t b0 = mid - 1, b2 = mid + 1;
                     const uint64_t w0 = bm[b0 >> 6];
                     const uint64_t w2 = (b2 >> 6) == (b0 >> 6) ? w0 : bm[b2 >> 6];
@@ -381,42 +413,60 @@ __global__ void __launch_bounds__(256) probe_kernel(const float4* __restrict__ r
                     if (sh > 61) occ |= (uint32_t)(w2 << (64 - sh)) & 7;
                     if (occ) {
                         const uint32_t r0 = bmpre[b0 >> 6] + (uint32_t)__popcll(sh ? (w0 & ((1ull << sh) - 1)) : 0ull);
-                        rng[0] = make_uint2(runs[r0], runs[r0 + (uint32_t)__popc(occ)]);
-                        nrng = 1;
+                        run = make_uint2(runs[r0], runs[r0 + (uint32_t)__popc(occ)]);
                     }
-                } else {
-                    for (int dx = -1; dx <= 1; ++dx) rng[nrng++] = lookup(hkey, hval, hmask, mid + (uint64_t)(int64_t)dx);
                 }
-                auto visit = [&](const float4& y) {
-                    const uint32_t j = __builtin_bit_cast(uint32_t, y.w);
-                    if (self_join && j <= i) return;
-                    ++cand;
-                    if (d2_row(q, y, g.d) <= T) {
-                        if (MODE == 1 && base + n < (uint64_t)out_cap) out_keys[base + n] = (i << jbits) | j;
-                        ++n;
-                    }
-                };
-                for (int ri = 0; ri < nrng; ++ri) {
-                    const uint2 run = rng[ri];
-                    // eight rows in flight at a time: near a dense blob a
-                    // lane's range holds hundreds of rows, and one load per
-                    // iteration made those few waves the whole kernel's tail
-                    uint32_t p = run.x;
-                    for (; p + 8 <= run.y; p += 8) {
-                        float4 y[8];
-#pragma unroll
-                        for (int u = 0; u < 8; ++u) y[u] = srow[p + u];
-#pragma unroll
-                        for (int u = 0; u < 8; ++u) visit(y[u]);
-                    }
-                    for (; p < run.y; ++p) visit(srow[p]);
+                const uint32_t len = run.y - run.x;
+                if (__ballot(len > 32) == 0) {
+                    sweep(run);
+                    continue;
                 }
+                // A lane near a dense blob holds hundreds of rows while the
+                // rest hold a few: the wave sweeps the concatenation of its
+                // lanes' ranges together, 64 rows a step (row k belongs to
+                // the lane whose range prefix holds k), and counts each
+                // lane's hits in LDS.
+                uint32_t inc = len;
+                for (int o = 1; o < 64; o <<= 1) {
+                    const uint32_t v = __shfl_up(inc, o);
+                    if (lane >= o) inc += v;
+                }
+                const uint32_t excl = inc - len;
+                const uint32_t total = __shfl(inc, 63);
+                sCnt[wave][lane] = 0;
+                const uint64_t at = base + n;  // this lane's next output slot
+                for (uint32_t k0 = 0; k0 < total; k0 += 64) {
+                    const uint32_t k = k0 + (uint32_t)lane;
+                    int o = 0;  // the largest lane whose prefix is <= k: the owner of row k
+                    for (int step = 32; step > 0; step >>= 1)
+                        if (__shfl(excl, o + step) <= k) o += step;
+                    const uint32_t p = __shfl(run.x, o) + (k - __shfl(excl, o));
+                    const float qx = __shfl(qr.x, o), qy = __shfl(qr.y, o), qz = __shfl(qr.z, o);
+                    const uint32_t io = __builtin_bit_cast(uint32_t, __shfl(qr.w, o));
+                    const uint64_t ato = __shfl(at, o);
+                    if (k < total) {
+                        const float4 y = srow[p];
+                        const uint32_t j = __builtin_bit_cast(uint32_t, y.w);
+                        if (!(self_join && j <= io)) {
+                            ++cand;
+                            const float qo[3] = {qx, qy, qz};
+                            if (d2_row(qo, y, g.d) <= T) {
+                                const uint32_t h = atomicAdd(&sCnt[wave][o], 1u);
+                                if (MODE == 1 && ato + h < (uint64_t)out_cap) out_keys[ato + h] = ((uint64_t)io << jbits) | j;
+                            }
+                        }
+                    }
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                n += sCnt[wave][lane];
+                __builtin_amdgcn_wave_barrier();
             }
-        if (MODE == 0) cnt[t] = n;
+        if (MODE == 0 && valid) cnt[t] = n;
     }
     if (MODE == 0) {
         for (int s = 32; s > 0; s >>= 1) cand += __shfl_xor(cand, s);
-        if ((threadIdx.x & 63) == 0 && cand) atomicAdd((unsigned long long*)&stats[TJ_ST_GRID_CAND], (unsigned long long)cand);
+        if (lane == 0 && cand) atomicAdd((unsigned long long*)&stats[TJ_ST_GRID_CAND], (unsigned long long)cand);
     }
 }
 
