@@ -501,25 +501,15 @@ __device__ __forceinline__ void screen_panel(const PanelIO& io, const CandOut& c
             }
         }
     };
-    const int pre = nblk < NSTAGE - 1 ? nblk : NSTAGE - 1;
-    for (int b = 0; b < pre; ++b) issue(b, b);
-    int slot = 0, islot = pre == NSTAGE - 1 ? NSTAGE - 1 : 0;  // consumed / next issued ring slots
-
-    for (int blk = 0; blk < nblk; ++blk) {
-        static_assert(NSTAGE - 2 <= 4, "wait_blocks covers up to 4 younger blocks");
-        if (blk + NSTAGE - 2 < nblk) wait_vm_c<P * (NSTAGE - 2)>();  // steady state
-        else wait_blocks<P>(nblk - 1 - blk);
-        lds_barrier();
-        if (blk + NSTAGE - 1 < nblk) {
-            issue(blk + NSTAGE - 1, islot);
-            islot = islot + 1 == NSTAGE ? 0 : islot + 1;
-        }
-        const int cslot = slot;
-        slot = slot + 1 == NSTAGE ? 0 : slot + 1;
+    // ---- block body: the MFMA tile of block blk (ring slot cslot) for the
+    // wave's RB row blocks, the fast reject and the survivors
+    // (always inlined: an outlined copy would take its uniform arguments in
+    // VGPRs, which the scalar asm operands below cannot accept)
+    auto body = [&](int blk, int cslot) __attribute__((always_inline)) {
         const int64_t c0 = cb + (int64_t)blk * 32;
         const int64_t jcol = c0 + c32;
         const bool skip = SELF && (c0 + 31 <= rw0);
-        if (skip) continue;
+        if (skip) return;
         // v_j (+inf past the last column: never a survivor) and FP8 scales
         float nj = sNp[cslot * 64 + c32];
         int sbh = 127, sbl = 127;
@@ -649,6 +639,42 @@ __device__ __forceinline__ void screen_panel(const PanelIO& io, const CandOut& c
             };
             if (SELF && rw0 + rb * 32 + 31 >= c0) stage(std::true_type{});
             else stage(std::false_type{});
+        }
+    };
+
+    if constexpr (RB == 4 && NSTAGE >= 4) {
+        // Four-block bodies: one barrier per two blocks.  After the
+        // barrier both blocks have landed for every wave and the two
+        // slots consumed before it are free, so the next two blocks go
+        // there (two blocks of compute ahead of their use); the waves of
+        // a workgroup, whose survivor paths differ, wait for each other
+        // half as often.
+        const int pre = nblk < 2 ? nblk : 2;
+        for (int b = 0; b < pre; ++b) issue(b, b);
+        for (int blk0 = 0; blk0 < nblk; blk0 += 2) {
+            wait_vm_c<0>();
+            lds_barrier();
+            if (blk0 + 2 < nblk) issue(blk0 + 2, (blk0 + 2) % NSTAGE);
+            if (blk0 + 3 < nblk) issue(blk0 + 3, (blk0 + 3) % NSTAGE);
+            body(blk0, blk0 % NSTAGE);
+            if (blk0 + 1 < nblk) body(blk0 + 1, (blk0 + 1) % NSTAGE);
+        }
+    } else {
+        const int pre = nblk < NSTAGE - 1 ? nblk : NSTAGE - 1;
+        for (int b = 0; b < pre; ++b) issue(b, b);
+        int slot = 0, islot = pre == NSTAGE - 1 ? NSTAGE - 1 : 0;  // consumed / next issued ring slots
+        for (int blk = 0; blk < nblk; ++blk) {
+            static_assert(NSTAGE - 2 <= 4, "wait_blocks covers up to 4 younger blocks");
+            if (blk + NSTAGE - 2 < nblk) wait_vm_c<P * (NSTAGE - 2)>();  // steady state
+            else wait_blocks<P>(nblk - 1 - blk);
+            lds_barrier();
+            if (blk + NSTAGE - 1 < nblk) {
+                issue(blk + NSTAGE - 1, islot);
+                islot = islot + 1 == NSTAGE ? 0 : islot + 1;
+            }
+            const int cslot = slot;
+            slot = slot + 1 == NSTAGE ? 0 : slot + 1;
+            body(blk, cslot);
         }
     }
 }
