@@ -642,8 +642,8 @@ __device__ __forceinline__ void screen_panel(const PanelIO& io, const CandOut& c
         }
     };
 
-    if constexpr (RB == 4 && NSTAGE >= 4) {
-        // Four-block bodies: one barrier per two blocks.  After the
+    if constexpr (NSTAGE >= 4) {
+        // Rings of four slots or more: one barrier per two blocks.  After the
         // barrier both blocks have landed for every wave and the two
         // slots consumed before it are free, so the next two blocks go
         // there (two blocks of compute ahead of their use); the waves of
