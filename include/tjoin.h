@@ -178,8 +178,9 @@ int tj_params(const uint32_t* maxabs_bits, double T, int d, int precision, void*
               uint64_t* stats, void* stream);
 /* meta: 12 floats per row {nlo, nhi, H, E, L, t, v, e, w, a, 0, 0}: the
  * bracket of ||x||^2, the bounds of ||hi||, ||residual|| and ||lo||, the
- * screen's accumulator seed t, reject level v and accept terms w, a, and
- * the FP8 scale exponents e (join units; see tj_params). */
+ * screen's row level t (its MFMA accumulator seed, or added to the dot in
+ * binary32 by the four-block bodies), reject level v and accept terms w, a,
+ * and the FP8 scale exponents e (join units; see tj_params). */
 int tj_rowmeta(const float* x, int64_t n, int d, int precision, const void* params,
                void* q_hi, void* q_lo, float* meta, uint64_t* stats, void* stream);
 
