@@ -76,6 +76,25 @@ def test_grid_edge_sizes(tj, oracle):
     assert tj.join(X[:0], X, T=1.0, method="grid").shape == (0, 2)
 
 
+@pytest.mark.parametrize("d", [2, 3])
+def test_grid_dense_blobs_shared_sweep(tj, oracle, d):
+    """Tight blobs in a sparse field: waves whose lanes hold hundreds of
+    candidate rows next to lanes with none (the wave-shared sweep), for R x S
+    and the self-join's j > i filter."""
+    rng = np.random.default_rng(40 + d)
+    n = 20000
+    X = rng.random((n, d)).astype(np.float32)
+    centres = rng.random((6, d))
+    k = n // 3
+    X[:k] = (centres[rng.integers(0, 6, k)] + rng.normal(0, 0.002, (k, d))).astype(np.float32)
+    Y = rng.random((n, d)).astype(np.float32)
+    Y[: n // 4] = (centres[rng.integers(0, 6, n // 4)] + rng.normal(0, 0.002, (n // 4, d))).astype(np.float32)
+    eps = 0.004
+    st = check(tj, oracle, X, None, eps * eps, f"dense blobs self d{d}")
+    say("dense blobs", d, st["out"], st["grid_candidates"])
+    check(tj, oracle, Y, X, eps * eps, f"dense blobs rxs d{d}")
+
+
 def test_grid_blobs_medium(tj, oracle):
     """The grid workload's distribution (90% uniform, 10% in tight blobs) at 1/16 size."""
     from tensorjoin_variants_amd import workloads
