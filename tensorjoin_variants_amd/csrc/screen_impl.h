@@ -642,6 +642,11 @@ __device__ __forceinline__ void screen_panel(const PanelIO& io, const CandOut& c
         }
     };
 
+    // Waves of this workgroup may still be reading the ring from a previous
+    // sweep (a batched work item sweeps up to four column ranges back to
+    // back, with no barrier between them): every wave passes here before a
+    // slot is refilled.
+    lds_barrier();
     if constexpr (NSTAGE >= 4) {
         // Rings of four slots or more: one barrier per two blocks.  After the
         // barrier both blocks have landed for every wave and the two

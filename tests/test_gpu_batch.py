@@ -194,6 +194,23 @@ def test_plan_graph_replay(tj):
         assert np.array_equal(pairs[:n].cpu().numpy(), eager)
 
 
+def test_batch_uneven_full_sizes_pruned(tj, oracle):
+    """Production mode (fast accepts on) at the batched workload's full size
+    range (n up to 16384), tile pruning on: items sweep up to four column
+    ranges back to back, which share the workgroup's LDS ring (a refill
+    before every wave has left the previous range lost pairs)."""
+    from tensorjoin_variants_amd import workloads
+    ws = workloads.batched_uneven(joins=96, samples=4000, seed_offset=11)
+    sets = [w.R for w in ws]
+    Ts = np.array([w.T for w in ws])
+    say("batched_uneven full sizes", len(sets), sum(x.shape[0] for x in sets))
+    for rep in range(2):
+        got, st = tj.batched_join(sets, T=Ts, precision="fp16", return_stats=True, prune=True)
+        say("stats", rep, {k: st[k] for k in ("out", "candidates", "pruned", "violations")})
+        assert st["pruned"] > 0
+        check_all(oracle, sets, Ts, got, "batched_uneven full sizes")
+
+
 def separated(rng, n, d, k, gap):
     """k Gaussian clusters (unit spread) whose centres are ~gap apart."""
     c = rng.standard_normal((k, d)) * gap / np.sqrt(2.0 * d)
