@@ -104,38 +104,54 @@ __device__ inline int find_join(const int64_t* __restrict__ cs, int nj, int64_t 
 
 // ------------------------------------------------------------ scale scan
 // max |x| per join (binary32 bits) and the count of non-finite coordinates.
+// Each wave takes a contiguous range of scan units, reads each join's part
+// of it as one flat run of 16-byte loads, and flushes the join's maximum
+// with one atomic when it moves to the next join (units of one join are
+// consecutive; a strided unit order cost one atomic per unit).
 __global__ void __launch_bounds__(256) batch_scan_kernel(const float* const* __restrict__ xs,
                                                          const BatchJoin* __restrict__ joins,
                                                          const int64_t* __restrict__ cstart, int nj,
                                                          uint32_t* __restrict__ maxabs, uint64_t* __restrict__ stats) {
     const int lane = threadIdx.x & 63;
-    const int64_t w0 = (int64_t)blockIdx.x * 4 + tjscreen::uni((int)(threadIdx.x >> 6));
+    const int64_t wv = (int64_t)blockIdx.x * 4 + tjscreen::uni((int)(threadIdx.x >> 6));
     const int64_t nw = (int64_t)gridDim.x * 4;
     const int64_t units = cstart[nj];
+    const int64_t per = (units + nw - 1) / nw;
+    const int64_t u0 = wv * per;
+    const int64_t u1 = u0 + per < units ? u0 + per : units;
     uint32_t bad = 0;
-    for (int64_t c = w0; c < units; c += nw) {
-        const int t = find_join(cstart, nj, c);
+    for (int64_t c = u0; c < u1;) {
+        const int t = tjscreen::uni(find_join(cstart, nj, c));
         const BatchJoin J = joins[t];
-        const int gw = tjmeta::group_width(J.dp);
-        const int rpw = 64 / gw;
-        const int g = lane / gw, lg = lane % gw;
-        const float* x = xs[t];
+        const int64_t rows_per_unit = (int64_t)SCAN_SUB * (64 / tjmeta::group_width(J.dp));
+        const int64_t ce = u1 < cstart[t + 1] ? u1 : cstart[t + 1];
+        const int64_t r0 = (c - cstart[t]) * rows_per_unit;
+        int64_t r1 = (ce - cstart[t]) * rows_per_unit;
+        if (r1 > J.n) r1 = J.n;
+        const float* base = xs[t] + r0 * (int64_t)J.d;
+        const int64_t cnt = r1 > r0 ? (r1 - r0) * (int64_t)J.d : 0;
         uint32_t m = 0;
-        for (int sub = 0; sub < SCAN_SUB; ++sub) {
-            const int64_t local = (c - cstart[t]) * (SCAN_SUB * rpw) + sub * rpw + g;
-            if (local >= J.n) continue;
-            const float* xr = x + local * (int64_t)J.d;
-            for (int k = lg; k < J.d; k += gw) {
-                const uint32_t u = __builtin_bit_cast(uint32_t, xr[k]) & 0x7FFFFFFFu;
-                if (u >= 0x7F800000u) bad++;
-                else m = u > m ? u : m;
-            }
+        auto take = [&](float f) {
+            const uint32_t u = __builtin_bit_cast(uint32_t, f) & 0x7FFFFFFFu;
+            if (u >= 0x7F800000u) bad++;
+            else m = u > m ? u : m;
+        };
+        int64_t head = (int64_t)(((16 - ((uintptr_t)base & 15)) & 15) / 4);
+        if (head > cnt) head = cnt;
+        const int64_t nv = (cnt - head) / 4;
+        const float4* xv = (const float4*)(base + head);
+        for (int64_t k = lane; k < nv; k += 64) {
+            const float4 v = xv[k];
+            take(v.x); take(v.y); take(v.z); take(v.w);
         }
+        for (int64_t k = lane; k < head; k += 64) take(base[k]);
+        for (int64_t k = head + 4 * nv + lane; k < cnt; k += 64) take(base[k]);
         for (int o = 32; o > 0; o >>= 1) {
             const uint32_t om = __shfl_xor(m, o);
             m = om > m ? om : m;
         }
         if (lane == 0 && m) atomicMax(&maxabs[t], m);
+        c = ce;
     }
     for (int o = 32; o > 0; o >>= 1) bad += __shfl_xor(bad, o);
     if (lane == 0 && bad) atomicAdd((unsigned long long*)&stats[TJ_ST_INVALID], (unsigned long long)bad);
