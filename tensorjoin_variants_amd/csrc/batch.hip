@@ -340,7 +340,7 @@ __global__ void __launch_bounds__(256) batch_ball_kernel(const float* const* __r
                                                          const uint64_t* __restrict__ perm, int jbits,
                                                          float* __restrict__ balls) {
     __shared__ float spart[4][256];
-    __shared__ float sc[256];
+    __shared__ __attribute__((aligned(16))) float sc[256];
     __shared__ double sr[4];
     const int2 it = items[blockIdx.x];
     const BatchJoin J = joins[it.x];
@@ -376,14 +376,28 @@ __global__ void __launch_bounds__(256) batch_ball_kernel(const float* const* __r
     const int g16 = lane >> 4, l16 = lane & 15;
     float rmax = 0.0f;
     bool bad = false;
+    // 16-byte loads when every row starts on a 16-byte boundary
+    const bool vec = (d & 3) == 0 && ((uintptr_t)x & 15) == 0;
     for (int64_t rb = r0 + wave * 4; rb < r1; rb += 16) {
         const int64_t r = rb + g16;
         const bool act = r < r1;
         const float* xr = x + perm_of(perm, J.gbase + (act ? r : r0), jmask) * d;
         float a = 0.0f;
-        for (int k = l16; k < d; k += 16) {
-            const float u = xr[k] - sc[k];
-            a += u * u;
+        if (vec) {
+            for (int k = 4 * l16; k < d; k += 64) {
+                const float4 v = *(const float4*)(xr + k);
+                const float4 cc = *(const float4*)(sc + k);
+                const float u0 = v.x - cc.x, u1 = v.y - cc.y, u2 = v.z - cc.z, u3 = v.w - cc.w;
+                a += u0 * u0;
+                a += u1 * u1;
+                a += u2 * u2;
+                a += u3 * u3;
+            }
+        } else {
+            for (int k = l16; k < d; k += 16) {
+                const float u = xr[k] - sc[k];
+                a += u * u;
+            }
         }
         a += __shfl_xor(a, 8);
         a += __shfl_xor(a, 4);
