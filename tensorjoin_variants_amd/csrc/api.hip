@@ -378,8 +378,7 @@ static int join_enqueue(const JoinCall& c, hipStream_t s) {
         }
         TJ_MARK(3);
     } else if (p->method == TJ_METHOD_GRID) {
-        if ((rc = launch_scale_scan(R, p->nR, p->d, maxabs, stats, s))) return rc;
-        if (!p->self_join && (rc = launch_scale_scan(S, p->nS, p->d, maxabs, stats, s))) return rc;
+        // (the grid's bounds pass counts the non-finite coordinates)
         TJ_MARK(1);
         if ((rc = launch_grid_join(R, p->nR, S, p->nS, p->d, T, p->self_join, ws + p->off_grid, p->glay, p->out_cap,
                                    pairs, out_format, stats, s)))

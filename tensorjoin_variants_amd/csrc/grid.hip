@@ -53,18 +53,35 @@ __device__ inline float ord2f(uint32_t o) {
     return __builtin_bit_cast(float, u);
 }
 
-// bounds[0..2] = ordered min, bounds[3..5] = ordered max
-__global__ void __launch_bounds__(256) bounds_kernel(const float* __restrict__ x, int64_t n, int d, uint32_t* bounds) {
+// bounds[0..2] = ordered min, bounds[3..5] = ordered max of the finite
+// coordinates; the non-finite ones are counted in stats[TJ_ST_INVALID] (the
+// join is then rejected).  Three-dimensional points go four at a time as
+// three 16-byte loads.
+__global__ void __launch_bounds__(256) bounds_kernel(const float* __restrict__ x, int64_t n, int d, uint32_t* bounds,
+                                                     uint64_t* __restrict__ stats) {
     uint32_t mn[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, mx[3] = {0, 0, 0};
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
-        for (int k = 0; k < d; ++k) {
-            const uint32_t o = f2ord(x[t * d + k]);
-            mn[k] = o < mn[k] ? o : mn[k];
-            mx[k] = o > mx[k] ? o : mx[k];
-        }
+    uint32_t bad = 0;
+    auto take = [&](int k, float f) {
+        if ((__builtin_bit_cast(uint32_t, f) & 0x7FFFFFFFu) >= 0x7F800000u) { bad++; return; }
+        const uint32_t o = f2ord(f);
+        mn[k] = o < mn[k] ? o : mn[k];
+        mx[k] = o > mx[k] ? o : mx[k];
+    };
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t n4 = (d == 3 && ((uintptr_t)x & 15) == 0) ? n / 4 : 0;
+    const float4* xv = (const float4*)x;
+    for (int64_t g = tid; g < n4; g += stride) {
+        const float4 a = xv[3 * g], b = xv[3 * g + 1], c = xv[3 * g + 2];
+        take(0, a.x); take(1, a.y); take(2, a.z);
+        take(0, a.w); take(1, b.x); take(2, b.y);
+        take(0, b.z); take(1, b.w); take(2, c.x);
+        take(0, c.y); take(1, c.z); take(2, c.w);
+    }
+    for (int64_t t = 4 * n4 + tid; t < n; t += stride)
+        for (int k = 0; k < d; ++k) take(k, x[t * d + k]);
     // wave, then block reduction; one atomic per block and bound (per-wave
     // atomics on six addresses serialised into most of the kernel's time)
-    __shared__ uint32_t smn[3][4], smx[3][4];
+    __shared__ uint32_t smn[3][4], smx[3][4], sbad[4];
     const int wave = threadIdx.x >> 6;
     for (int k = 0; k < d; ++k) {
         for (int s = 32; s > 0; s >>= 1) {
@@ -74,8 +91,13 @@ __global__ void __launch_bounds__(256) bounds_kernel(const float* __restrict__ x
         }
         if ((threadIdx.x & 63) == 0) { smn[k][wave] = mn[k]; smx[k][wave] = mx[k]; }
     }
+    for (int s = 32; s > 0; s >>= 1) bad += __shfl_xor(bad, s);
+    if ((threadIdx.x & 63) == 0) sbad[wave] = bad;
     __syncthreads();
     if (threadIdx.x == 0) {
+        uint32_t nb = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) nb += sbad[w];
+        if (nb) atomicAdd((unsigned long long*)&stats[TJ_ST_INVALID], (unsigned long long)nb);
         for (int k = 0; k < d; ++k) {
             uint32_t a = smn[k][0], b = smx[k][0];
             for (int w = 1; w < (int)(blockDim.x >> 6); ++w) {
@@ -606,8 +628,9 @@ int launch_grid_join(const float* R, int64_t nR, const float* S, int64_t nS, int
     const unsigned gb = (unsigned)(num_cus() * 4);
     hipLaunchKernelGGL(init_bounds_kernel, dim3(1), dim3(64), 0, s, bounds);
     const unsigned bb = (unsigned)(num_cus() * 2);
-    hipLaunchKernelGGL(bounds_kernel, dim3(blocks_for(nR, bb)), dim3(256), 0, s, R, nR, d, bounds);
-    if (!self_join) hipLaunchKernelGGL(bounds_kernel, dim3(blocks_for(nS, bb)), dim3(256), 0, s, S, nS, d, bounds);
+    hipLaunchKernelGGL(bounds_kernel, dim3(blocks_for((nR + 3) / 4, bb)), dim3(256), 0, s, R, nR, d, bounds, stats);
+    if (!self_join)
+        hipLaunchKernelGGL(bounds_kernel, dim3(blocks_for((nS + 3) / 4, bb)), dim3(256), 0, s, S, nS, d, bounds, stats);
     hipLaunchKernelGGL(grid_params_kernel, dim3(1), dim3(64), 0, s, bounds, T, d, L.idxbits, L.bm_bits, gp);
     uint64_t* bm = (uint64_t*)(ws + L.bitmap);
     hipLaunchKernelGGL(clear_bitmap_kernel, dim3(blocks_for((int64_t)(L.bm_bits / 64), gb)), dim3(256), 0, s, gp, bm);

@@ -95,6 +95,22 @@ def test_grid_dense_blobs_shared_sweep(tj, oracle, d):
     check(tj, oracle, Y, X, eps * eps, f"dense blobs rxs d{d}")
 
 
+def test_grid_non_finite_rejected(tj):
+    """The grid's bounds pass counts non-finite coordinates (in its 4-point
+    vector body and in the tail) and the join is rejected."""
+    rng = np.random.default_rng(50)
+    X = rng.random((1001, 3)).astype(np.float32)
+    Y = rng.random((700, 3)).astype(np.float32)
+    for name, A, B, r, c, v in [("self nan body", X, None, 17, 1, np.nan),
+                                ("self inf tail", X, None, 1000, 2, np.inf),
+                                ("rxs -inf in S", X, Y, 5, 0, -np.inf)]:
+        A2, B2 = A.copy(), None if B is None else B.copy()
+        (A2 if B is None else B2)[r, c] = v
+        say("grid non-finite", name)
+        with pytest.raises(ValueError):
+            tj.join(A2, B2, T=0.01, method="grid")
+
+
 def test_grid_blobs_medium(tj, oracle):
     """The grid workload's distribution (90% uniform, 10% in tight blobs) at 1/16 size."""
     from tensorjoin_variants_amd import workloads
