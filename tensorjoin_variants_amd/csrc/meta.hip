@@ -67,13 +67,16 @@ __global__ void __launch_bounds__(256) rowmeta_kernel(const float* __restrict__ 
                                                       int dp, const JoinParams* __restrict__ pp,
                                                       uint8_t* __restrict__ qhi, uint8_t* __restrict__ qlo,
                                                       RowMeta* __restrict__ meta) {
-    // 64 rows per wave, 4 waves per block
+    // ROWS = 16 steps of 64 / GW rows per wave, 4 waves per block: a wave's
+    // steps run one after another (each waits for its rows), so short waves
+    // and many of them hide the latency (64 rows a wave ran 3-4x longer)
+    constexpr int ROWS = 16 * (64 / GW);
     const int lane = threadIdx.x & 63;
-    const int64_t r0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 64;
+    const int64_t r0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * ROWS;
     if (r0 >= n) return;  // wave-uniform
     const JoinParams P = *pp;
-    tjmeta::rowmeta_wave<PREC, GW>([=](int64_t r) { return x + r * (int64_t)d; }, n, r0, d, dp, P, lane, qhi, qlo,
-                                   meta, 0);
+    auto src = [=](int64_t r) { return x + r * (int64_t)d; };
+    tjmeta::rowmeta_wave<PREC, GW, decltype(src), ROWS>(src, n, r0, d, dp, P, lane, qhi, qlo, meta, 0);
 }
 
 __global__ void fill_u64_kernel(uint64_t* p, int64_t n, uint64_t v) {
@@ -106,7 +109,7 @@ int launch_rowmeta(const float* x, int64_t n, int d, int dp, int prec, const Joi
     (void)stats;
     if (n == 0) return TJ_OK;
     const int gw = tjmeta::meta_width(dp);
-    const int64_t rows_per_block = 4 * 64;
+    const int64_t rows_per_block = 4 * 16 * (64 / gw);  // the kernel's 4 waves x ROWS
     dim3 grid((unsigned)((n + rows_per_block - 1) / rows_per_block)), block(256);
     uint8_t* h = (uint8_t*)qhi;
     uint8_t* l = (uint8_t*)qlo;

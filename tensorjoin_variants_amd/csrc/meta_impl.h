@@ -226,20 +226,21 @@ __device__ inline RowMeta rowmeta_terms(const RowSums& S, int dp, const JoinPara
     return m;
 }
 
-// One wave, 64 consecutive rows [r0, r0 + 64) of a range of n rows: the
+// One wave, ROWS consecutive rows [r0, r0 + ROWS) of a range of n rows: the
 // rows are quantized GW lanes each (64 / GW at a time), lane L collects row
 // r0 + L's sums, and then every lane derives its own row's terms -- the
 // binary64 tail runs on 64 lanes at once instead of one lane per group.
 // src(r): the input row r of the range; row r's codes and metadata go to
 // index base + r.
-template <int PREC, int GW, class Src>
+template <int PREC, int GW, class Src, int ROWS = 64>
 __device__ inline void rowmeta_wave(Src src, int64_t n, int64_t r0, int d, int dp, const JoinParams& P, int lane,
                                     uint8_t* __restrict__ qhi, uint8_t* __restrict__ qlo, RowMeta* __restrict__ meta,
                                     int64_t base) {
     constexpr int RPW = 64 / GW;
+    static_assert(ROWS % RPW == 0 && ROWS <= 64, "whole steps, one lane per row");
     const int g = lane / GW, lg = lane % GW;
     RowSums mine{};
-    for (int step = 0; step < GW; ++step) {
+    for (int step = 0; step < ROWS / RPW; ++step) {
         const int64_t first = r0 + (int64_t)step * RPW;
         if (first >= n) break;  // uniform
         const int64_t r = first + g;
@@ -250,7 +251,7 @@ __device__ inline void rowmeta_wave(Src src, int64_t n, int64_t r0, int d, int d
         if (lane / RPW == step) mine = t;
     }
     const int64_t my = r0 + lane;
-    if (my < n) meta[base + my] = rowmeta_terms<PREC>(mine, dp, P);
+    if (lane < ROWS && my < n) meta[base + my] = rowmeta_terms<PREC>(mine, dp, P);
 }
 
 }  // namespace tjmeta
