@@ -92,6 +92,9 @@ int rows_per_wg(int combo) {
 // int64 [joins + 1]).
 constexpr int SCAN_SUB = 8;
 
+// Rows of one row-metadata unit (one wave): 16 steps of 64 / width rows.
+__host__ __device__ inline int64_t meta_unit_rows(int dp) { return 16 * (64 / tjmeta::meta_width(dp)); }
+
 __device__ inline int find_join(const int64_t* __restrict__ cs, int nj, int64_t c) {
     int lo = 0, hi = nj;  // cs[lo] <= c < cs[hi]
     while (hi - lo > 1) {
@@ -529,7 +532,7 @@ __global__ void __launch_bounds__(256) batch_rowmeta_kernel(const float* const* 
     uint8_t* q = ws + G.q_off;
     RowMeta* meta = (RowMeta*)(ws + G.meta_off);
     const int gw = tjmeta::meta_width(J.dp);
-    const int64_t r0 = (c - cstart[t]) * 64;
+    const int64_t r0 = (c - cstart[t]) * meta_unit_rows(J.dp);
     const float* x = xs[t];
     const uint64_t jm = (1ull << jbits) - 1;
     const int64_t gb = J.gbase;
@@ -537,16 +540,17 @@ __global__ void __launch_bounds__(256) batch_rowmeta_kernel(const float* const* 
     // the caller's row behind position r (rows are read through the spatial
     // order when there is one)
     auto src = [=](int64_t r) { return x + (perm ? perm_of(perm, gb + r, jm) : r) * (int64_t)d; };
+    using S = decltype(src);
     if (J.prec == P_FP8) {
-        if (gw == 16) tjmeta::rowmeta_wave<P_FP8, 16>(src, J.n, r0, d, J.dp, P, lane, q, q, meta, J.grow);
-        else if (gw == 32) tjmeta::rowmeta_wave<P_FP8, 32>(src, J.n, r0, d, J.dp, P, lane, q, q, meta, J.grow);
-        else tjmeta::rowmeta_wave<P_FP8, 64>(src, J.n, r0, d, J.dp, P, lane, q, q, meta, J.grow);
+        if (gw == 16) tjmeta::rowmeta_wave<P_FP8, 16, S, 64>(src, J.n, r0, d, J.dp, P, lane, q, q, meta, J.grow);
+        else if (gw == 32) tjmeta::rowmeta_wave<P_FP8, 32, S, 32>(src, J.n, r0, d, J.dp, P, lane, q, q, meta, J.grow);
+        else tjmeta::rowmeta_wave<P_FP8, 64, S, 16>(src, J.n, r0, d, J.dp, P, lane, q, q, meta, J.grow);
     } else if (gw == 16) {
-        tjmeta::rowmeta_wave<P_FP16, 16>(src, J.n, r0, d, J.dp, P, lane, q, q, meta, J.grow);
+        tjmeta::rowmeta_wave<P_FP16, 16, S, 64>(src, J.n, r0, d, J.dp, P, lane, q, q, meta, J.grow);
     } else if (gw == 32) {
-        tjmeta::rowmeta_wave<P_FP16, 32>(src, J.n, r0, d, J.dp, P, lane, q, q, meta, J.grow);
+        tjmeta::rowmeta_wave<P_FP16, 32, S, 32>(src, J.n, r0, d, J.dp, P, lane, q, q, meta, J.grow);
     } else {
-        tjmeta::rowmeta_wave<P_FP16, 64>(src, J.n, r0, d, J.dp, P, lane, q, q, meta, J.grow);
+        tjmeta::rowmeta_wave<P_FP16, 64, S, 16>(src, J.n, r0, d, J.dp, P, lane, q, q, meta, J.grow);
     }
 }
 
@@ -1031,7 +1035,8 @@ int tj_batch_create(tj_batch** out, int joins, const int64_t* n, const int* d, c
     std::vector<int64_t> cmeta(joins + 1, 0), cscan(joins + 1, 0);
     for (int t = 0; t < joins; ++t) {
         const int rpw = 64 / tjmeta::group_width(J[t].dp);
-        cmeta[t + 1] = cmeta[t] + (J[t].n + 63) / 64;
+        const int64_t mrows = meta_unit_rows(J[t].dp);
+        cmeta[t + 1] = cmeta[t] + (J[t].n + mrows - 1) / mrows;
         cscan[t + 1] = cscan[t] + (J[t].n + (int64_t)SCAN_SUB * rpw - 1) / ((int64_t)SCAN_SUB * rpw);
     }
     b->units_meta = cmeta[joins];
