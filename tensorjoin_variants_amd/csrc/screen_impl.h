@@ -684,15 +684,15 @@ __device__ __forceinline__ void screen_panel(const PanelIO& io, const CandOut& c
     }
 }
 
-// Workgroups per CU the compiler should plan registers for: two; one when
-// the stationary fragments of a one- or two-block wave take 128 VGPRs (up to
-// 512 each); three for single-tile waves with at most 64 fragment VGPRs (up
-// to 168 each).  Four-block waves are unseeded and stay at two.
+// Workgroups per CU the compiler should plan registers for: two (up to 256
+// VGPRs each; bodies with 128 fragment VGPRs fit too, without spills, and
+// their 77 KiB of LDS twice fits the CU's 160 KiB); three for single-tile
+// waves with at most 64 fragment VGPRs (up to 168 each).
 template <int PREC, int DP, int RB>
 constexpr int min_blocks() {
     using C = Cfg<PREC>;
     constexpr int frag = C::NT * RB * (DP / C::KSTEP) * (C::CH / 4);
-    return RB == 4 ? 2 : (frag >= 128 ? 1 : (RB == 1 && frag <= 64 ? 3 : 2));
+    return (RB == 1 && frag <= 64) ? 3 : 2;
 }
 
 template <int PREC, int DP, int RB, bool SELF>
