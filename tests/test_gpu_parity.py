@@ -307,3 +307,20 @@ def test_fp32_filter_near_ties(tj, oracle, d):
         assert np.array_equal(got, want), (d, mode, got.shape, want.shape)
         assert st["violations"] == 0, st
         assert st["fp32_accept"] + st["fp32_reject"] > 0 and st["fp64_refined"] > 0, st
+
+
+@pytest.mark.parametrize("prec,d", [("fp16", 128), ("bf16", 96), ("fp16", 64)])
+def test_four_block_screen_uneven_norms(tj, oracle, prec, d):
+    """R x S at the widths of the four-block (unseeded) screen bodies with
+    heavy-tailed rows (Student-t, nu = 3) scaled by factors spanning 2^-6 to
+    2^6: the row levels t of a lane's 16 rows differ widely, so the per-lane
+    level lets many tiles through that only the rows' own levels reject."""
+    rng = np.random.default_rng(400 + d)
+
+    def rows(n):
+        z = rng.standard_normal((n, d)) / np.sqrt(rng.chisquare(3.0, (n, 1)) / 3.0)
+        return (z * np.exp2(rng.uniform(-6, 6, (n, 1)))).astype(np.float32)
+
+    R, S = rows(2600), rows(2100)
+    T = sample_T(oracle, R, S, 0.002)
+    run_check(tj, oracle, R, S, T, prec, name="four-block uneven norms")
