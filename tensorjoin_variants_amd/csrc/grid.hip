@@ -21,10 +21,11 @@
 //    rows too; a self-join reuses S's), so the 64 lanes of a wave visit the
 //    same neighbour cells and the same S rows, and read their queries,
 //    counts and offsets contiguously;
-//  * output: a count pass, an exclusive scan, a write pass of pair keys at
-//    exact per-query offsets, and one radix sort of the keys over all their
-//    bits (a per-row insertion sort was quadratic in the longest row: the
-//    dense blobs made it the costliest kernel);
+//  * output: one probe pass stages each wave's pair keys in LDS (an LDS
+//    atomic per pair, one global atomic per flush of up to 1024 keys) and
+//    one radix sort of the keys over all their bits orders them (a count
+//    pass, scan and exact-offset write pass probed everything twice; a
+//    per-row insertion sort was quadratic in the longest row);
 //  * the predicate is the sequential binary64 difference form of the CPU
 //    oracle (d2 <= T; GDS used sqrt(d2) <= eps, GDS/kernel.cu:368-371).
 // The cell side is sqrt(T) * (1 + 2^-10) (or larger when the cell ids would
@@ -361,24 +362,55 @@ __device__ inline double d2_row(const float* q, const float4& y, int d) {
     return s;
 }
 
-// One query per lane, queries in cell order.  mode 0: count into cnt[i];
-// mode 1: write the pair keys (i << jbits) | j at out_keys[off[t] + k].
-// Query t is row t of R in cell order (rrow: x, y, z, original index).
-template <int MODE>
+// One query per lane, queries in cell order (rrow: x, y, z, original index).
+// Each pair found is appended to this wave's LDS buffer (an LDS atomic per
+// pair); at the end of a neighbour row, a buffer at least half full is
+// flushed to out_keys with one atomic on the output count (a pair that
+// finds the buffer full goes to out_keys directly).  Entries past out_cap
+// are counted, not written; the key sort that follows orders the keys.
+constexpr int PCAP = 1024;  // staged keys per wave
 __global__ void __launch_bounds__(256) probe_kernel(const float4* __restrict__ rrow, int64_t nR, const GridParams* __restrict__ gp,
                                                     const float4* __restrict__ srow,
                                                     const uint64_t* __restrict__ hkey, const uint2* __restrict__ hval,
                                                     uint64_t hmask, const uint64_t* __restrict__ bm,
                                                     const uint32_t* __restrict__ bmpre, const uint32_t* __restrict__ runs,
-                                                    double T, int self_join, uint32_t* __restrict__ cnt,
-                                                    const uint64_t* __restrict__ off, uint64_t* __restrict__ out_keys,
+                                                    double T, int self_join, uint64_t* __restrict__ out_keys,
                                                     int jbits, int64_t out_cap, uint64_t* __restrict__ stats) {
     const GridParams g = *gp;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    __shared__ uint32_t sCnt[4][64];  // per-query hits of a wave's shared sweep
+    __shared__ uint64_t sBuf[4][PCAP];
+    __shared__ uint32_t sN[4];
+    if (lane == 0) sN[wave] = 0;
+    __builtin_amdgcn_wave_barrier();
     uint64_t cand = 0;
+    auto emit = [&](uint64_t key) {
+        const uint32_t pos = atomicAdd(&sN[wave], 1u);
+        if (pos < (uint32_t)PCAP) {
+            sBuf[wave][pos] = key;
+        } else {
+            const uint64_t at = atomicAdd((unsigned long long*)&stats[TJ_ST_OUT], 1ull);
+            if (at < (uint64_t)out_cap) out_keys[at] = key;
+        }
+    };
+    // all lanes: write the staged keys out and empty the buffer
+    auto flush = [&]() {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        uint32_t n = __builtin_amdgcn_readfirstlane(sN[wave]);
+        if (n > (uint32_t)PCAP) n = PCAP;
+        if (n) {
+            uint64_t base = 0;
+            if (lane == 0) base = atomicAdd((unsigned long long*)&stats[TJ_ST_OUT], (unsigned long long)n);
+            base = __shfl(base, 0);
+            for (uint32_t k = lane; k < n; k += 64)
+                if (base + k < (uint64_t)out_cap) out_keys[base + k] = sBuf[wave][k];
+        }
+        __builtin_amdgcn_wave_barrier();
+        if (lane == 0) sN[wave] = 0;
+        __builtin_amdgcn_wave_barrier();
+    };
     // whole waves step together (tb is the wave's first query), so the
-    // shared sweep below always has all 64 lanes
+    // shared sweep and the flushes below always have all 64 lanes
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t tb = (int64_t)blockIdx.x * blockDim.x + wave * 64; tb < nR; tb += stride) {
         const int64_t t = tb + lane;
@@ -388,17 +420,11 @@ __global__ void __launch_bounds__(256) probe_kernel(const float4* __restrict__ r
         const float q[3] = {qr.x, qr.y, qr.z};
         int64_t c[3];
         cell_of(q, g, c);
-        uint32_t n = 0;
-        uint64_t base = 0;
-        if (MODE == 1 && valid) base = off[t];
         auto visit = [&](const float4& y) {
             const uint32_t j = __builtin_bit_cast(uint32_t, y.w);
             if (self_join && j <= i) return;
             ++cand;
-            if (d2_row(q, y, g.d) <= T) {
-                if (MODE == 1 && base + n < (uint64_t)out_cap) out_keys[base + n] = (i << jbits) | j;
-                ++n;
-            }
+            if (d2_row(q, y, g.d) <= T) emit((i << jbits) | j);
         };
         auto sweep = [&](uint2 run) {
             // eight rows in flight at a time
@@ -422,138 +448,64 @@ __global__ void __launch_bounds__(256) probe_kernel(const float4* __restrict__ r
                     // one hash run per cell
                     if (valid)
                         for (int dx = -1; dx <= 1; ++dx) sweep(lookup(hkey, hval, hmask, mid + (uint64_t)(int64_t)dx));
-                    continue;
-                }
-                // the S rows of the row's three cells: one range
-                uint2 run = make_uint2(0, 0);
-                if (valid) {
-                    const uint64_t b0 = mid - 1, b2 = mid + 1;
-                    const uint64_t w0 = bm[b0 >> 6];
-                    const uint64_t w2 = (b2 >> 6) == (b0 >> 6) ? w0 : bm[b2 >> 6];
-                    const uint32_t sh = (uint32_t)(b0 & 63);
-                    uint32_t occ = (uint32_t)((w0 >> sh) & 7);
-                    if (sh > 61) occ |= (uint32_t)(w2 << (64 - sh)) & 7;
-                    if (occ) {
-                        const uint32_t r0 = bmpre[b0 >> 6] + (uint32_t)__popcll(sh ? (w0 & ((1ull << sh) - 1)) : 0ull);
-                        run = make_uint2(runs[r0], runs[r0 + (uint32_t)__popc(occ)]);
+                } else {
+                    // the S rows of the row's three cells: one range
+                    uint2 run = make_uint2(0, 0);
+                    if (valid) {
+                        const uint64_t b0 = mid - 1, b2 = mid + 1;
+                        const uint64_t w0 = bm[b0 >> 6];
+                        const uint64_t w2 = (b2 >> 6) == (b0 >> 6) ? w0 : bm[b2 >> 6];
+                        const uint32_t sh = (uint32_t)(b0 & 63);
+                        uint32_t occ = (uint32_t)((w0 >> sh) & 7);
+                        if (sh > 61) occ |= (uint32_t)(w2 << (64 - sh)) & 7;
+                        if (occ) {
+                            const uint32_t r0 =
+                                bmpre[b0 >> 6] + (uint32_t)__popcll(sh ? (w0 & ((1ull << sh) - 1)) : 0ull);
+                            run = make_uint2(runs[r0], runs[r0 + (uint32_t)__popc(occ)]);
+                        }
                     }
-                }
-                const uint32_t len = run.y - run.x;
-                if (__ballot(len > 32) == 0) {
-                    sweep(run);
-                    continue;
-                }
-                // A lane near a dense blob holds hundreds of rows while the
-                // rest hold a few: the wave sweeps the concatenation of its
-                // lanes' ranges together, 64 rows a step (row k belongs to
-                // the lane whose range prefix holds k), and counts each
-                // lane's hits in LDS.
-                uint32_t inc = len;
-                for (int o = 1; o < 64; o <<= 1) {
-                    const uint32_t v = __shfl_up(inc, o);
-                    if (lane >= o) inc += v;
-                }
-                const uint32_t excl = inc - len;
-                const uint32_t total = __shfl(inc, 63);
-                sCnt[wave][lane] = 0;
-                const uint64_t at = base + n;  // this lane's next output slot
-                for (uint32_t k0 = 0; k0 < total; k0 += 64) {
-                    const uint32_t k = k0 + (uint32_t)lane;
-                    int o = 0;  // the largest lane whose prefix is <= k: the owner of row k
-                    for (int step = 32; step > 0; step >>= 1)
-                        if (__shfl(excl, o + step) <= k) o += step;
-                    const uint32_t p = __shfl(run.x, o) + (k - __shfl(excl, o));
-                    const float qx = __shfl(qr.x, o), qy = __shfl(qr.y, o), qz = __shfl(qr.z, o);
-                    const uint32_t io = __builtin_bit_cast(uint32_t, __shfl(qr.w, o));
-                    const uint64_t ato = __shfl(at, o);
-                    if (k < total) {
-                        const float4 y = srow[p];
-                        const uint32_t j = __builtin_bit_cast(uint32_t, y.w);
-                        if (!(self_join && j <= io)) {
-                            ++cand;
-                            const float qo[3] = {qx, qy, qz};
-                            if (d2_row(qo, y, g.d) <= T) {
-                                const uint32_t h = atomicAdd(&sCnt[wave][o], 1u);
-                                if (MODE == 1 && ato + h < (uint64_t)out_cap) out_keys[ato + h] = ((uint64_t)io << jbits) | j;
+                    const uint32_t len = run.y - run.x;
+                    if (__ballot(len > 32) == 0) {
+                        sweep(run);
+                    } else {
+                        // A lane near a dense blob holds hundreds of rows while
+                        // the rest hold a few: the wave sweeps the concatenation
+                        // of its lanes' ranges together, 64 rows a step (row k
+                        // belongs to the lane whose range prefix holds k).
+                        uint32_t inc = len;
+                        for (int o = 1; o < 64; o <<= 1) {
+                            const uint32_t v = __shfl_up(inc, o);
+                            if (lane >= o) inc += v;
+                        }
+                        const uint32_t excl = inc - len;
+                        const uint32_t total = __shfl(inc, 63);
+                        for (uint32_t k0 = 0; k0 < total; k0 += 64) {
+                            const uint32_t k = k0 + (uint32_t)lane;
+                            int o = 0;  // the largest lane whose prefix is <= k: the owner of row k
+                            for (int step = 32; step > 0; step >>= 1)
+                                if (__shfl(excl, o + step) <= k) o += step;
+                            const uint32_t p = __shfl(run.x, o) + (k - __shfl(excl, o));
+                            const float qx = __shfl(qr.x, o), qy = __shfl(qr.y, o), qz = __shfl(qr.z, o);
+                            const uint32_t io = __builtin_bit_cast(uint32_t, __shfl(qr.w, o));
+                            if (k < total) {
+                                const float4 y = srow[p];
+                                const uint32_t j = __builtin_bit_cast(uint32_t, y.w);
+                                if (!(self_join && j <= io)) {
+                                    ++cand;
+                                    const float qo[3] = {qx, qy, qz};
+                                    if (d2_row(qo, y, g.d) <= T) emit(((uint64_t)io << jbits) | j);
+                                }
                             }
                         }
                     }
                 }
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
-                n += sCnt[wave][lane];
-                __builtin_amdgcn_wave_barrier();
+                if (__builtin_amdgcn_readfirstlane(sN[wave]) >= (uint32_t)(PCAP / 2)) flush();
             }
-        if (MODE == 0 && valid) cnt[t] = n;
     }
-    if (MODE == 0) {
-        for (int s = 32; s > 0; s >>= 1) cand += __shfl_xor(cand, s);
-        if (lane == 0 && cand) atomicAdd((unsigned long long*)&stats[TJ_ST_GRID_CAND], (unsigned long long)cand);
-    }
-}
-
-// ---- exclusive scan of n uint32 counts into uint64 offsets (total -> stats[OUT])
-constexpr int SCAN_B = 1024;
-__global__ void __launch_bounds__(256) scan_block_sums(const uint32_t* __restrict__ in, int64_t n, uint64_t* __restrict__ bsum) {
-    __shared__ uint64_t part[256];
-    const int64_t b0 = (int64_t)blockIdx.x * SCAN_B;
-    uint64_t s = 0;
-    for (int k = threadIdx.x; k < SCAN_B; k += 256) if (b0 + k < n) s += in[b0 + k];
-    part[threadIdx.x] = s;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) part[threadIdx.x] += part[threadIdx.x + o];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) bsum[blockIdx.x] = part[0];
-}
-__global__ void __launch_bounds__(1024) scan_sums(uint64_t* bsum, int64_t nb, uint64_t* stats) {
-    __shared__ uint64_t part[1024];
-    const int64_t per = (nb + 1023) / 1024;
-    const int64_t a = threadIdx.x * per, b = a + per < nb ? a + per : nb;
-    uint64_t s = 0;
-    for (int64_t t = a; t < b; ++t) s += bsum[t];
-    part[threadIdx.x] = s;
-    __syncthreads();
-    for (int o = 1; o < 1024; o <<= 1) {
-        const uint64_t v = threadIdx.x >= (unsigned)o ? part[threadIdx.x - o] : 0;
-        __syncthreads();
-        part[threadIdx.x] += v;
-        __syncthreads();
-    }
-    uint64_t run = threadIdx.x ? part[threadIdx.x - 1] : 0;
-    for (int64_t t = a; t < b; ++t) {
-        const uint64_t v = bsum[t];
-        bsum[t] = run;
-        run += v;
-    }
-    if (threadIdx.x == 1023) stats[TJ_ST_OUT] = part[1023];
-}
-__global__ void __launch_bounds__(256) scan_apply(const uint32_t* __restrict__ in, int64_t n, const uint64_t* __restrict__ bsum,
-                                                  uint64_t* __restrict__ out) {
-    __shared__ uint64_t part[256];
-    const int64_t b0 = (int64_t)blockIdx.x * SCAN_B;
-    // each thread owns 4 consecutive entries
-    uint64_t v[4], s = 0;
-    for (int u = 0; u < 4; ++u) {
-        const int64_t t = b0 + threadIdx.x * 4 + u;
-        v[u] = t < n ? in[t] : 0;
-        s += v[u];
-    }
-    part[threadIdx.x] = s;
-    __syncthreads();
-    for (int o = 1; o < 256; o <<= 1) {
-        const uint64_t w = threadIdx.x >= (unsigned)o ? part[threadIdx.x - o] : 0;
-        __syncthreads();
-        part[threadIdx.x] += w;
-        __syncthreads();
-    }
-    uint64_t run = bsum[blockIdx.x] + (threadIdx.x ? part[threadIdx.x - 1] : 0);
-    for (int u = 0; u < 4; ++u) {
-        const int64_t t = b0 + threadIdx.x * 4 + u;
-        if (t < n) out[t] = run;
-        run += v[u];
-    }
+    flush();
+    for (int s = 32; s > 0; s >>= 1) cand += __shfl_xor(cand, s);
+    if (lane == 0 && cand) atomicAdd((unsigned long long*)&stats[TJ_ST_GRID_CAND], (unsigned long long)cand);
 }
 
 }  // namespace
@@ -581,9 +533,6 @@ GridLayout grid_layout(int64_t nR, int64_t nS, int64_t out_cap) {
     L.rrow = carve(16 * (size_t)(nR > 0 ? nR : 1));
     L.hkey = carve(8 * hs);
     L.hval = carve(8 * hs);
-    L.cnt = carve(4 * (size_t)(nR > 0 ? nR : 1));
-    L.off = carve(8 * (size_t)(nR > 0 ? nR : 1));
-    L.bsum = carve(8 * (size_t)((nR + SCAN_B - 1) / SCAN_B + 1));
     L.outj = carve(8 * (size_t)(out_cap > 0 ? out_cap : 1));  // pair keys
     // occupancy bitmap: 64 bits per S point (at least 2^20); sparser grids
     // go without it
@@ -617,9 +566,6 @@ int launch_grid_join(const float* R, int64_t nR, const float* S, int64_t nS, int
     float4* srow = (float4*)(ws + L.srow);
     uint64_t* hkey = (uint64_t*)(ws + L.hkey);
     uint2* hval = (uint2*)(ws + L.hval);
-    uint32_t* cnt = (uint32_t*)(ws + L.cnt);
-    uint64_t* off = (uint64_t*)(ws + L.off);
-    uint64_t* bsum = (uint64_t*)(ws + L.bsum);
     uint64_t* okeys = (uint64_t*)(ws + L.outj);
     const int jbits = bits_for(nS > 1 ? nS : 2);
     uint64_t* cR = (uint64_t*)(ws + L.count_nr);
@@ -660,18 +606,10 @@ int launch_grid_join(const float* R, int64_t nR, const float* S, int64_t nS, int
     hipLaunchKernelGGL(bm_top_scan, dim3(1), dim3(1024), 0, s, gp, bmsum);
     hipLaunchKernelGGL(bm_apply, dim3(nbm), dim3(256), 0, s, gp, bm, bmsum, bmpre);
     hipLaunchKernelGGL(runs_kernel, dim3(blocks_for(nS, gb)), dim3(256), 0, s, nS, gp, skeys, bm, bmpre, runs);
-    // count -> scan -> write -> key sort
+    // one probe pass appends the pair keys; the key sort orders them
     const unsigned pb = blocks_for(nR, (int64_t)num_cus() * 16);
-    hipLaunchKernelGGL(probe_kernel<0>, dim3(pb), dim3(256), 0, s, rrow, nR, gp, srow, hkey, hval,
-                       (uint64_t)(L.hslots - 1), bm, bmpre, runs, T, (int)self_join, cnt, off, okeys, jbits, out_cap,
-                       stats);
-    const int64_t nb = (nR + SCAN_B - 1) / SCAN_B;
-    hipLaunchKernelGGL(scan_block_sums, dim3((unsigned)nb), dim3(256), 0, s, cnt, nR, bsum);
-    hipLaunchKernelGGL(scan_sums, dim3(1), dim3(1024), 0, s, bsum, nb, stats);
-    hipLaunchKernelGGL(scan_apply, dim3((unsigned)nb), dim3(256), 0, s, cnt, nR, bsum, off);
-    hipLaunchKernelGGL(probe_kernel<1>, dim3(pb), dim3(256), 0, s, rrow, nR, gp, srow, hkey, hval,
-                       (uint64_t)(L.hslots - 1), bm, bmpre, runs, T, (int)self_join, cnt, off, okeys, jbits, out_cap,
-                       stats);
+    hipLaunchKernelGGL(probe_kernel, dim3(pb), dim3(256), 0, s, rrow, nR, gp, srow, hkey, hval,
+                       (uint64_t)(L.hslots - 1), bm, bmpre, runs, T, (int)self_join, okeys, jbits, out_cap, stats);
     TJ_HIP(hipGetLastError());
     // the keys were written in query (cell) order; one radix sort over all
     // their bits orders them by (i, j), then they unpack to pairs
