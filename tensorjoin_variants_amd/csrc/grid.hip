@@ -54,11 +54,13 @@ __device__ inline float ord2f(uint32_t o) {
     return __builtin_bit_cast(float, u);
 }
 
-// bounds[0..2] = ordered min, bounds[3..5] = ordered max of the finite
-// coordinates; the non-finite ones are counted in stats[TJ_ST_INVALID] (the
-// join is then rejected).  Three-dimensional points go four at a time as
-// three 16-byte loads.
-__global__ void __launch_bounds__(256) bounds_kernel(const float* __restrict__ x, int64_t n, int d, uint32_t* bounds,
+// Per block: part[6 b + k] = ordered min, part[6 b + 3 + k] = ordered max
+// of the finite coordinates of dimension k (reduced by bounds_reduce_kernel:
+// one same-address atomic per block had serialised into most of the time);
+// the non-finite ones are counted in stats[TJ_ST_INVALID] (the join is then
+// rejected).  Three-dimensional points go four at a time as three 16-byte
+// loads.
+__global__ void __launch_bounds__(256) bounds_kernel(const float* __restrict__ x, int64_t n, int d, uint32_t* part,
                                                      uint64_t* __restrict__ stats) {
     uint32_t mn[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, mx[3] = {0, 0, 0};
     uint32_t bad = 0;
@@ -80,8 +82,7 @@ __global__ void __launch_bounds__(256) bounds_kernel(const float* __restrict__ x
     }
     for (int64_t t = 4 * n4 + tid; t < n; t += stride)
         for (int k = 0; k < d; ++k) take(k, x[t * d + k]);
-    // wave, then block reduction; one atomic per block and bound (per-wave
-    // atomics on six addresses serialised into most of the kernel's time)
+    // wave, then block reduction
     __shared__ uint32_t smn[3][4], smx[3][4], sbad[4];
     const int wave = threadIdx.x >> 6;
     for (int k = 0; k < d; ++k) {
@@ -99,20 +100,41 @@ __global__ void __launch_bounds__(256) bounds_kernel(const float* __restrict__ x
         uint32_t nb = 0;
         for (int w = 0; w < (int)(blockDim.x >> 6); ++w) nb += sbad[w];
         if (nb) atomicAdd((unsigned long long*)&stats[TJ_ST_INVALID], (unsigned long long)nb);
-        for (int k = 0; k < d; ++k) {
-            uint32_t a = smn[k][0], b = smx[k][0];
-            for (int w = 1; w < (int)(blockDim.x >> 6); ++w) {
-                a = smn[k][w] < a ? smn[k][w] : a;
-                b = smx[k][w] > b ? smx[k][w] : b;
-            }
-            atomicMin(&bounds[k], a);
-            atomicMax(&bounds[3 + k], b);
+        for (int k = 0; k < 3; ++k) {
+            uint32_t a = 0xFFFFFFFFu, b = 0;
+            if (k < d)
+                for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+                    a = smn[k][w] < a ? smn[k][w] : a;
+                    b = smx[k][w] > b ? smx[k][w] : b;
+                }
+            part[6 * blockIdx.x + k] = a;
+            part[6 * blockIdx.x + 3 + k] = b;
         }
     }
 }
 
-__global__ void init_bounds_kernel(uint32_t* bounds) {
-    if (threadIdx.x < 3) { bounds[threadIdx.x] = 0xFFFFFFFFu; bounds[3 + threadIdx.x] = 0; }
+// bounds[0..5] from the nparts block partials.
+__global__ void __launch_bounds__(256) bounds_reduce_kernel(const uint32_t* __restrict__ part, int nparts,
+                                                            uint32_t* __restrict__ bounds) {
+    uint32_t v[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0, 0, 0};
+    for (int b = threadIdx.x; b < nparts; b += blockDim.x)
+        for (int k = 0; k < 3; ++k) {
+            const uint32_t a = part[6 * b + k], c = part[6 * b + 3 + k];
+            v[k] = a < v[k] ? a : v[k];
+            v[3 + k] = c > v[3 + k] ? c : v[3 + k];
+        }
+    __shared__ uint32_t sv[6][256];
+    for (int k = 0; k < 6; ++k) sv[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o)
+            for (int k = 0; k < 6; ++k) {
+                const uint32_t a = sv[k][threadIdx.x], c = sv[k][threadIdx.x + o];
+                sv[k][threadIdx.x] = k < 3 ? (c < a ? c : a) : (c > a ? c : a);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) bounds[threadIdx.x] = sv[threadIdx.x][0];
 }
 
 __global__ void grid_params_kernel(const uint32_t* bounds, double T, int d, int idxbits, uint64_t bm_bits,
@@ -513,6 +535,9 @@ __global__ void __launch_bounds__(256) probe_kernel(const float4* __restrict__ r
 namespace tj {
 
 
+// Blocks of one bounds pass (a partial per block, no atomics): four per CU.
+static int64_t bounds_blocks() { return 4 * (int64_t)num_cus(); }
+
 GridLayout grid_layout(int64_t nR, int64_t nS, int64_t out_cap) {
     GridLayout L{};
     size_t cur = 0;
@@ -523,6 +548,7 @@ GridLayout grid_layout(int64_t nR, int64_t nS, int64_t out_cap) {
     L.hslots = hs;
     L.idxbits = bits_for(nmax > 1 ? nmax : 2);
     L.bounds = carve(64);
+    L.bpart = carve(4 * 6 * 2 * (size_t)bounds_blocks());
     L.params = carve(sizeof(GridParams));
     L.count_nr = carve(8);
     L.count_ns = carve(8);
@@ -572,11 +598,16 @@ int launch_grid_join(const float* R, int64_t nR, const float* S, int64_t nS, int
     uint64_t* cS = (uint64_t*)(ws + L.count_ns);
     int rc;
     const unsigned gb = (unsigned)(num_cus() * 4);
-    hipLaunchKernelGGL(init_bounds_kernel, dim3(1), dim3(64), 0, s, bounds);
-    const unsigned bb = (unsigned)(num_cus() * 2);
-    hipLaunchKernelGGL(bounds_kernel, dim3(blocks_for((nR + 3) / 4, bb)), dim3(256), 0, s, R, nR, d, bounds, stats);
-    if (!self_join)
-        hipLaunchKernelGGL(bounds_kernel, dim3(blocks_for((nS + 3) / 4, bb)), dim3(256), 0, s, S, nS, d, bounds, stats);
+    uint32_t* bpart = (uint32_t*)(ws + L.bpart);
+    const unsigned bbR = blocks_for((nR + 3) / 4, bounds_blocks());
+    hipLaunchKernelGGL(bounds_kernel, dim3(bbR), dim3(256), 0, s, R, nR, d, bpart, stats);
+    unsigned nparts = bbR;
+    if (!self_join) {
+        const unsigned bbS = blocks_for((nS + 3) / 4, bounds_blocks());
+        hipLaunchKernelGGL(bounds_kernel, dim3(bbS), dim3(256), 0, s, S, nS, d, bpart + 6 * (size_t)bbR, stats);
+        nparts += bbS;
+    }
+    hipLaunchKernelGGL(bounds_reduce_kernel, dim3(1), dim3(256), 0, s, bpart, (int)nparts, bounds);
     hipLaunchKernelGGL(grid_params_kernel, dim3(1), dim3(64), 0, s, bounds, T, d, L.idxbits, L.bm_bits, gp);
     uint64_t* bm = (uint64_t*)(ws + L.bitmap);
     hipLaunchKernelGGL(clear_bitmap_kernel, dim3(blocks_for((int64_t)(L.bm_bits / 64), gb)), dim3(256), 0, s, gp, bm);

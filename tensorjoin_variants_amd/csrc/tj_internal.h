@@ -215,7 +215,7 @@ inline int bits_for(int64_t n) { int b = 0; while (((int64_t)1 << b) < n) ++b; r
 namespace tj {
 // Workspace layout of the grid join (grid.hip), kept in the plan (api.hip).
 struct GridLayout {
-    size_t bounds, params, rkeys, skeys, sort_tmp, srow, rrow, hkey, hval, outj, count_nr, count_ns, bitmap, bmpre,
+    size_t bounds, bpart, params, rkeys, skeys, sort_tmp, srow, rrow, hkey, hval, outj, count_nr, count_ns, bitmap, bmpre,
         bmsum, runs, total;
     uint64_t hslots;
     uint64_t bm_bits;  // capacity of the cell-occupancy bitmap
