@@ -206,12 +206,26 @@ __device__ inline void cell_of(const float* x, const GridParams& g, int64_t c[3]
 __global__ void __launch_bounds__(256) keys_kernel(const float* __restrict__ x, int64_t n, const GridParams* __restrict__ gp,
                                                    uint64_t* __restrict__ keys) {
     const GridParams g = *gp;
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+    auto key_of = [&](const float* p, int64_t t) {
         int64_t c[3];
-        cell_of(x + t * g.d, g, c);
+        cell_of(p, g, c);
         const uint64_t id = (uint64_t)c[0] * g.mul[0] + (uint64_t)c[1] * g.mul[1] + (uint64_t)c[2] * g.mul[2];
-        keys[t] = (id << g.idxbits) | (uint64_t)t;
+        return (id << g.idxbits) | (uint64_t)t;
+    };
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
+    // three-dimensional points four at a time: three 16-byte loads, two
+    // 16-byte stores
+    const int64_t n4 = (g.d == 3 && ((uintptr_t)x & 15) == 0 && ((uintptr_t)keys & 15) == 0) ? n / 4 : 0;
+    const float4* xv = (const float4*)x;
+    for (int64_t q = tid; q < n4; q += stride) {
+        const float4 a = xv[3 * q], b = xv[3 * q + 1], c = xv[3 * q + 2];
+        const float p0[3] = {a.x, a.y, a.z}, p1[3] = {a.w, b.x, b.y}, p2[3] = {b.z, b.w, c.x}, p3[3] = {c.y, c.z, c.w};
+        const int64_t t = 4 * q;
+        ulonglong2* kv = (ulonglong2*)(keys + t);
+        kv[0] = make_ulonglong2(key_of(p0, t), key_of(p1, t + 1));
+        kv[1] = make_ulonglong2(key_of(p2, t + 2), key_of(p3, t + 3));
     }
+    for (int64_t t = 4 * n4 + tid; t < n; t += stride) keys[t] = key_of(x + t * g.d, t);
 }
 
 __device__ inline uint64_t hmix(uint64_t k) {
@@ -620,11 +634,11 @@ int launch_grid_join(const float* R, int64_t nR, const float* S, int64_t nS, int
     const int lo_digit = L.idxbits / 8 * 8;
     float4* rrow = self_join ? srow : (float4*)(ws + L.rrow);
     if (!self_join) {
-        hipLaunchKernelGGL(keys_kernel, dim3(blocks_for(nR, gb)), dim3(256), 0, s, R, nR, gp, rkeys);
+        hipLaunchKernelGGL(keys_kernel, dim3(blocks_for((nR + 3) / 4, gb)), dim3(256), 0, s, R, nR, gp, rkeys);
         if ((rc = launch_sort(ws + L.sort_tmp, rkeys, cR, nR, 63, s, nullptr, lo_digit))) return rc;
         hipLaunchKernelGGL(rows_kernel, dim3(blocks_for(nR, gb)), dim3(256), 0, s, R, nR, gp, rkeys, rrow);
     }
-    hipLaunchKernelGGL(keys_kernel, dim3(blocks_for(nS, gb)), dim3(256), 0, s, S, nS, gp, skeys);
+    hipLaunchKernelGGL(keys_kernel, dim3(blocks_for((nS + 3) / 4, gb)), dim3(256), 0, s, S, nS, gp, skeys);
     if ((rc = launch_sort(ws + L.sort_tmp, skeys, cS, nS, 63, s, nullptr, lo_digit))) return rc;
     hipLaunchKernelGGL(gather_kernel, dim3(blocks_for(nS, gb)), dim3(256), 0, s, S, nS, gp, skeys, srow, hkey, hval,
                        (uint64_t)(L.hslots - 1), bm);
