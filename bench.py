@@ -127,6 +127,9 @@ def main() -> int:
                              "gist_shape_selfjoin"])
     ap.add_argument("--precision", default=None)
     ap.add_argument("--scale", type=float, default=1.0, help="row-count scale (1.0 = full workload)")
+    ap.add_argument("--variants", default=None,
+                    help="other screen precisions timed on the same join (comma list; '' for none; "
+                         "default: the 16-bit hi/lo splits for certified_d128)")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-host", action="store_true", help="skip the host-to-host pipeline leg")
@@ -228,6 +231,36 @@ def main() -> int:
     }
     if grid:
         result["grid_candidate_evals"] = stats["grid_candidates"]
+
+    if args.variants is None:
+        args.variants = "fp16x3,bf16x3" if args.workload == "certified_d128" else ""
+    if not grid and args.variants:
+        # the other screen precisions on the same join: the hi/lo splits
+        # issue three MFMA terms per product for a band ~2^11 narrower
+        result["precision_variants"] = {}
+        for vp in [p for p in args.variants.split(",") if p and p != prec]:
+            log(f"precision variant {vp}")
+            vplan = Plan(nR, nS, d, "certified", vp, out_capacity=plan.out_capacity,
+                         amb_capacity=max(4096, plan.amb_capacity), timing=True)
+            vres = vplan.run(R, S, w.T, w.out_dtype)
+            vout = torch.empty((vplan.out_capacity, 2), dtype=tdt, device="cuda")
+            vms = timed(vplan, R, S, w.T, vout, fmt, max(2, min(args.steps, 3)), 1)
+            vst = _lib.decode_stats(vplan._stats.cpu().numpy().view(np.uint64))
+            vstages = vplan.stage_times()
+            n1 = stats["out"]
+            same = vst["out"] == n1 and bool(torch.equal(vout[:n1], out[:n1]))
+            vflops = 2.0 * d * pairs_total
+            result["precision_variants"][vp] = {
+                "ms_per_step": vms, "value": pairs_total / (vms / 1e3), "stages_ms": vstages,
+                "screen_mfma_frac_algorithmic": vflops / (vstages["screen"] / 1e3) / PEAK[vp],
+                "screen_mfma_frac_issued": 3 * vflops / (vstages["screen"] / 1e3) / PEAK[vp]
+                if vp.endswith("x3") else None,
+                "candidates": vst["candidates"], "ambiguous": vst["ambiguous"],
+                "ambiguous_fraction": vst["ambiguous"] / pairs_total, "fp64_refined": vst["fp64_refined"],
+                "max_abs_d2_err_vs_fp64": vst["max_err"], "certificate_violations": vst["violations"],
+                "identical_to_headline": same}
+            log(f"  {vp}: {vms:.1f} ms, ambiguous {vst['ambiguous']}, identical={same}")
+            del vout, vplan
 
     if not args.no_plain and not grid:
         log("plain FP64 GPU method (one run)")

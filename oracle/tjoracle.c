@@ -210,16 +210,34 @@ int64_t tjo_grid_join(const float* R, int64_t nR, const float* S, int64_t nS, in
     if (self) { S = R; nS = nR; }
     double side = sqrt(T) * (1.0 + 0x1p-10);
     if (!(side > 0.0)) return -3;
+    double mn[3] = {0, 0, 0}, mx[3] = {0, 0, 0};
     double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
     int64_t ncell[3] = {1, 1, 1};
     for (int64_t k = 0; k < d; ++k) {
-        lo[k] = INFINITY; hi[k] = -INFINITY;
-        for (int64_t i = 0; i < nR; ++i) { double v = R[i * d + k]; if (v < lo[k]) lo[k] = v; if (v > hi[k]) hi[k] = v; }
-        for (int64_t j = 0; j < nS; ++j) { double v = S[j * d + k]; if (v < lo[k]) lo[k] = v; if (v > hi[k]) hi[k] = v; }
-        if (nR + nS == 0) { lo[k] = 0; hi[k] = 0; }
-        lo[k] -= side; hi[k] += side;
-        ncell[k] = (int64_t)ceil((hi[k] - lo[k]) / side) + 1;
+        mn[k] = INFINITY; mx[k] = -INFINITY;
+        for (int64_t i = 0; i < nR; ++i) { double v = R[i * d + k]; if (v < mn[k]) mn[k] = v; if (v > mx[k]) mx[k] = v; }
+        for (int64_t j = 0; j < nS; ++j) { double v = S[j * d + k]; if (v < mn[k]) mn[k] = v; if (v > mx[k]) mx[k] = v; }
+        if (nR + nS == 0) { mn[k] = 0; mx[k] = 0; }
     }
+    /* Widen the side until every axis has at most 2^38 cells and the linear
+     * id fits 62 bits: (x - lo) / side is then exact to far less than the
+     * 2^-10 slack of the side (relative error < 2^-51 of at most 2^38 cells),
+     * so a pair closer than sqrt(T) on an axis sits in adjacent cells.  One
+     * far outlier would otherwise stretch an axis to ~2^55 cells, where
+     * rounding moves points by several cells. */
+    for (;;) {
+        double prod = 1.0;
+        int wide = 0;
+        for (int64_t k = 0; k < d; ++k) {
+            lo[k] = mn[k] - side; hi[k] = mx[k] + side;
+            double nc = ceil((hi[k] - lo[k]) / side) + 1.0;
+            if (nc > 0x1p38) wide = 1;
+            prod *= nc;
+        }
+        if (!wide && prod <= 0x1p62) break;
+        side *= 2.0;
+    }
+    for (int64_t k = 0; k < d; ++k) ncell[k] = (int64_t)ceil((hi[k] - lo[k]) / side) + 1;
     kv* cells = (kv*)malloc((size_t)(nS > 0 ? nS : 1) * sizeof(kv));
     if (!cells) return -1;
     for (int64_t j = 0; j < nS; ++j) {

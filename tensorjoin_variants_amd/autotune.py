@@ -3,7 +3,9 @@
 Every choice is certified (the output never depends on it), so the tuner
 only trades speed.  ``choose_precision`` is the static default; ``Autotuner``
 times candidate precisions on a sample of the actual join with HIP events and
-caches the winner per (d, size class) in a JSON file.
+keeps the winner per (d, size class) in an in-memory table.  The table is
+written to a JSON file only when one is named (the ``cache`` argument or the
+TJ_AUTOTUNE_CACHE environment variable); by default tuning leaves no file.
 """
 from __future__ import annotations
 
@@ -12,7 +14,34 @@ import math
 import os
 from pathlib import Path
 
-CACHE = Path(os.environ.get("TJ_AUTOTUNE_CACHE", str(Path.home() / ".cache" / "tjoin_autotune.json")))
+
+def default_cache() -> Path | None:
+    """The tuning table's file: TJ_AUTOTUNE_CACHE if set, else none (memory only)."""
+    v = os.environ.get("TJ_AUTOTUNE_CACHE")
+    return Path(v) if v else None
+
+
+class _Table:
+    """Winner table, loaded from and saved to ``cache`` when one is given."""
+
+    def __init__(self, cache: Path | None):
+        self.cache = default_cache() if cache is None else Path(cache)
+        self.table = {}
+        if self.cache is not None:
+            try:
+                self.table = json.loads(self.cache.read_text())
+            except (OSError, ValueError):
+                self.table = {}
+
+    def save(self) -> None:
+        if self.cache is None:
+            return
+        try:
+            self.cache.parent.mkdir(parents=True, exist_ok=True)
+            self.cache.write_text(json.dumps(self.table, indent=1, sort_keys=True))
+        except OSError:
+            pass
+
 
 CANDIDATES = ("fp16", "fp8", "bf16", "fp16x3", "fp8x3")
 
@@ -28,13 +57,9 @@ def size_class(n: int) -> int:
     return int(math.log2(max(n, 1)))
 
 
-class Autotuner:
-    def __init__(self, cache: Path = CACHE):
-        self.cache = cache
-        try:
-            self.table = json.loads(cache.read_text())
-        except Exception:
-            self.table = {}
+class Autotuner(_Table):
+    def __init__(self, cache: Path | None = None):
+        super().__init__(cache)
 
     def key(self, d: int, nR: int, nS: int | None) -> str:
         return f"d{d}_r{size_class(nR)}_s{size_class(nS or nR)}"
@@ -66,11 +91,7 @@ class Autotuner:
                 times[prec] = float("inf")
         win = min(times, key=times.get)
         self.table[self.key(R.shape[1], R.shape[0], nS)] = win
-        try:
-            self.cache.parent.mkdir(parents=True, exist_ok=True)
-            self.cache.write_text(json.dumps(self.table, indent=1, sort_keys=True))
-        except OSError:
-            pass
+        self.save()
         return {"winner": win, "ms": times}
 
 
@@ -82,25 +103,21 @@ def width_class(d: int) -> int:
     raise ValueError(f"d={d} exceeds the batch screen's widest instantiation (256)")
 
 
-class BatchTuner:
+class BatchTuner(_Table):
     """Per-width screen precision for batches of joins.
 
     ``tune`` times each batch precision (FP16, FP8) end to end on a sample of
     the batch's own joins -- the largest few of every width class, where the
-    pair work is -- and keeps the faster per width class (cached in the same
-    JSON file as ``Autotuner``).  ``choose`` maps a batch to per-join
+    pair work is -- and keeps the faster per width class (kept in the same
+    kind of table as ``Autotuner``).  ``choose`` maps a batch to per-join
     precisions from that table (FP16 where a class was never tuned).  The
     choice never changes the output, only the time.
     """
 
     CANDIDATES = ("fp16", "fp8")
 
-    def __init__(self, cache: Path = CACHE):
-        self.cache = cache
-        try:
-            self.table = json.loads(cache.read_text())
-        except Exception:
-            self.table = {}
+    def __init__(self, cache: Path | None = None):
+        super().__init__(cache)
 
     @staticmethod
     def key(w: int) -> str:
@@ -147,11 +164,7 @@ class BatchTuner:
             plan.close()
         win = times["prune"] < times["no_prune"]
         self.table["batch_prune"] = win
-        try:
-            self.cache.parent.mkdir(parents=True, exist_ok=True)
-            self.cache.write_text(json.dumps(self.table, indent=1, sort_keys=True))
-        except OSError:
-            pass
+        self.save()
         return {"prune": win, "ms": times, "joins": len(idx)}
 
     def tune(self, sets, Ts, per_class: int = 8, reps: int = 2) -> dict:
@@ -188,9 +201,5 @@ class BatchTuner:
             win = min(times, key=times.get)
             self.table[self.key(w)] = win
             report[w] = {"winner": win, "ms": times, "joins": len(idx)}
-        try:
-            self.cache.parent.mkdir(parents=True, exist_ok=True)
-            self.cache.write_text(json.dumps(self.table, indent=1, sort_keys=True))
-        except OSError:
-            pass
+        self.save()
         return report

@@ -28,9 +28,10 @@
 //    per-row insertion sort was quadratic in the longest row);
 //  * the predicate is the sequential binary64 difference form of the CPU
 //    oracle (d2 <= T; GDS used sqrt(d2) <= eps, GDS/kernel.cu:368-371).
-// The cell side is sqrt(T) * (1 + 2^-10) (or larger when the cell ids would
-// not fit a 64-bit key): every pair with d2_seq <= T has all coordinate
-// differences below one cell side, so it lies in adjacent cells.
+// The cell side is sqrt(T) * (1 + 2^-10) (or larger when an axis would
+// span more than 2^38 cells or the cell ids would not fit a 64-bit key):
+// every pair with d2_seq <= T has all coordinate differences below one cell
+// side, so it lies in adjacent cells.
 #include "tj_internal.h"
 
 namespace {
@@ -149,16 +150,25 @@ __global__ void grid_params_kernel(const uint32_t* bounds, double T, int d, int 
         lo[k] = a;
         ext[k] = b > a ? b - a : 0.0;
     }
-    // widen the cell until the linear id of every (padded) cell fits beside the index bits
+    // Widen the cell until (1) no axis has more than 2^38 cells and (2) the
+    // linear id of every (padded) cell fits beside the index bits.  (1) keeps
+    // cell_of exact enough: its binary64 value (x - lo) / side carries a
+    // relative error below 2^-51, i.e. below 2^-13 cells at 2^38 cells, far
+    // inside the 2^-10 slack of the side, so two points closer than sqrt(T)
+    // on an axis always land in the same or adjacent cells.  Without it one
+    // far outlier stretches an axis to ~2^55 cells, where rounding moves a
+    // point by several cells and the probe misses pairs.
     for (;;) {
         int bits = 0;
+        bool wide = false;
         for (int k = 0; k < d; ++k) {
             const double nc = floor(ext[k] / side) + 4.0;
+            if (nc > 0x1p38) wide = true;
             int b = 0;
             while (b < 62 && ldexp(1.0, b) < nc) ++b;
             bits += b;
         }
-        if (bits + idxbits <= 63) break;
+        if (!wide && bits + idxbits <= 63) break;
         side *= 2.0;
     }
     uint64_t mul = 1;

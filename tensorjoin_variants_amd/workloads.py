@@ -8,7 +8,7 @@ workload definitions ask.
 
 Sizes (``scale=1.0``):
   clusters_selfjoin  N=8192, d=32, 16 Gaussian clusters, eps = 0.25th pct     (int32 i<j)
-  grid_rxs           |R|=|S|=4,194,304, d=3, 90% uniform + 10% in 64 blobs, eps=0.002
+  grid_rxs           |R|=|S|=4,194,304, d=3, 90% uniform + 10% in 64 shared blobs, eps=0.002
   certified_d128     |R|=1,048,576, |S|=262,144, d=128, half-normal 40% zeros, L2-normalised,
                      eps at selectivity 1e-4
   heavytail_fp8      |R|=|S|=262,144, d=512, Student-t nu=3, eps at selectivity 1e-5
@@ -60,12 +60,12 @@ def clusters_selfjoin(n: int = 8192, d: int = 32, clusters: int = 16, pct: float
                          seed=SEED_BASE + seed_offset, assignment="uniform random"))
 
 
-def _points_with_blobs(rng, n, d, uniform_frac, blobs, sigma):
+def _points_with_blobs(rng, n, d, uniform_frac, centres, sigma):
+    blobs = centres.shape[0]
     nu = int(round(n * uniform_frac))
     nb = n - nu
     X = np.empty((n, d), dtype=np.float32)
     X[:nu] = rng.random((nu, d), dtype=np.float32)
-    centres = rng.random((blobs, d))
     per = np.full(blobs, nb // blobs)
     per[: nb % blobs] += 1
     idx = np.repeat(np.arange(blobs), per)
@@ -75,12 +75,17 @@ def _points_with_blobs(rng, n, d, uniform_frac, blobs, sigma):
 
 
 def grid_rxs(n: int = 4_194_304, d: int = 3, eps: float = 0.002, seed_offset: int = 2) -> Workload:
-    """90% uniform in [0,1]^3, 10% in 64 Gaussian blobs (sigma 0.003), R and S independent."""
+    """90% uniform in [0,1]^3, 10% in 64 Gaussian blobs (sigma 0.003).  The
+    64 blob centres (uniform in [0,1]^3) are drawn once and shared by R and
+    S, so R's blob points meet S's: about 170 S neighbours per R blob point,
+    ~7e7 output pairs at full size (the output-bound half of the workload);
+    the points themselves are drawn independently."""
     rng = _rng(seed_offset)
-    R = _points_with_blobs(rng, n, d, 0.9, 64, 0.003)
-    S = _points_with_blobs(rng, n, d, 0.9, 64, 0.003)
+    centres = rng.random((64, d))
+    R = _points_with_blobs(rng, n, d, 0.9, centres, 0.003)
+    S = _points_with_blobs(rng, n, d, 0.9, centres, 0.003)
     return Workload("grid_rxs", R, S, eps, eps * eps, np.int64,
-                    dict(n=n, d=d, uniform_frac=0.9, blobs=64, sigma=0.003,
+                    dict(n=n, d=d, uniform_frac=0.9, blobs=64, sigma=0.003, shared_centres=True,
                          seed=SEED_BASE + seed_offset))
 
 

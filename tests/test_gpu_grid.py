@@ -129,3 +129,18 @@ def test_grid_full_size(tj, oracle):
     want = oracle.grid_join(w.R, w.S, w.T)
     assert got.shape == want.shape
     assert np.array_equal(got, want)
+
+
+def test_grid_far_outlier(tj, oracle):
+    """Outliers 1e10 away from points a few ulps apart: the cell side must
+    widen so rounding of the cell coordinate cannot separate a pair.
+    Checked against the brute-force oracle, not the grid oracle."""
+    from test_oracle import far_outlier_sets
+    for name, X, T in far_outlier_sets():
+        say("grid far outlier", name)
+        want = oracle.join(X, None, T)
+        got = tj.join(X, None, T=T, method="grid", out_dtype=np.int64)
+        assert np.array_equal(got, want), name
+        Y = X[::-1].copy()
+        got = tj.join(X, Y, T=T, method="grid", out_dtype=np.int64)
+        assert np.array_equal(got, oracle.join(X, Y, T)), name

@@ -122,3 +122,27 @@ def test_canonical_format(oracle):
     pairs = np.array([[0, 2], [1, 3]], np.int64)
     ids = oracle.canonical_u64(pairs, 4)
     assert ids.tolist() == [0, 2, 5, 7, 8, 10, 13, 15]
+
+
+def far_outlier_sets():
+    """Points a few ulps apart near 1.0 plus outliers 1e10 away: one axis then
+    spans ~2^55 cells of side sqrt(T), where binary64 rounding of the cell
+    coordinate moves points by several cells unless the side is widened."""
+    out = [("3-point d1", np.array([[-1e10], [1 + 7 * 2 ** -23], [1 + 9 * 2 ** -23]], np.float32), 3e-7 ** 2)]
+    rng = np.random.default_rng(15)
+    for d in (2, 3):
+        X = (1.0 + rng.integers(0, 16, (600, d)) * 2.0 ** -23).astype(np.float32)
+        X[0, 0] = -1e10
+        X[1, d - 1] = 1e10
+        out.append((f"ulp cloud d{d}", X, 3e-7 ** 2))
+    return out
+
+
+@pytest.mark.parametrize("case", far_outlier_sets(), ids=lambda c: c[0])
+def test_grid_join_far_outlier(oracle, case):
+    _, X, T = case
+    want = oracle.join(X, None, T)
+    assert len(want) > 0
+    assert np.array_equal(oracle.grid_join(X, None, T), want)
+    Y = X[::-1].copy()
+    assert np.array_equal(oracle.grid_join(X, Y, T), oracle.join(X, Y, T))
