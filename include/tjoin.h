@@ -184,13 +184,56 @@ int tj_params(const uint32_t* maxabs_bits, double T, int d, int precision, void*
 int tj_rowmeta(const float* x, int64_t n, int d, int precision, const void* params,
                void* q_hi, void* q_lo, float* meta, uint64_t* stats, void* stream);
 
-/* FP64 refinement of queued pair keys (i << jbits | j): the sequential
- * difference form, appending accepted keys at out_keys[stats[TJ_ST_OUT]++].
- * Replaces refine_ambiguous_fp64_i64 (CEJ/src/run_g2a_tensorjoin.py:193-232). */
-int tj_refine_fp64(const float* R, const float* S, int d, int jbits, double T,
-                   const uint64_t* amb_keys, const float* amb_aux, int64_t amb_capacity,
-                   const void* params, uint64_t* out_keys, int64_t out_capacity,
-                   uint64_t* stats, void* stream);
+/* Sizes of one side's row metadata buffers: q_bytes for each code array
+ * (q_hi, and q_lo for the hi/lo precisions) and meta_bytes for `meta`.
+ * Host only. */
+int tj_rowmeta_sizes(int64_t n, int d, int precision, size_t* q_bytes, size_t* meta_bytes);
+
+/* The MFMA screen over A x B (nb == 0: the self-join of A, i < j), the
+ * certified classification of every pair from its MFMA dot and the row
+ * terms: pairs certified inside by the fast accept are appended to
+ * out_keys (count stats[TJ_ST_OUT]); the band the screen cannot decide
+ * (every survivor of the fast reject with TJ_FLAG_AUDIT) is appended to the
+ * candidate queue: cand_keys[k] = i << jbits | j (jbits = the bits of the
+ * number of B rows, the smallest b with 2^b >= max(nB, 2)) and cand_s[k] =
+ * its binary32 screen value, count stats[TJ_ST_CANDIDATES].  The queue is in
+ * no particular order.  Entries past a capacity are
+ * counted, not written.  Inputs come from tj_rowmeta of each side with the
+ * same params; stats must hold the counts of earlier stages (zero at the
+ * start of a join); the screen's work cursor stats[TJ_ST_WORK] is reset
+ * here.  Replaces scan16 / scan8 (TG/src/g2_kernels.py:7-82) with their
+ * compaction (half_classify, TG/src/gpu_kernels.py:55-88), as launched
+ * stage by stage by TG/src/frozen_driver.py:69-77. */
+int tj_screen(int precision, int d, const void* qa_hi, const void* qa_lo, const float* meta_a, int64_t na,
+              const void* qb_hi, const void* qb_lo, const float* meta_b, int64_t nb, const void* params,
+              uint64_t* out_keys, int64_t out_capacity, uint64_t* cand_keys, float* cand_s, int64_t cand_capacity,
+              unsigned flags, uint64_t* stats, void* stream);
+
+/* The candidate queue of tj_screen -> decisions: per-pair certified
+ * interval, then the certified FP32 filter and the sequential FP64 form for
+ * what stays ambiguous; accepted keys appended to out_keys at
+ * stats[TJ_ST_OUT].  Reads min(stats[TJ_ST_CANDIDATES], cand_capacity)
+ * entries.  S == NULL for a self-join; n_cols = the number of S rows (of R
+ * for a self-join), which fixes the key layout.  TJ_FLAG_AUDIT recomputes
+ * certified pairs in FP64 too and counts violations.  Replaces the accept /
+ * ambiguous queue / FP64 terminal cascade of TG/src/g2_operator_r1.py:52-57
+ * with certified_fp32_filter_i64 (CEJ/src/run_g3c_b_r1_gpu_cascade.py:57-129)
+ * and refine_ambiguous_fp64_i64 (CEJ/src/run_g2a_tensorjoin.py:193-232). */
+int tj_certify(const float* R, const float* S, int d, int64_t n_cols, const uint64_t* cand_keys, const float* cand_s,
+               int64_t cand_capacity, const float* meta_R, const float* meta_S, const void* params, uint64_t* out_keys,
+               int64_t out_capacity, unsigned flags, uint64_t* stats, void* stream);
+
+/* FP64 refinement of a queue of pair keys i << jbits | j (bit 63 ignored):
+ * every key whose sequential binary64 difference form d2_seq(R_i, S_j) <= T
+ * (real units) is appended to out_keys at stats[TJ_ST_OUT] (positions past
+ * out_capacity counted, not written).  The queue length is read from the
+ * device scalar *count, clamped to capacity -- e.g. stats + TJ_ST_CANDIDATES
+ * for tj_screen's queue.  S == NULL: S = R.  Adds to stats[TJ_ST_FP64_REFINED]
+ * and stats[TJ_ST_FP64_ACCEPT].  Replaces refine_ambiguous_fp64_i64
+ * (CEJ/src/run_g2a_tensorjoin.py:193-232): ids in, accepted ids out. */
+int tj_refine_fp64(const float* R, const float* S, int d, int jbits, double T, const uint64_t* keys,
+                   const uint64_t* count, int64_t capacity, uint64_t* out_keys, int64_t out_capacity, uint64_t* stats,
+                   void* stream);
 
 /* Plain FP64 brute force (the comparison method): every pair evaluated in
  * the sequential FP64 difference form, accepted keys appended to out_keys.

@@ -65,7 +65,10 @@ def load():
         "tj_scale_scan": (i32, [vp, i64, i32, vp, vp, vp]),
         "tj_params": (i32, [vp, f64, i32, i32, vp, vp, vp]),
         "tj_rowmeta": (i32, [vp, i64, i32, i32, vp, vp, vp, vp, vp, vp]),
-        "tj_refine_fp64": (i32, [vp, vp, i32, i32, f64, vp, vp, i64, vp, vp, i64, vp, vp]),
+        "tj_refine_fp64": (i32, [vp, vp, i32, i32, f64, vp, vp, i64, vp, i64, vp, vp]),
+        "tj_rowmeta_sizes": (i32, [i64, i32, i32, C.POINTER(sz), C.POINTER(sz)]),
+        "tj_screen": (i32, [i32, i32, vp, vp, vp, i64, vp, vp, vp, i64, vp, vp, i64, vp, vp, i64, C.c_uint, vp, vp]),
+        "tj_certify": (i32, [vp, vp, i32, i64, vp, vp, i64, vp, vp, vp, vp, i64, C.c_uint, vp, vp]),
         "tj_plain_fp64": (i32, [vp, i64, vp, i64, i32, i32, f64, vp, i64, vp, vp]),
         "tj_sort_keys": (i32, [vp, C.POINTER(sz), vp, vp, i64, i32, vp]),
         "tj_unpack_pairs": (i32, [vp, vp, i64, i32, vp, i32, vp]),

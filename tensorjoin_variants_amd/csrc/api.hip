@@ -13,7 +13,7 @@
 #include <string.h>
 #include <algorithm>
 #include <new>
-#include "tj_internal.h"
+#include "plan_impl.h"
 
 namespace tj {
 int screen_fp16(int dp, bool self_join, const ScreenArgs& a, int grid, hipStream_t s, int query);
@@ -77,6 +77,43 @@ int screen_max_grid(int prec, int dp, bool self_join) {
     return occ * num_cus();
 }
 
+int screen_layout(int64_t nA, int64_t nB, bool self_join, int dp, int prec, ScreenLayout* L) {
+    const int es = prec_esize(prec);
+    const bool x3 = prec_terms(prec) == 3;
+    L->bm = screen_rows_per_wg(prec, dp, self_join);
+    if (L->bm <= 0) return TJ_ERR_UNSUPPORTED;
+    L->panels = (int)((nA + L->bm - 1) / L->bm);
+    const int maxg = screen_max_grid(prec, dp, self_join);
+    const int64_t panels = std::max(L->panels, 1);
+    int64_t cs;
+    if (self_join) {
+        // Column splits of the triangle's rows (panel p sweeps columns from
+        // its first row on, split by split; splits past nB are empty items):
+        // enough items to balance the persistent grid when a small join has
+        // few panels, none narrower than 1024 columns.
+        const int64_t want = (8ll * maxg + panels - 1) / panels;
+        const int64_t by_width = (nB + 1023) / 1024;
+        cs = std::max<int64_t>(1, std::min(want, by_width));
+    } else {
+        // Column splits of about 4 MiB of quantized S (one XCD's L2), items
+        // ordered split-major: the workgroups running at any time sweep the
+        // same chunk of S, which then stays in L2, instead of each panel
+        // streaming all of S from memory.  Also enough items to balance the
+        // persistent grid when there are few panels.
+        const int64_t row_bytes = (int64_t)dp * es * (x3 ? 2 : 1);
+        const int64_t by_l2 = std::max<int64_t>(1, (nB * row_bytes + (4ll << 20) - 1) / (4ll << 20));
+        const int64_t want = (4ll * maxg + panels - 1) / panels;
+        const int64_t by_width = (nB + 1023) / 1024;  // splits of at least 1024 columns
+        cs = std::max<int64_t>(1, std::min(std::max(by_l2, want), by_width));
+    }
+    const int64_t per = ((nB + cs - 1) / cs + 31) / 32 * 32;
+    L->cols_per_split = (int)std::max<int64_t>(per, 32);
+    L->col_splits = (int)((nB + L->cols_per_split - 1) / L->cols_per_split);
+    const int64_t items = (int64_t)L->panels * L->col_splits;
+    L->grid = (int)std::max<int64_t>(1, std::min<int64_t>(items, maxg));
+    return TJ_OK;
+}
+
 // Record what fn enqueues into an instantiated graph.  The capture runs on a
 // private stream (default priority) so that the caller's stream, which may
 // be the legacy default stream, is never put in capture mode.
@@ -106,23 +143,6 @@ int capture_graph(hipGraphExec_t* exec, int (*fn)(void*, hipStream_t), void* ctx
 
 using namespace tj;
 
-struct tj_plan {
-    int64_t nR, nS;
-    bool self_join;
-    int d, dp, method, prec;
-    int64_t out_cap, amb_cap;
-    unsigned flags;
-    int jbits, keybits;
-    int bm, panels, col_splits, cols_per_split, grid;
-    size_t off_grid;
-    GridLayout glay;
-    size_t off_maxabs, off_params, off_metaR, off_metaS, off_qRh, off_qRl, off_qSh, off_qSl, off_out, off_amb,
-        off_aux, off_sort, total;
-    hipEvent_t ev[5];
-    bool timing;
-    hipGraphExec_t graph;
-    bool has_graph;
-};
 
 static size_t carve(size_t& cur, size_t bytes) {
     size_t at = (cur + 255) / 256 * 256;
@@ -147,10 +167,10 @@ int tj_device_info(int* cu_count, int* arch_major, int* arch_minor) {
     return TJ_OK;
 }
 
-int tj_plan_create(tj_plan** out, int64_t nR, int64_t nS, int d, int method, int precision, int64_t out_capacity,
-                   int64_t amb_capacity, unsigned flags) {
-    TJ_CHECK(out != nullptr, TJ_ERR_INVALID, "plan pointer is NULL");
-    *out = nullptr;
+}  // extern "C"
+
+int tj::plan_init(tj_plan* p, int64_t nR, int64_t nS, int d, int method, int precision, int64_t out_capacity,
+                  int64_t amb_capacity, unsigned flags) {
     TJ_CHECK(nR >= 0 && nS >= 0 && d >= 1, TJ_ERR_INVALID, "bad shape nR=%lld nS=%lld d=%d", (long long)nR,
              (long long)nS, d);
     TJ_CHECK(method == TJ_METHOD_CERTIFIED || method == TJ_METHOD_PLAIN_FP64 || method == TJ_METHOD_GRID,
@@ -159,8 +179,6 @@ int tj_plan_create(tj_plan** out, int64_t nR, int64_t nS, int d, int method, int
     TJ_CHECK(out_capacity >= 0 && out_capacity < (1ll << 32), TJ_ERR_INVALID, "out_capacity must be in [0, 2^32)");
     TJ_CHECK(amb_capacity >= 0 && amb_capacity < (1ll << 32), TJ_ERR_INVALID, "amb_capacity must be in [0, 2^32)");
     TJ_CHECK(nR < (1ll << 31) && nS < (1ll << 31), TJ_ERR_UNSUPPORTED, "point sets beyond 2^31 rows");
-    tj_plan* p = new (std::nothrow) tj_plan();
-    TJ_CHECK(p != nullptr, TJ_ERR_INVALID, "out of host memory");
     p->nR = nR;
     p->self_join = (nS == 0);
     p->nS = p->self_join ? nR : nS;
@@ -172,24 +190,16 @@ int tj_plan_create(tj_plan** out, int64_t nR, int64_t nS, int d, int method, int
     p->flags = flags;
     p->jbits = bits_for(p->nS > 1 ? p->nS : 2);
     p->keybits = bits_for(p->nR > 1 ? p->nR : 2) + p->jbits;
-    if (p->keybits > 63) {
-        const int kb = p->keybits;
-        delete p;
-        set_error("pair keys need %d bits", kb);
-        return TJ_ERR_UNSUPPORTED;
-    }
+    TJ_CHECK(p->keybits <= 63, TJ_ERR_UNSUPPORTED, "pair keys need %d bits", p->keybits);
     size_t cur = 0;
     p->off_maxabs = carve(cur, 256);
     p->off_params = carve(cur, sizeof(JoinParams));
     if (method == TJ_METHOD_CERTIFIED) {
-        if (p->prec < P_FP16 || p->prec > P_FP8X3) { delete p; set_error("unknown precision %d", precision); return TJ_ERR_INVALID; }
+        TJ_CHECK(p->prec >= P_FP16 && p->prec <= P_FP8X3, TJ_ERR_INVALID, "unknown precision %d", precision);
         p->dp = padded_dim(d, p->prec);
         const int es = prec_esize(p->prec);
-        if (p->dp < 0 || (prec_terms(p->prec) == 3 && !prec_is_fp8(p->prec) && p->dp > 256)) {
-            delete p;
-            set_error("the screen serves d <= 512 (d <= 256 for a 16-bit hi/lo split); d=%d", d);
-            return TJ_ERR_UNSUPPORTED;
-        }
+        TJ_CHECK(p->dp > 0 && !(prec_terms(p->prec) == 3 && !prec_is_fp8(p->prec) && p->dp > 256), TJ_ERR_UNSUPPORTED,
+                 "the screen serves d <= 512 (d <= 256 for a 16-bit hi/lo split); d=%d", d);
         const bool x3 = prec_terms(p->prec) == 3;
         p->off_metaR = carve(cur, sizeof(RowMeta) * (size_t)nR);
         p->off_qRh = carve(cur, (size_t)nR * p->dp * es);
@@ -205,38 +215,14 @@ int tj_plan_create(tj_plan** out, int64_t nR, int64_t nS, int d, int method, int
         }
         p->off_amb = carve(cur, 8 * (size_t)(p->amb_cap > 0 ? p->amb_cap : 1));
         p->off_aux = carve(cur, 4 * (size_t)(p->amb_cap > 0 ? p->amb_cap : 1));
-        p->bm = screen_rows_per_wg(p->prec, p->dp, p->self_join);
-        if (p->bm <= 0) { delete p; return TJ_ERR_UNSUPPORTED; }
-        p->panels = (int)((nR + p->bm - 1) / p->bm);
-        const int maxg = screen_max_grid(p->prec, p->dp, p->self_join);
-        if (p->self_join) {
-            // Column splits of the triangle's rows (panel p sweeps columns
-            // from its first row on, split by split; splits past nS are empty
-            // items): enough items to balance the persistent grid when a
-            // small join has few panels, none narrower than 1024 columns.
-            const int64_t want = (8ll * maxg + std::max(p->panels, 1) - 1) / std::max(p->panels, 1);
-            const int64_t by_width = (p->nS + 1023) / 1024;
-            const int64_t cs = std::max<int64_t>(1, std::min(want, by_width));
-            const int64_t per = ((p->nS + cs - 1) / cs + 31) / 32 * 32;
-            p->cols_per_split = (int)std::max<int64_t>(per, 32);
-            p->col_splits = (int)((p->nS + p->cols_per_split - 1) / p->cols_per_split);
-        } else {
-            // Column splits of about 4 MiB of quantized S (one XCD's L2),
-            // items ordered split-major: the workgroups running at any time
-            // sweep the same chunk of S, which then stays in L2, instead of
-            // each panel streaming all of S from memory.  Also enough items
-            // to balance the persistent grid when there are few panels.
-            const int64_t row_bytes = (int64_t)p->dp * es * (x3 ? 2 : 1);
-            const int64_t by_l2 = std::max<int64_t>(1, (p->nS * row_bytes + (4ll << 20) - 1) / (4ll << 20));
-            const int64_t want = (4ll * maxg + std::max(p->panels, 1) - 1) / std::max(p->panels, 1);
-            const int64_t by_width = (p->nS + 1023) / 1024;  // splits of at least 1024 columns
-            const int64_t cs = std::max<int64_t>(1, std::min(std::max(by_l2, want), by_width));
-            int64_t per = ((p->nS + cs - 1) / cs + 31) / 32 * 32;
-            p->cols_per_split = (int)std::max<int64_t>(per, 32);
-            p->col_splits = (int)((p->nS + p->cols_per_split - 1) / p->cols_per_split);
-        }
-        const int64_t items = (int64_t)p->panels * p->col_splits;
-        p->grid = (int)std::max<int64_t>(1, std::min<int64_t>(items, maxg));
+        ScreenLayout sl;
+        int rc;
+        if ((rc = screen_layout(nR, p->nS, p->self_join, p->dp, p->prec, &sl))) return rc;
+        p->bm = sl.bm;
+        p->panels = sl.panels;
+        p->col_splits = sl.col_splits;
+        p->cols_per_split = sl.cols_per_split;
+        p->grid = sl.grid;
     }
     if (method == TJ_METHOD_GRID) {
         p->glay = grid_layout(p->nR, p->nS, p->out_cap);
@@ -248,6 +234,23 @@ int tj_plan_create(tj_plan** out, int64_t nR, int64_t nS, int d, int method, int
     }
     p->total = (cur + 255) / 256 * 256;
     p->timing = false;
+    p->has_graph = false;
+    return TJ_OK;
+}
+
+extern "C" {
+
+int tj_plan_create(tj_plan** out, int64_t nR, int64_t nS, int d, int method, int precision, int64_t out_capacity,
+                   int64_t amb_capacity, unsigned flags) {
+    TJ_CHECK(out != nullptr, TJ_ERR_INVALID, "plan pointer is NULL");
+    *out = nullptr;
+    tj_plan* p = new (std::nothrow) tj_plan();
+    TJ_CHECK(p != nullptr, TJ_ERR_INVALID, "out of host memory");
+    const int rc = plan_init(p, nR, nS, d, method, precision, out_capacity, amb_capacity, flags);
+    if (rc != TJ_OK) {
+        delete p;
+        return rc;
+    }
     if (flags & TJ_FLAG_TIMING) {
         for (int k = 0; k < 5; ++k) {
             if (hipEventCreate(&p->ev[k]) != hipSuccess) {
@@ -259,7 +262,6 @@ int tj_plan_create(tj_plan** out, int64_t nR, int64_t nS, int d, int method, int
         }
         p->timing = true;
     }
-    p->has_graph = false;
     *out = p;
     return TJ_OK;
 }
@@ -297,8 +299,10 @@ int tj_plan_stage_times(tj_plan* p, float* ms, int n) {
     return TJ_OK;
 }
 
-static int join_check(const tj_plan* p, const float* R, const float* S, double T, void* workspace, void* pairs,
-                      int out_format, uint64_t* stats) {
+}  // extern "C"
+
+int tj::join_check(const tj_plan* p, const float* R, const float* S, double T, void* workspace, void* pairs,
+                   int out_format, uint64_t* stats) {
     TJ_CHECK(p != nullptr, TJ_ERR_INVALID, "plan is NULL");
     TJ_CHECK(workspace != nullptr && stats != nullptr, TJ_ERR_INVALID, "workspace and stats are required");
     TJ_CHECK(R != nullptr || p->nR == 0, TJ_ERR_INVALID, "R is NULL");
@@ -311,19 +315,8 @@ static int join_check(const tj_plan* p, const float* R, const float* S, double T
     return TJ_OK;
 }
 
-struct JoinCall {
-    tj_plan* p;
-    const float* R;
-    const float* S;
-    double T;
-    void* workspace;
-    void* pairs;
-    int out_format;
-    uint64_t* stats;
-    bool marks;
-};
 
-static int join_enqueue(const JoinCall& c, hipStream_t s) {
+int tj::join_enqueue(const JoinCall& c, hipStream_t s) {
     tj_plan* p = c.p;
     const float* R = c.R;
     const float* S = c.S;
@@ -402,6 +395,8 @@ static int join_enqueue(const JoinCall& c, hipStream_t s) {
     return TJ_OK;
 }
 
+extern "C" {
+
 int tj_join(tj_plan* p, const float* R, const float* S, double T, void* workspace, void* pairs, int out_format,
             uint64_t* stats, void* stream) {
     int rc = join_check(p, R, S, T, workspace, pairs, out_format, stats);
@@ -454,13 +449,88 @@ int tj_rowmeta(const float* x, int64_t n, int d, int precision, const void* para
                           stats, (hipStream_t)stream);
 }
 
-int tj_refine_fp64(const float* R, const float* S, int d, int jbits, double T, const uint64_t* amb_keys,
-                   const float* amb_aux, int64_t amb_capacity, const void* params, uint64_t* out_keys,
-                   int64_t out_capacity, uint64_t* stats, void* stream) {
-    (void)T;  // the threshold travels in the parameter block
-    TJ_CHECK(R && S && amb_keys && amb_aux && params && out_keys && stats, TJ_ERR_INVALID, "NULL argument");
-    return launch_refine(R, S, d, jbits, amb_keys, amb_aux, amb_capacity, (const JoinParams*)params, out_keys,
-                         out_capacity, stats, (hipStream_t)stream);
+int tj_rowmeta_sizes(int64_t n, int d, int precision, size_t* q_bytes, size_t* meta_bytes) {
+    TJ_CHECK(q_bytes && meta_bytes, TJ_ERR_INVALID, "NULL argument");
+    TJ_CHECK(n >= 0 && d >= 1, TJ_ERR_INVALID, "bad shape n=%lld d=%d", (long long)n, d);
+    TJ_CHECK(precision >= P_FP16 && precision <= P_FP8X3, TJ_ERR_INVALID, "unknown precision %d", precision);
+    const int dp = padded_dim(d, precision);
+    TJ_CHECK(dp > 0, TJ_ERR_UNSUPPORTED, "d=%d not served at precision %d", d, precision);
+    *q_bytes = (size_t)n * dp * prec_esize(precision);
+    *meta_bytes = sizeof(RowMeta) * (size_t)n;
+    return TJ_OK;
+}
+
+int tj_screen(int precision, int d, const void* qa_hi, const void* qa_lo, const float* meta_a, int64_t na,
+              const void* qb_hi, const void* qb_lo, const float* meta_b, int64_t nb, const void* params,
+              uint64_t* out_keys, int64_t out_capacity, uint64_t* cand_keys, float* cand_s, int64_t cand_capacity,
+              unsigned flags, uint64_t* stats, void* stream) {
+    const bool self_join = (nb == 0);
+    TJ_CHECK(precision >= P_FP16 && precision <= P_FP8X3, TJ_ERR_INVALID, "unknown precision %d", precision);
+    TJ_CHECK(na >= 0 && nb >= 0 && na < (1ll << 31) && nb < (1ll << 31), TJ_ERR_INVALID, "bad sizes");
+    TJ_CHECK(out_capacity >= 0 && out_capacity < (1ll << 32) && cand_capacity >= 0 && cand_capacity < (1ll << 32),
+             TJ_ERR_INVALID, "capacities must be in [0, 2^32)");
+    TJ_CHECK(params && stats && (out_keys || out_capacity == 0) && ((cand_keys && cand_s) || cand_capacity == 0),
+             TJ_ERR_INVALID, "NULL argument");
+    TJ_CHECK(na == 0 || (qa_hi && meta_a), TJ_ERR_INVALID, "rows of A are NULL");
+    TJ_CHECK(self_join || (qb_hi && meta_b), TJ_ERR_INVALID, "rows of B are NULL");
+    const int x3 = prec_terms(precision) == 3;
+    TJ_CHECK(!x3 || na == 0 || (qa_lo && (self_join || qb_lo)), TJ_ERR_INVALID, "hi/lo precision needs the lo codes");
+    const int dp = padded_dim(d, precision);
+    TJ_CHECK(dp > 0 && (!x3 || prec_is_fp8(precision) || dp <= 256), TJ_ERR_UNSUPPORTED,
+             "d=%d not served at precision %d", d, precision);
+    const int64_t nB = self_join ? na : nb;
+    if (na == 0 || nB == 0) return TJ_OK;
+    ScreenLayout sl;
+    int rc;
+    if ((rc = screen_layout(na, nB, self_join, dp, precision, &sl))) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    ScreenArgs a{};
+    a.qa_hi = qa_hi; a.qa_lo = x3 ? qa_lo : qa_hi; a.ma = (const RowMeta*)meta_a; a.na = na;
+    if (self_join) {
+        a.qb_hi = a.qa_hi; a.qb_lo = a.qa_lo; a.mb = a.ma;
+    } else {
+        a.qb_hi = qb_hi; a.qb_lo = x3 ? qb_lo : qb_hi; a.mb = (const RowMeta*)meta_b;
+    }
+    a.nb = nB;
+    a.params = (const JoinParams*)params;
+    a.out_keys = out_keys; a.out_cap = out_capacity;
+    a.amb_keys = cand_keys; a.amb_aux = cand_s; a.amb_cap = cand_capacity;
+    a.stats = stats;
+    a.jbits = bits_for(nB > 1 ? nB : 2);
+    a.panels = sl.panels; a.col_splits = sl.col_splits; a.cols_per_split = sl.cols_per_split;
+    a.audit = (flags & TJ_FLAG_AUDIT) ? 1 : 0;
+    // the persistent grid takes its items from a cursor that starts at zero
+    if ((rc = launch_fill_u64(stats + TJ_ST_WORK, 1, 0, s))) return rc;
+    return launch_screen(precision, dp, self_join, a, sl.grid, s);
+}
+
+int tj_certify(const float* R, const float* S, int d, int64_t nS, const uint64_t* cand_keys, const float* cand_s,
+               int64_t cand_capacity, const float* meta_R, const float* meta_S, const void* params, uint64_t* out_keys,
+               int64_t out_capacity, unsigned flags, uint64_t* stats, void* stream) {
+    TJ_CHECK(R && meta_R && params && stats && (out_keys || out_capacity == 0) &&
+                 ((cand_keys && cand_s) || cand_capacity == 0),
+             TJ_ERR_INVALID, "NULL argument");
+    TJ_CHECK(nS >= 0 && nS < (1ll << 31) && d >= 1, TJ_ERR_INVALID, "bad shape");
+    TJ_CHECK(cand_capacity >= 0 && cand_capacity < (1ll << 32) && out_capacity >= 0 && out_capacity < (1ll << 32),
+             TJ_ERR_INVALID, "capacities must be in [0, 2^32)");
+    const bool self_join = (S == nullptr);
+    TJ_CHECK(self_join || meta_S, TJ_ERR_INVALID, "meta_S is NULL for an R x S join");
+    return launch_certify(R, self_join ? R : S, d, bits_for(nS > 1 ? nS : 2), cand_keys, cand_s, cand_capacity,
+                          (const RowMeta*)meta_R, (const RowMeta*)(self_join ? meta_R : meta_S),
+                          (const JoinParams*)params, out_keys, out_capacity, stats, (flags & TJ_FLAG_AUDIT) ? 1 : 0,
+                          (hipStream_t)stream);
+}
+
+int tj_refine_fp64(const float* R, const float* S, int d, int jbits, double T, const uint64_t* keys,
+                   const uint64_t* count, int64_t capacity, uint64_t* out_keys, int64_t out_capacity, uint64_t* stats,
+                   void* stream) {
+    TJ_CHECK(R && keys && count && stats && (out_keys || out_capacity == 0), TJ_ERR_INVALID, "NULL argument");
+    TJ_CHECK(d >= 1 && jbits >= 1 && jbits <= 32, TJ_ERR_INVALID, "bad d=%d or jbits=%d", d, jbits);
+    TJ_CHECK(T == T && T >= 0.0, TJ_ERR_INVALID, "T must be >= 0");
+    TJ_CHECK(capacity >= 0 && capacity < (1ll << 32) && out_capacity >= 0 && out_capacity < (1ll << 32),
+             TJ_ERR_INVALID, "capacities must be in [0, 2^32)");
+    return launch_refine(R, S ? S : R, d, jbits, T, keys, count, capacity, out_keys, out_capacity, stats,
+                         (hipStream_t)stream);
 }
 
 int tj_plain_fp64(const float* R, int64_t nR, const float* S, int64_t nS, int d, int jbits, double T, uint64_t* out_keys,

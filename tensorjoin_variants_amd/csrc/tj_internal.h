@@ -235,14 +235,18 @@ struct ScreenArgs {
 int launch_screen(int prec, int dp, bool self_join, const ScreenArgs& a, int grid, hipStream_t s);
 int screen_rows_per_wg(int prec, int dp, bool self_join);
 int screen_max_grid(int prec, int dp, bool self_join);
+// Work decomposition of the persistent screen over an nA x nB join: row
+// panels of bm rows, column splits, and the persistent grid.
+struct ScreenLayout { int bm, panels, col_splits, cols_per_split, grid; };
+int screen_layout(int64_t nA, int64_t nB, bool self_join, int dp, int prec, ScreenLayout* L);
 
 int launch_scale_scan(const float* x, int64_t n, int d, uint32_t* maxabs, uint64_t* stats, hipStream_t s);
 int launch_params(const uint32_t* maxabs, double T, int d, int dp, int prec, JoinParams* p, uint64_t* stats, hipStream_t s);
 int launch_rowmeta(const float* x, int64_t n, int d, int dp, int prec, const JoinParams* p,
                    void* qhi, void* qlo, RowMeta* meta, uint64_t* stats, hipStream_t s);
-int launch_refine(const float* R, const float* S, int d, int jbits, const uint64_t* amb_keys,
-                  const float* amb_aux, int64_t amb_cap, const JoinParams* p, uint64_t* out_keys,
-                  int64_t out_cap, uint64_t* stats, hipStream_t s);
+int launch_refine(const float* R, const float* S, int d, int jbits, double T, const uint64_t* keys,
+                  const uint64_t* count, int64_t cap, uint64_t* out_keys, int64_t out_cap, uint64_t* stats,
+                  hipStream_t s);
 int launch_certify(const float* R, const float* S, int d, int jbits, const uint64_t* cand_keys,
                    const float* cand_p, int64_t cand_cap, const RowMeta* ma, const RowMeta* mb,
                    const JoinParams* p, uint64_t* out_keys, int64_t out_cap, uint64_t* stats, int audit,

@@ -160,3 +160,32 @@ def test_host_join_rejects_null_handle(L):
     assert L.tj_host_join(None, None, None, 1.0, 0, C.byref(res), C.byref(cnt), None) == -1
     assert L.tj_host_destroy(None) == 0
     L.tj_host_free(None)
+
+
+@pytest.mark.parametrize("n,d,prec,q,m", [(1000, 128, "fp16", 1000 * 128 * 2, 48000), (10, 3, "fp8", 10 * 64, 480),
+                                          (7, 100, "bf16x3", 7 * 128 * 2, 336), (0, 16, "fp16", 0, 0)])
+def test_rowmeta_sizes(L, n, d, prec, q, m):
+    qb, mb = C.c_size_t(), C.c_size_t()
+    assert L.tj_rowmeta_sizes(n, d, _lib.PREC[prec], C.byref(qb), C.byref(mb)) == 0
+    assert (qb.value, mb.value) == (q, m)
+    assert L.tj_rowmeta_sizes(n, 600, _lib.PREC["fp16"], C.byref(qb), C.byref(mb)) == -3
+    assert L.tj_rowmeta_sizes(n, d, 0, C.byref(qb), C.byref(mb)) == -1
+
+
+def test_stage_calls_reject_bad_arguments(L):
+    """Argument checks run before anything touches the device."""
+    one = C.c_void_p(1)
+    # tj_screen: unknown precision, NULL params, capacity beyond 2^32
+    assert L.tj_screen(9, 8, one, None, one, 10, one, None, one, 10, one, one, 10, one, one, 10, 0, one, None) == -1
+    assert L.tj_screen(1, 8, one, None, one, 10, one, None, one, 10, None, one, 10, one, one, 10, 0, one, None) == -1
+    assert L.tj_screen(1, 8, one, None, one, 10, one, None, one, 10, one, one, 1 << 33, one, one, 10, 0, one,
+                       None) == -1
+    # hi/lo precision without lo codes
+    assert L.tj_screen(2, 8, one, None, one, 10, one, None, one, 10, one, one, 10, one, one, 10, 0, one, None) == -1
+    # tj_certify: NULL stats, R x S without meta_S
+    assert L.tj_certify(one, one, 8, 10, one, one, 10, one, one, one, one, 10, 0, None, None) == -1
+    assert L.tj_certify(one, one, 8, 10, one, one, 10, one, None, one, one, 10, 0, one, None) == -1
+    # tj_refine_fp64: NULL count, negative T, jbits out of range
+    assert L.tj_refine_fp64(one, None, 8, 6, 1.0, one, None, 10, one, 10, one, None) == -1
+    assert L.tj_refine_fp64(one, None, 8, 6, -1.0, one, one, 10, one, 10, one, None) == -1
+    assert L.tj_refine_fp64(one, None, 8, 40, 1.0, one, one, 10, one, 10, one, None) == -1
