@@ -120,6 +120,23 @@ int tj_plan_stage_times(tj_plan* plan, float* ms, int n);
 int tj_join(tj_plan* plan, const float* R, const float* S, double T, void* workspace,
             void* pairs, int out_format, uint64_t* stats, void* stream);
 
+/* tj_join without any allocation, for callers that cannot allocate device
+ * memory: code under hipGraph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  No counterpart in the reference.  The arguments are
+ * those of tj_plan_create and tj_join together (nS == 0: self-join of R,
+ * S ignored); the caller passes only inputs and outputs.  The workspace a
+ * tj_plan of the same arguments would need comes from a device array of
+ * TJ_NOALLOC_SCRATCH_BYTES that the library holds on each GPU until the
+ * process ends; a join needing more (tj_plan_workspace_bytes of that plan)
+ * returns TJ_ERR_WORKSPACE.  Flags: TJ_FLAG_AUDIT only.  Every call on one
+ * GPU shares that GPU's array: calls that overlap on one GPU (two streams or
+ * threads at once) race and must be serialised by the caller.  Capacity
+ * overflow is reported in stats as for tj_join. */
+#define TJ_NOALLOC_SCRATCH_BYTES (64u << 20)
+int tj_join_noalloc(const float* R, int64_t nR, const float* S, int64_t nS, int d, int method, int precision,
+                    double T, int64_t out_capacity, int64_t amb_capacity, unsigned flags, void* pairs,
+                    int out_format, uint64_t* stats, void* stream);
+
 /* hipGraph form of tj_join.  tj_plan_capture (synchronous) records the whole
  * tj_join sequence for these buffers into a graph owned by the plan;
  * tj_plan_replay launches it on `stream`.  No memset nodes: every clear is

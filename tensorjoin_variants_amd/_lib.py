@@ -21,6 +21,7 @@ PREC = {"none": 0, "fp16": 1, "fp16x3": 2, "bf16": 3, "bf16x3": 4, "fp8": 5, "fp
 METHOD = {"certified": 0, "plain_fp64": 1, "grid": 2}
 OUT = {"int32": 0, "int64": 1}
 OUT_U64_CANONICAL = 2
+NOALLOC_SCRATCH_BYTES = 64 << 20
 FLAG_AUDIT = 1
 FLAG_TIMING = 2
 FLAG_PRUNE = 4
@@ -74,6 +75,7 @@ def load():
         "tj_unpack_pairs": (i32, [vp, vp, i64, i32, vp, i32, vp]),
         "tj_mfma_probe": (i32, [i32, vp, vp, vp, vp, vp, i32, i32, vp, vp]),
         "tj_canonical_u64": (i32, [vp, C.POINTER(sz), vp, vp, i64, i64, vp, vp]),
+        "tj_join_noalloc": (i32, [vp, i64, vp, i64, i32, i32, i32, f64, i64, i64, C.c_uint, vp, i32, vp, vp]),
         "tj_plan_capture": (i32, [vp, vp, vp, f64, vp, vp, i32, vp]),
         "tj_plan_replay": (i32, [vp, vp]),
         "tj_batch_create": (i32, [C.POINTER(vp), i32, C.POINTER(i64), C.POINTER(i32), C.POINTER(i32), i64, i64,
