@@ -260,6 +260,27 @@ int tj_plain_fp64(const float* R, int64_t nR, const float* S, int64_t nS, int d,
                   double T, uint64_t* out_keys, int64_t out_capacity, uint64_t* stats,
                   void* stream);
 
+/* The uniform-grid join (d <= 3) as one stage, with its own workspace
+ * (tj_grid_workspace_bytes; capacity = rows of the output).  tj_grid_join:
+ * R x S (nS == 0: self-join of R, i < j) -> sorted (i, j) rows in
+ * out_format, count stats[TJ_ST_OUT] (rows past capacity counted, not
+ * written); stats is cleared first and receives TJ_ST_OUT, TJ_ST_GRID_CAND,
+ * TJ_ST_INVALID, TJ_ST_PAIRS.  Replaces GDS-Join's GPU join
+ * (GDS/main.cu:690-950, GDS/kernel.cu:132-486, GDS/GPU.cu:775-872) with the
+ * predicate d2_seq <= T of this library. */
+int tj_grid_workspace_bytes(int64_t nR, int64_t nS, int64_t capacity, size_t* bytes);
+int tj_grid_join(const float* R, int64_t nR, const float* S, int64_t nS, int d, double T, void* workspace,
+                 size_t workspace_bytes, int64_t capacity, void* pairs, int out_format, uint64_t* stats, void* stream);
+/* GDS-Join's output (GDSJoinPy + copyResultIntoPythonArray, GDS/main.cu:360,
+ * 507): for every point i of X, counts[i] = the number of points j (i
+ * itself included) with d2_seq(X_i, X_j) <= T, and table = their ids grouped
+ * by i in ascending i, each group ascending; total = stats[TJ_ST_OUT] (ids
+ * past capacity counted, not written).  Workspace: tj_grid_workspace_bytes
+ * (n, 0, capacity).  (GDS-Join decides sqrt(d2) <= eps in its own summation
+ * order and leaves each group unordered.) */
+int tj_grid_neighbors(const float* X, int64_t n, int d, double T, void* workspace, size_t workspace_bytes,
+                      int64_t capacity, uint32_t* counts, uint32_t* table, uint64_t* stats, void* stream);
+
 /* Ascending LSD radix sort of `count` (device scalar) uint64 keys over the
  * low `bits` bits.  Two-phase scratch query as output_sort
  * (OA/src/output.cu:26-31): call with temp == NULL to get temp_bytes.

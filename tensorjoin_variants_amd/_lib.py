@@ -72,6 +72,9 @@ def load():
         "tj_certify": (i32, [vp, vp, i32, i64, vp, vp, i64, vp, vp, vp, vp, i64, C.c_uint, vp, vp]),
         "tj_plain_fp64": (i32, [vp, i64, vp, i64, i32, i32, f64, vp, i64, vp, vp]),
         "tj_sort_keys": (i32, [vp, C.POINTER(sz), vp, vp, i64, i32, vp]),
+        "tj_grid_workspace_bytes": (i32, [i64, i64, i64, C.POINTER(sz)]),
+        "tj_grid_join": (i32, [vp, i64, vp, i64, i32, f64, vp, sz, i64, vp, i32, vp, vp]),
+        "tj_grid_neighbors": (i32, [vp, i64, i32, f64, vp, sz, i64, vp, vp, vp, vp]),
         "tj_unpack_pairs": (i32, [vp, vp, i64, i32, vp, i32, vp]),
         "tj_mfma_probe": (i32, [i32, vp, vp, vp, vp, vp, i32, i32, vp, vp]),
         "tj_canonical_u64": (i32, [vp, C.POINTER(sz), vp, vp, i64, i64, vp, vp]),

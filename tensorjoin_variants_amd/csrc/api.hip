@@ -24,10 +24,6 @@ int screen_fp8(int dp, bool self_join, const ScreenArgs& a, int grid, hipStream_
 int screen_fp8x3(int dp, bool self_join, const ScreenArgs& a, int grid, hipStream_t s, int query);
 int launch_mirror(const int64_t* pairs, const uint64_t* count, int64_t cap, int64_t n, uint64_t* ids, uint64_t* total,
                   hipStream_t s);
-GridLayout grid_layout(int64_t nR, int64_t nS, int64_t out_cap);
-int launch_grid_join(const float* R, int64_t nR, const float* S, int64_t nS, int d, double T, bool self_join,
-                     uint8_t* ws, const GridLayout& L, int64_t out_cap, void* pairs, int out_format, uint64_t* stats,
-                     hipStream_t s);
 
 static thread_local char g_err[512] = "";
 
@@ -373,8 +369,8 @@ int tj::join_enqueue(const JoinCall& c, hipStream_t s) {
     } else if (p->method == TJ_METHOD_GRID) {
         // (the grid's bounds pass counts the non-finite coordinates)
         TJ_MARK(1);
-        if ((rc = launch_grid_join(R, p->nR, S, p->nS, p->d, T, p->self_join, ws + p->off_grid, p->glay, p->out_cap,
-                                   pairs, out_format, stats, s)))
+        if ((rc = launch_grid_join(R, p->nR, S, p->nS, p->d, T, p->self_join, p->self_join, ws + p->off_grid, p->glay,
+                                   p->out_cap, out_format, pairs, nullptr, stats, s)))
             return rc;
         TJ_MARK(2);
         TJ_MARK(3);
@@ -539,6 +535,49 @@ int tj_plain_fp64(const float* R, int64_t nR, const float* S, int64_t nS, int d,
     const bool self_join = (S == nullptr);
     return launch_plain(R, nR, self_join ? R : S, self_join ? nR : nS, d, jbits, T, self_join, out_keys, out_capacity,
                         stats, (hipStream_t)stream);
+}
+
+int tj_grid_workspace_bytes(int64_t nR, int64_t nS, int64_t capacity, size_t* bytes) {
+    TJ_CHECK(bytes != nullptr, TJ_ERR_INVALID, "bytes is NULL");
+    TJ_CHECK(nR >= 0 && nS >= 0 && nR < (1ll << 31) && nS < (1ll << 31), TJ_ERR_INVALID, "bad sizes");
+    TJ_CHECK(capacity >= 0 && capacity < (1ll << 32), TJ_ERR_INVALID, "capacity must be in [0, 2^32)");
+    *bytes = grid_layout(nR, nS == 0 ? nR : nS, capacity).total;
+    return TJ_OK;
+}
+
+static int grid_stage(const float* R, int64_t nR, const float* S, int64_t nS, int d, double T, bool upper,
+                      void* workspace, size_t workspace_bytes, int64_t capacity, int emit, void* out, uint32_t* counts,
+                      uint64_t* stats, hipStream_t s) {
+    TJ_CHECK(d >= 1 && d <= 3, TJ_ERR_UNSUPPORTED, "the grid serves d <= 3 (d=%d)", d);
+    TJ_CHECK(nR >= 0 && nS >= 0 && nR < (1ll << 31) && nS < (1ll << 31), TJ_ERR_INVALID, "bad sizes");
+    TJ_CHECK(capacity >= 0 && capacity < (1ll << 32), TJ_ERR_INVALID, "capacity must be in [0, 2^32)");
+    TJ_CHECK(T == T && T >= 0.0 && T < INFINITY, TJ_ERR_INVALID, "T must be finite and >= 0");
+    TJ_CHECK(stats && workspace && (R || nR == 0) && (out || capacity == 0), TJ_ERR_INVALID, "NULL argument");
+    const bool self_join = (nS == 0);
+    TJ_CHECK(self_join || S, TJ_ERR_INVALID, "S is NULL for an R x S join");
+    const GridLayout L = grid_layout(nR, self_join ? nR : nS, capacity);
+    TJ_CHECK(workspace_bytes >= L.total, TJ_ERR_WORKSPACE, "grid workspace needs %zu bytes", L.total);
+    int rc;
+    if ((rc = launch_fill_u64(stats, TJ_STATS_SLOTS, 0, s))) return rc;
+    const int64_t nB = self_join ? nR : nS;
+    const int64_t pairs_total = (self_join && upper) ? nR * (nR - 1) / 2 : nR * nB;
+    if ((rc = launch_fill_u64(stats + TJ_ST_PAIRS, 1, (uint64_t)pairs_total, s))) return rc;
+    return launch_grid_join(R, nR, self_join ? R : S, nB, d, T, self_join, upper, (uint8_t*)workspace, L, capacity,
+                            emit, out, counts, stats, s);
+}
+
+int tj_grid_join(const float* R, int64_t nR, const float* S, int64_t nS, int d, double T, void* workspace,
+                 size_t workspace_bytes, int64_t capacity, void* pairs, int out_format, uint64_t* stats, void* stream) {
+    TJ_CHECK(out_format == TJ_OUT_I32 || out_format == TJ_OUT_I64, TJ_ERR_INVALID, "bad out_format %d", out_format);
+    return grid_stage(R, nR, S, nS, d, T, nS == 0, workspace, workspace_bytes, capacity, out_format, pairs, nullptr,
+                      stats, (hipStream_t)stream);
+}
+
+int tj_grid_neighbors(const float* X, int64_t n, int d, double T, void* workspace, size_t workspace_bytes,
+                      int64_t capacity, uint32_t* counts, uint32_t* table, uint64_t* stats, void* stream) {
+    TJ_CHECK(counts || n == 0, TJ_ERR_INVALID, "counts is NULL");
+    return grid_stage(X, n, nullptr, 0, d, T, false, workspace, workspace_bytes, capacity, GRID_EMIT_TABLE, table,
+                      counts, stats, (hipStream_t)stream);
 }
 
 int tj_sort_keys(void* temp, size_t* temp_bytes, uint64_t* keys, const uint64_t* count, int64_t capacity, int bits,

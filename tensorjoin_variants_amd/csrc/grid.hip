@@ -6,7 +6,7 @@
 // Same index idea -- cells of side >= eps, a query only visits its own and
 // the adjacent cells -- rebuilt for MI355X:
 //  * the index is built on the device: bounds (order-preserving integer
-//    atomics), cell keys, my radix sort by cell, S gathered into cell order
+//    atomics), cell keys, the radix sort by cell, S gathered into cell order
 //    as 16-byte rows, and an open-addressing hash of the non-empty cells
 //    (memory bounded by 2|S| slots however sparse the grid is);
 //  * when the grid has at most 64 |S| cells, no hash at all: a bitmap of
@@ -21,17 +21,20 @@
 //    rows too; a self-join reuses S's), so the 64 lanes of a wave visit the
 //    same neighbour cells and the same S rows, and read their queries,
 //    counts and offsets contiguously;
-//  * output: one probe pass stages each wave's pair keys in LDS (an LDS
-//    atomic per pair, one global atomic per flush of up to 1024 keys) and
-//    one radix sort of the keys over all their bits orders them (a count
-//    pass, scan and exact-offset write pass probed everything twice; a
-//    per-row insertion sort was quadratic in the longest row);
+//  * output without a global sort: a count pass stores each query's number
+//    of neighbours, an exclusive scan turns the counts into row offsets, a
+//    write pass stores every neighbour id in its row's final range, and a
+//    per-row sort orders each row (by its lane, its wave or, for rows beyond
+//    2,048 ids, its workgroup) while it emits the (i, j) rows -- the same
+//    shape as GDS-Join's neighbour table (counts per point + ids grouped by
+//    point, GDS/main.cu:507-540), which the table output gives directly;
 //  * the predicate is the sequential binary64 difference form of the CPU
 //    oracle (d2 <= T; GDS used sqrt(d2) <= eps, GDS/kernel.cu:368-371).
 // The cell side is sqrt(T) * (1 + 2^-10) (or larger when an axis would
 // span more than 2^38 cells or the cell ids would not fit a 64-bit key):
 // every pair with d2_seq <= T has all coordinate differences below one cell
 // side, so it lies in adjacent cells.
+#include <type_traits>
 #include "tj_internal.h"
 
 namespace {
@@ -204,12 +207,16 @@ __global__ void __launch_bounds__(256) clear_bitmap_kernel(const GridParams* __r
 }
 
 __device__ inline void cell_of(const float* x, const GridParams& g, int64_t c[3]) {
-    for (int k = 0; k < 3; ++k) c[k] = 0;
-    for (int k = 0; k < g.d; ++k) {
-        int64_t v = (int64_t)floor(((double)x[k] - g.lo[k]) * g.inv_side);
-        if (v < 1) v = 1;
-        if (v > (int64_t)g.ncell[k] - 2) v = (int64_t)g.ncell[k] - 2;
-        c[k] = v;
+    // unrolled over the three axes (static indices keep c and x in registers)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        c[k] = 0;
+        if (k < g.d) {
+            int64_t v = (int64_t)floor(((double)x[k] - g.lo[k]) * g.inv_side);
+            if (v < 1) v = 1;
+            if (v > (int64_t)g.ncell[k] - 2) v = (int64_t)g.ncell[k] - 2;
+            c[k] = v;
+        }
     }
 }
 
@@ -400,77 +407,73 @@ __device__ inline uint2 lookup(const uint64_t* __restrict__ hkey, const uint2* _
     }
 }
 
-__device__ inline double d2_row(const float* q, const float4& y, int d) {
+// The sequential binary64 difference form over D <= 3 coordinates.
+template <int D>
+__device__ inline double d2_row(const float* q, const float4& y) {
     double s = 0.0, t;
     t = (double)q[0] - (double)y.x; s = s + t * t;
-    if (d > 1) { t = (double)q[1] - (double)y.y; s = s + t * t; }
-    if (d > 2) { t = (double)q[2] - (double)y.z; s = s + t * t; }
+    if (D > 1) { t = (double)q[1] - (double)y.y; s = s + t * t; }
+    if (D > 2) { t = (double)q[2] - (double)y.z; s = s + t * t; }
     return s;
 }
 
+__device__ inline uint64_t shfl_u64(uint64_t v, int src) {
+    const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)v, src);
+    const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(v >> 32), src);
+    return ((uint64_t)hi << 32) | lo;
+}
+
 // One query per lane, queries in cell order (rrow: x, y, z, original index).
-// Each pair found is appended to this wave's LDS buffer (an LDS atomic per
-// pair); at the end of a neighbour row, a buffer at least half full is
-// flushed to out_keys with one atomic on the output count (a pair that
-// finds the buffer full goes to out_keys directly).  Entries past out_cap
-// are counted, not written; the key sort that follows orders the keys.
-constexpr int PCAP = 1024;  // staged keys per wave
+// The same traversal runs twice: the count pass stores each query's number
+// of neighbours at cnt[i] (and counts candidate evaluations); after an
+// exclusive scan of the counts, the write pass stores each neighbour id j
+// at outj[off[i] + k], k the query's running count, so every row's ids land
+// in their final range without any global atomic.  A row's ids arrive in
+// cell order, not sorted: the row sort kernels order each row afterwards.
+// Positions at or past out_cap are skipped (the caller retries larger).
+enum { PROBE_COUNT = 0, PROBE_WRITE = 1 };
+
+template <int MODE, int D>
 __global__ void __launch_bounds__(256) probe_kernel(const float4* __restrict__ rrow, int64_t nR, const GridParams* __restrict__ gp,
                                                     const float4* __restrict__ srow,
                                                     const uint64_t* __restrict__ hkey, const uint2* __restrict__ hval,
                                                     uint64_t hmask, const uint64_t* __restrict__ bm,
                                                     const uint32_t* __restrict__ bmpre, const uint32_t* __restrict__ runs,
-                                                    double T, int self_join, uint64_t* __restrict__ out_keys,
-                                                    int jbits, int64_t out_cap, uint64_t* __restrict__ stats) {
+                                                    double T, int upper, uint32_t* __restrict__ cnt,
+                                                    const uint64_t* __restrict__ off, uint32_t* __restrict__ outj,
+                                                    int64_t out_cap, uint64_t* __restrict__ stats) {
     const GridParams g = *gp;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    __shared__ uint64_t sBuf[4][PCAP];
-    __shared__ uint32_t sN[4];
-    if (lane == 0) sN[wave] = 0;
-    __builtin_amdgcn_wave_barrier();
+    // neighbours found for each lane's query by other lanes (wave-shared sweep)
+    __shared__ uint32_t sCnt[4][64];
+    // each lane's S range per neighbour row
+    __shared__ uint2 sRun[4][9][64];
+    sCnt[wave][lane] = 0;
     uint64_t cand = 0;
-    auto emit = [&](uint64_t key) {
-        const uint32_t pos = atomicAdd(&sN[wave], 1u);
-        if (pos < (uint32_t)PCAP) {
-            sBuf[wave][pos] = key;
-        } else {
-            const uint64_t at = atomicAdd((unsigned long long*)&stats[TJ_ST_OUT], 1ull);
-            if (at < (uint64_t)out_cap) out_keys[at] = key;
-        }
-    };
-    // all lanes: write the staged keys out and empty the buffer
-    auto flush = [&]() {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        uint32_t n = __builtin_amdgcn_readfirstlane(sN[wave]);
-        if (n > (uint32_t)PCAP) n = PCAP;
-        if (n) {
-            uint64_t base = 0;
-            if (lane == 0) base = atomicAdd((unsigned long long*)&stats[TJ_ST_OUT], (unsigned long long)n);
-            base = __shfl(base, 0);
-            for (uint32_t k = lane; k < n; k += 64)
-                if (base + k < (uint64_t)out_cap) out_keys[base + k] = sBuf[wave][k];
-        }
-        __builtin_amdgcn_wave_barrier();
-        if (lane == 0) sN[wave] = 0;
-        __builtin_amdgcn_wave_barrier();
-    };
     // whole waves step together (tb is the wave's first query), so the
-    // shared sweep and the flushes below always have all 64 lanes
+    // shared sweep always has all 64 lanes
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t tb = (int64_t)blockIdx.x * blockDim.x + wave * 64; tb < nR; tb += stride) {
         const int64_t t = tb + lane;
         const bool valid = t < nR;
         const float4 qr = valid ? rrow[t] : make_float4(0.f, 0.f, 0.f, 0.f);
-        const uint64_t i = __builtin_bit_cast(uint32_t, qr.w);
+        const uint32_t i = __builtin_bit_cast(uint32_t, qr.w);
         const float q[3] = {qr.x, qr.y, qr.z};
+        const uint64_t base = (MODE == PROBE_WRITE && valid) ? off[i] : 0;
+        uint32_t mine = 0;  // neighbours of this lane's query so far
         int64_t c[3];
         cell_of(q, g, c);
         auto visit = [&](const float4& y) {
             const uint32_t j = __builtin_bit_cast(uint32_t, y.w);
-            if (self_join && j <= i) return;
-            ++cand;
-            if (d2_row(q, y, g.d) <= T) emit((i << jbits) | j);
+            if (upper && j <= i) return;
+            if (MODE == PROBE_COUNT) ++cand;
+            if (d2_row<D>(q, y) <= T) {
+                if (MODE == PROBE_WRITE) {
+                    const uint64_t at = base + mine;
+                    if (at < (uint64_t)out_cap) outj[at] = j;
+                }
+                ++mine;
+            }
         };
         auto sweep = [&](uint2 run) {
             // eight rows in flight at a time
@@ -485,73 +488,443 @@ __global__ void __launch_bounds__(256) probe_kernel(const float4* __restrict__ r
             for (; p < run.y; ++p) visit(srow[p]);
         };
         const int r1 = g.d > 1 ? 1 : 0, r2 = g.d > 2 ? 1 : 0;
-        for (int dz = -r2; dz <= r2; ++dz)
-            for (int dy = -r1; dy <= r1; ++dy) {
-                // ids of the x-adjacent cells: mid - 1, mid, mid + 1 (cell_of
-                // keeps c in [1, ncell - 2], so all three exist)
-                const uint64_t mid = (uint64_t)c[0] + (uint64_t)(c[1] + dy) * g.mul[1] + (uint64_t)(c[2] + dz) * g.mul[2];
-                if (!g.use_bm) {
-                    // one hash run per cell
-                    if (valid)
-                        for (int dx = -1; dx <= 1; ++dx) sweep(lookup(hkey, hval, hmask, mid + (uint64_t)(int64_t)dx));
-                } else {
-                    // the S rows of the row's three cells: one range
-                    uint2 run = make_uint2(0, 0);
-                    if (valid) {
-                        const uint64_t b0 = mid - 1, b2 = mid + 1;
-                        const uint64_t w0 = bm[b0 >> 6];
-                        const uint64_t w2 = (b2 >> 6) == (b0 >> 6) ? w0 : bm[b2 >> 6];
-                        const uint32_t sh = (uint32_t)(b0 & 63);
-                        uint32_t occ = (uint32_t)((w0 >> sh) & 7);
-                        if (sh > 61) occ |= (uint32_t)(w2 << (64 - sh)) & 7;
-                        if (occ) {
-                            const uint32_t r0 =
-                                bmpre[b0 >> 6] + (uint32_t)__popcll(sh ? (w0 & ((1ull << sh) - 1)) : 0ull);
-                            run = make_uint2(runs[r0], runs[r0 + (uint32_t)__popc(occ)]);
-                        }
+        // ids of the x-adjacent cells of neighbour row (dy, dz): mid - 1,
+        // mid, mid + 1 (cell_of keeps c in [1, ncell - 2], so all exist)
+        auto row_mid = [&](int dy, int dz) {
+            return (uint64_t)c[0] + (uint64_t)(c[1] + dy) * g.mul[1] + (uint64_t)(c[2] + dz) * g.mul[2];
+        };
+        if (!g.use_bm) {
+            // sparse grid: one hash run per cell
+            if (valid)
+                for (int dz = -r2; dz <= r2; ++dz)
+                    for (int dy = -r1; dy <= r1; ++dy)
+                        for (int dx = -1; dx <= 1; ++dx)
+                            sweep(lookup(hkey, hval, hmask, row_mid(dy, dz) + (uint64_t)(int64_t)dx));
+            if (MODE == PROBE_COUNT && valid) cnt[i] = mine;
+            continue;
+        }
+        // The S rows of each neighbour row's three cells are one range.  The
+        // ranges of all (up to nine) rows are found first, with their loads
+        // in two independent rounds (bitmap word + its prefix count, then the
+        // run bounds) instead of nine dependent chains, and kept in LDS.
+        {
+            uint64_t w0[9];
+            uint32_t pre[9];
+#pragma unroll
+            for (int r = 0; r < 9; ++r) {
+                const int dz = r / 3 - 1, dy = r % 3 - 1;
+                const bool use = valid && (dz == 0 || r2) && (dy == 0 || r1);
+                const uint64_t b0 = row_mid(use ? dy : 0, use ? dz : 0) - 1;
+                w0[r] = use ? bm[b0 >> 6] : 0ull;
+                pre[r] = use ? bmpre[b0 >> 6] : 0u;
+            }
+#pragma unroll
+            for (int r = 0; r < 9; ++r) {
+                const int dz = r / 3 - 1, dy = r % 3 - 1;
+                const bool use = valid && (dz == 0 || r2) && (dy == 0 || r1);
+                const uint64_t b0 = row_mid(use ? dy : 0, use ? dz : 0) - 1, b2 = b0 + 2;
+                const uint32_t sh = (uint32_t)(b0 & 63);
+                uint32_t occ = use ? (uint32_t)((w0[r] >> sh) & 7) : 0u;
+                if (use && sh > 61 && (b2 >> 6) != (b0 >> 6)) occ |= (uint32_t)(bm[b2 >> 6] << (64 - sh)) & 7;
+                uint2 run = make_uint2(0, 0);
+                if (occ) {
+                    const uint32_t r0 = pre[r] + (uint32_t)__popcll(sh ? (w0[r] & ((1ull << sh) - 1)) : 0ull);
+                    run = make_uint2(runs[r0], runs[r0 + (uint32_t)__popc(occ)]);
+                }
+                sRun[wave][r][lane] = run;
+            }
+        }
+        for (int r = 0; r < 9; ++r) {
+            const int dz = r / 3 - 1, dy = r % 3 - 1;
+            if ((dz != 0 && !r2) || (dy != 0 && !r1)) continue;
+                const uint2 run = sRun[wave][r][lane];
+                const uint32_t len = run.y - run.x;
+                if (__ballot(len > 32) == 0) {
+                    sweep(run);
+                    continue;
+                }
+                // Long ranges (a dense blob).  Lanes of a wave hold queries of
+                // the same few cells, so their ranges mostly overlap: the wave
+                // walks the union [u0, u1) of its ranges row by row, every lane
+                // reading the same row (one broadcast load) and evaluating it
+                // only if it lies in its own range -- each pair is still met
+                // once, and in ascending row order per lane.
+                uint32_t u0 = len ? run.x : 0xFFFFFFFFu, u1 = len ? run.y : 0u, mx = len;
+                for (int o = 32; o > 0; o >>= 1) {
+                    u0 = min(u0, (uint32_t)__shfl_xor((int)u0, o));
+                    u1 = max(u1, (uint32_t)__shfl_xor((int)u1, o));
+                    mx = max(mx, (uint32_t)__shfl_xor((int)mx, o));
+                }
+                if (u1 - u0 <= 2 * mx + 64) {
+                    const uint32_t a = __builtin_amdgcn_readfirstlane(u0), b = __builtin_amdgcn_readfirstlane(u1);
+                    uint32_t pr = a;
+                    for (; pr + 8 <= b; pr += 8) {
+                        float4 y[8];
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) y[u] = srow[pr + u];
+#pragma unroll
+                        for (int u = 0; u < 8; ++u)
+                            if (pr + u >= run.x && pr + u < run.y) visit(y[u]);
                     }
-                    const uint32_t len = run.y - run.x;
-                    if (__ballot(len > 32) == 0) {
-                        sweep(run);
-                    } else {
-                        // A lane near a dense blob holds hundreds of rows while
-                        // the rest hold a few: the wave sweeps the concatenation
-                        // of its lanes' ranges together, 64 rows a step (row k
-                        // belongs to the lane whose range prefix holds k).
-                        uint32_t inc = len;
-                        for (int o = 1; o < 64; o <<= 1) {
-                            const uint32_t v = __shfl_up(inc, o);
-                            if (lane >= o) inc += v;
-                        }
-                        const uint32_t excl = inc - len;
-                        const uint32_t total = __shfl(inc, 63);
-                        for (uint32_t k0 = 0; k0 < total; k0 += 64) {
-                            const uint32_t k = k0 + (uint32_t)lane;
-                            int o = 0;  // the largest lane whose prefix is <= k: the owner of row k
-                            for (int step = 32; step > 0; step >>= 1)
-                                if (__shfl(excl, o + step) <= k) o += step;
-                            const uint32_t p = __shfl(run.x, o) + (k - __shfl(excl, o));
-                            const float qx = __shfl(qr.x, o), qy = __shfl(qr.y, o), qz = __shfl(qr.z, o);
-                            const uint32_t io = __builtin_bit_cast(uint32_t, __shfl(qr.w, o));
-                            if (k < total) {
-                                const float4 y = srow[p];
-                                const uint32_t j = __builtin_bit_cast(uint32_t, y.w);
-                                if (!(self_join && j <= io)) {
-                                    ++cand;
-                                    const float qo[3] = {qx, qy, qz};
-                                    if (d2_row(qo, y, g.d) <= T) emit(((uint64_t)io << jbits) | j);
+                    for (; pr < b; ++pr) {
+                        const float4 y = srow[pr];
+                        if (pr >= run.x && pr < run.y) visit(y);
+                    }
+                    continue;
+                }
+                // A lane near a dense blob holds hundreds of rows while the
+                // rest hold a few: the wave sweeps the concatenation of its
+                // lanes' ranges together, 64 rows a step (row k belongs to
+                // the lane whose range prefix holds k); hits are counted per
+                // owner in LDS and the owner's next free position is its
+                // own count so far plus that LDS count.
+                uint32_t inc = len;
+                for (int o = 1; o < 64; o <<= 1) {
+                    const uint32_t v = __shfl_up(inc, o);
+                    if (lane >= o) inc += v;
+                }
+                const uint32_t excl = inc - len;
+                const uint32_t total = __shfl(inc, 63);
+                for (uint32_t k0 = 0; k0 < total; k0 += 64) {
+                    const uint32_t k = k0 + (uint32_t)lane;
+                    int o = 0;  // the largest lane whose prefix is <= k: the owner of row k
+                    for (int step = 32; step > 0; step >>= 1)
+                        if (__shfl(excl, o + step) <= k) o += step;
+                    const uint32_t p = __shfl(run.x, o) + (k - __shfl(excl, o));
+                    const float qx = __shfl(qr.x, o), qy = __shfl(qr.y, o), qz = __shfl(qr.z, o);
+                    const uint32_t io = __builtin_bit_cast(uint32_t, __shfl(qr.w, o));
+                    const uint64_t obase = MODE == PROBE_WRITE ? shfl_u64(base, o) + (uint64_t)__shfl(mine, o) : 0;
+                    if (k < total) {
+                        const float4 y = srow[p];
+                        const uint32_t j = __builtin_bit_cast(uint32_t, y.w);
+                        if (!(upper && j <= io)) {
+                            if (MODE == PROBE_COUNT) ++cand;
+                            const float qo[3] = {qx, qy, qz};
+                            if (d2_row<D>(qo, y) <= T) {
+                                const uint32_t slot = atomicAdd(&sCnt[wave][o], 1u);
+                                if (MODE == PROBE_WRITE) {
+                                    const uint64_t at = obase + slot;
+                                    if (at < (uint64_t)out_cap) outj[at] = j;
                                 }
                             }
                         }
                     }
                 }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
-                if (__builtin_amdgcn_readfirstlane(sN[wave]) >= (uint32_t)(PCAP / 2)) flush();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                mine += sCnt[wave][lane];
+                sCnt[wave][lane] = 0;
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             }
+        if (MODE == PROBE_COUNT && valid) cnt[i] = mine;
     }
-    flush();
-    for (int s = 32; s > 0; s >>= 1) cand += __shfl_xor(cand, s);
-    if (lane == 0 && cand) atomicAdd((unsigned long long*)&stats[TJ_ST_GRID_CAND], (unsigned long long)cand);
+    if (MODE == PROBE_COUNT) {
+        for (int s = 32; s > 0; s >>= 1) cand += __shfl_xor(cand, s);
+        if (lane == 0 && cand) atomicAdd((unsigned long long*)&stats[TJ_ST_GRID_CAND], (unsigned long long)cand);
+    }
+}
+
+// Exclusive scan of the row counts: off[r] = sum of cnt[0..r), off[n] = the
+// total (also stored to stats[TJ_ST_OUT]).  Three steps: sums of blocks of
+// SCB counts, one block scanning those sums, one pass writing the offsets.
+constexpr int SCB = 4096;
+__global__ void __launch_bounds__(256) scan_sums_kernel(const uint32_t* __restrict__ cnt, int64_t n,
+                                                        uint64_t* __restrict__ part) {
+    const int64_t b0 = (int64_t)blockIdx.x * SCB;
+    uint64_t s = 0;
+    for (int k = threadIdx.x; k < SCB; k += 256)
+        if (b0 + k < n) s += cnt[b0 + k];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    __shared__ uint64_t w[4];
+    if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = w[0] + w[1] + w[2] + w[3];
+}
+__global__ void __launch_bounds__(1024) scan_top_kernel(uint64_t* __restrict__ part, int64_t nb,
+                                                        uint64_t* __restrict__ off_end, uint64_t* __restrict__ stats) {
+    __shared__ uint64_t sp[1024];
+    const int64_t per = (nb + 1023) / 1024;
+    const int64_t a = threadIdx.x * per, b = a + per < nb ? a + per : nb;
+    uint64_t s = 0;
+    for (int64_t t = a; t < b; ++t) s += part[t];
+    sp[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        const uint64_t v = threadIdx.x >= (unsigned)o ? sp[threadIdx.x - o] : 0;
+        __syncthreads();
+        sp[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint64_t run = threadIdx.x ? sp[threadIdx.x - 1] : 0;
+    for (int64_t t = a; t < b; ++t) {
+        const uint64_t v = part[t];
+        part[t] = run;
+        run += v;
+    }
+    if (threadIdx.x == 1023) {
+        *off_end = sp[1023];
+        stats[TJ_ST_OUT] = sp[1023];
+    }
+}
+__global__ void __launch_bounds__(256) scan_apply_kernel(const uint32_t* __restrict__ cnt, int64_t n,
+                                                         const uint64_t* __restrict__ part, uint64_t* __restrict__ off) {
+    const int64_t b0 = (int64_t)blockIdx.x * SCB;
+    constexpr int PER = SCB / 256;  // consecutive counts per thread
+    uint32_t v[PER];
+    uint64_t s = 0;
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int64_t t = b0 + threadIdx.x * PER + u;
+        v[u] = t < n ? cnt[t] : 0;
+        s += v[u];
+    }
+    __shared__ uint64_t sp[256];
+    sp[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 1; o < 256; o <<= 1) {
+        const uint64_t w = threadIdx.x >= (unsigned)o ? sp[threadIdx.x - o] : 0;
+        __syncthreads();
+        sp[threadIdx.x] += w;
+        __syncthreads();
+    }
+    uint64_t run = part[blockIdx.x] + (threadIdx.x ? sp[threadIdx.x - 1] : 0);
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int64_t t = b0 + threadIdx.x * PER + u;
+        if (t < n) off[t] = run;
+        run += v[u];
+    }
+}
+
+// Emit the sorted ids of row i at output positions [o0, o0 + n): (i, j) rows
+// of int32 / int64, or the bare ids of the neighbour table.
+__device__ inline void emit_one(int emit, void* out, uint64_t at, uint32_t i, uint32_t j) {
+    if (emit == tj::GRID_EMIT_TABLE) {
+        ((uint32_t*)out)[at] = j;
+    } else if (emit == tj::GRID_EMIT_I32) {
+        ((int2*)out)[at] = make_int2((int)i, (int)j);
+    } else {
+        ((longlong2*)out)[at] = make_longlong2((long long)i, (long long)j);
+    }
+}
+
+// Per-row sort of the neighbour ids (distinct within a row) + emit, by row
+// length: rows of at most LANE_MAX ids are sorted by their own lane in
+// registers (segsort_lane_kernel); rows of up to P ids, P a power of two from
+// 16 to 512, by a group of G lanes holding E = P / G ids each
+// (midsort_kernel<G, E>: 16 lanes up to 128 ids, then 32 and 64, so no lane
+// holds more than 8); longer rows are listed for longsort_kernel.  Both sorts are
+// bitonic networks unrolled at compile time: compare-exchanges between the
+// registers of a lane, or between lanes of a group (shuffles), no LDS and no
+// barriers.  Ids are distinct within a row, so any correct sort gives the
+// same order.
+constexpr int LANE_MAX = 8;
+constexpr int GROUP_MAX = 512;  // 16 lanes x 32 ids
+
+__device__ __forceinline__ void cx(uint32_t& a, uint32_t& b, bool asc) {
+    const uint32_t lo = min(a, b), hi = max(a, b);
+    a = asc ? lo : hi;
+    b = asc ? hi : lo;
+}
+
+// Class of a row length: the padded size P (16 .. GROUP_MAX) of its
+// midsort_kernel, 0 below, -1 above.
+__device__ inline int row_class(uint32_t n) {
+    if (n <= (uint32_t)LANE_MAX) return 0;
+    if (n > (uint32_t)GROUP_MAX) return -1;
+    int p = 16;
+    while ((uint32_t)p < n) p <<= 1;
+    return p;
+}
+
+__global__ void __launch_bounds__(256) segsort_lane_kernel(const uint64_t* __restrict__ off, int64_t nR,
+                                                           const uint32_t* __restrict__ outj, int64_t out_cap, int emit,
+                                                           void* __restrict__ out, uint32_t* __restrict__ longrows,
+                                                           uint32_t* __restrict__ nlong) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nR; i += stride) {
+        uint64_t o0 = off[i], o1 = off[i + 1];
+        if (o1 > (uint64_t)out_cap) o1 = (uint64_t)out_cap;  // overflowed run: skip what has no room
+        if (o0 > o1) o0 = o1;
+        const uint32_t n = (uint32_t)(o1 - o0);
+        if (n > (uint32_t)GROUP_MAX) longrows[atomicAdd(nlong, 1u)] = (uint32_t)i;
+        if (n > (uint32_t)LANE_MAX || n == 0) continue;
+        uint32_t v[LANE_MAX];
+#pragma unroll
+        for (int k = 0; k < LANE_MAX; ++k) v[k] = (uint32_t)k < n ? outj[o0 + k] : 0xFFFFFFFFu;
+#pragma unroll
+        for (int kk = 2; kk <= LANE_MAX; kk <<= 1)
+#pragma unroll
+            for (int jj = kk >> 1; jj > 0; jj >>= 1)
+#pragma unroll
+                for (int x = 0; x < LANE_MAX; ++x)
+                    if ((x ^ jj) > x) cx(v[x], v[x ^ jj], (x & kk) == 0);
+#pragma unroll
+        for (int k = 0; k < LANE_MAX; ++k)
+            if ((uint32_t)k < n) emit_one(emit, out, o0 + k, (uint32_t)i, v[k]);
+    }
+}
+
+// Rows of class P = G * E ids: each wave scans 64 rows at a time and sorts
+// the ones of its class 64 / G at a time, one group of G lanes per row.  Lane
+// l of a group holds ids x = G e + l (e < E) of its row: steps with a
+// distance of at least G are between registers of a lane, shorter ones
+// between lanes (shuffles within the group).
+template <int G, int E>
+__global__ void __launch_bounds__(256) midsort_kernel(const uint64_t* __restrict__ off, int64_t nR,
+                                                      const uint32_t* __restrict__ outj, int64_t out_cap, int emit,
+                                                      void* __restrict__ out) {
+    constexpr int P = G * E;
+    constexpr int NG = 64 / G;  // rows per wave step
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int g = lane / G, l = lane % G;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t tb = (int64_t)blockIdx.x * blockDim.x + wave * 64; tb < nR; tb += stride) {
+        const int64_t i = tb + lane;
+        uint64_t o0 = 0, o1 = 0;
+        if (i < nR) {
+            o0 = off[i];
+            o1 = off[i + 1];
+            if (o1 > (uint64_t)out_cap) o1 = (uint64_t)out_cap;
+            if (o0 > o1) o0 = o1;
+        }
+        const uint32_t n = (uint32_t)(o1 - o0);
+        uint64_t mine = __ballot(i < nR && row_class(n) == P);
+        while (mine) {
+            // the next NG rows of the class, one per group
+            int src = -1;
+#pragma unroll
+            for (int q = 0; q < NG; ++q) {
+                const int b = mine ? __builtin_ctzll(mine) : -1;
+                if (b >= 0) mine &= mine - 1;
+                if (q == g) src = b;
+            }
+            // every lane takes part in the shuffles (a lane left out of a
+            // shuffle supplies no value to the lanes that read it)
+            const int s = src < 0 ? 0 : src;
+            const uint64_t r0 = shfl_u64(o0, s);
+            const uint32_t rs = (uint32_t)__shfl((int)n, s);
+            const uint32_t rn = src < 0 ? 0u : rs;
+            uint32_t v[E];
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const uint32_t x = (uint32_t)(G * e + l);
+                v[e] = x < rn ? outj[r0 + x] : 0xFFFFFFFFu;
+            }
+#pragma unroll
+            for (int kk = 2; kk <= P; kk <<= 1)
+#pragma unroll
+                for (int jj = kk >> 1; jj > 0; jj >>= 1) {
+                    if (jj >= G) {
+                        // x and x ^ jj differ in e; kk > jj >= G, so the
+                        // direction depends on e only
+#pragma unroll
+                        for (int e = 0; e < E; ++e) {
+                            const int f = e ^ (jj / G);
+                            if (f > e) cx(v[e], v[f], ((G * e) & kk) == 0);
+                        }
+                    } else {
+                        // keep the low value when this lane is the lower index
+                        // of an ascending pair (or the upper of a descending one)
+#pragma unroll
+                        for (int e = 0; e < E; ++e) {
+                            const uint32_t p = (uint32_t)__shfl_xor((int)v[e], jj);
+                            const int x = G * e + l;
+                            const bool asc = (x & kk) == 0;
+                            const bool lower = (l & jj) == 0;
+                            v[e] = (asc == lower) ? min(v[e], p) : max(v[e], p);
+                        }
+                    }
+                }
+            const uint32_t ri = (uint32_t)(tb + s);
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const uint32_t x = (uint32_t)(G * e + l);
+                if (x < rn) emit_one(emit, out, r0 + x, ri, v[e]);
+            }
+        }
+    }
+}
+
+// Rows longer than GROUP_MAX (dense clusters, large eps): one workgroup per
+// row.  Chunks of CHUNK ids are sorted in LDS (bitonic), then sorted runs are
+// merged pairwise (merge path: each thread finds where its slice of the
+// output splits the two runs by a binary search, then merges sequentially),
+// ping-ponging between outj and outj2 over the row's own range, until one run
+// remains; then emitted.  The list of long rows and its length come from
+// segsort_lane_kernel through device memory.
+constexpr int CHUNK = 4096;
+__global__ void __launch_bounds__(256) longsort_kernel(const uint64_t* __restrict__ off, const uint32_t* __restrict__ longrows,
+                                                       const uint32_t* __restrict__ nlong, uint32_t* __restrict__ outj,
+                                                       uint32_t* __restrict__ outj2, int64_t out_cap, int emit,
+                                                       void* __restrict__ out) {
+    __shared__ uint32_t S[CHUNK];
+    const uint32_t count = *nlong;
+    for (uint32_t r = blockIdx.x; r < count; r += gridDim.x) {
+        const uint32_t i = longrows[r];
+        uint64_t o0 = off[i], o1 = off[i + 1];
+        if (o1 > (uint64_t)out_cap) o1 = (uint64_t)out_cap;
+        if (o0 > o1) o0 = o1;
+        const uint64_t n = o1 - o0;
+        uint32_t* src = outj + o0;
+        uint32_t* dst = outj2 + o0;
+        // sorted chunks
+        for (uint64_t c0 = 0; c0 < n; c0 += CHUNK) {
+            const uint32_t cn = (uint32_t)(n - c0 < (uint64_t)CHUNK ? n - c0 : (uint64_t)CHUNK);
+            for (uint32_t k = threadIdx.x; k < (uint32_t)CHUNK; k += 256) S[k] = k < cn ? src[c0 + k] : 0xFFFFFFFFu;
+            for (uint32_t kk = 2; kk <= (uint32_t)CHUNK; kk <<= 1)
+                for (uint32_t jj = kk >> 1; jj > 0; jj >>= 1) {
+                    __syncthreads();
+                    for (uint32_t x = threadIdx.x; x < (uint32_t)CHUNK; x += 256) {
+                        const uint32_t y = x ^ jj;
+                        if (y > x) {
+                            const uint32_t a = S[x], b = S[y];
+                            const bool up = (x & kk) == 0;
+                            if ((a > b) == up) {
+                                S[x] = b;
+                                S[y] = a;
+                            }
+                        }
+                    }
+                }
+            __syncthreads();
+            for (uint32_t k = threadIdx.x; k < cn; k += 256) src[c0 + k] = S[k];
+            __syncthreads();
+        }
+        // pairwise merges of runs of width w
+        for (uint64_t w = CHUNK; w < n; w <<= 1) {
+            for (uint64_t a0 = 0; a0 < n; a0 += 2 * w) {
+                const uint64_t a1 = a0 + w < n ? a0 + w : n;
+                const uint64_t b1 = a0 + 2 * w < n ? a0 + 2 * w : n;
+                const uint64_t na = a1 - a0, nb = b1 - a1, tot = na + nb;
+                const uint64_t per = (tot + 255) / 256;
+                const uint64_t d0 = (uint64_t)threadIdx.x * per;
+                if (d0 < tot) {
+                    const uint64_t d1 = d0 + per < tot ? d0 + per : tot;
+                    // split of the first d0 outputs: ia from A, d0 - ia from B
+                    uint64_t lo = d0 > nb ? d0 - nb : 0, hi = d0 < na ? d0 : na;
+                    while (lo < hi) {
+                        const uint64_t m = (lo + hi) >> 1;
+                        if (src[a0 + m] < src[a1 + (d0 - m - 1)]) lo = m + 1; else hi = m;
+                    }
+                    uint64_t ia = lo, ib = d0 - lo;
+                    for (uint64_t dd = d0; dd < d1; ++dd) {
+                        const bool takeA = ib >= nb || (ia < na && src[a0 + ia] < src[a1 + ib]);
+                        dst[a0 + dd] = takeA ? src[a0 + ia++] : src[a1 + ib++];
+                    }
+                }
+            }
+            __syncthreads();
+            uint32_t* t = src;
+            src = dst;
+            dst = t;
+        }
+        for (uint64_t k = threadIdx.x; k < n; k += 256) emit_one(emit, out, o0 + k, i, src[k]);
+        __syncthreads();
+    }
 }
 
 }  // namespace
@@ -578,12 +951,11 @@ GridLayout grid_layout(int64_t nR, int64_t nS, int64_t out_cap) {
     L.count_ns = carve(8);
     L.rkeys = carve(8 * (size_t)(nR > 0 ? nR : 1));
     L.skeys = carve(8 * (size_t)(nS > 0 ? nS : 1));
-    L.sort_tmp = carve(sort_temp_bytes(std::max<int64_t>(std::max<int64_t>(nmax, out_cap), 1)));
+    L.sort_tmp = carve(sort_temp_bytes(std::max<int64_t>(nmax, 1)));
     L.srow = carve(16 * (size_t)(nS > 0 ? nS : 1));
     L.rrow = carve(16 * (size_t)(nR > 0 ? nR : 1));
     L.hkey = carve(8 * hs);
     L.hval = carve(8 * hs);
-    L.outj = carve(8 * (size_t)(out_cap > 0 ? out_cap : 1));  // pair keys
     // occupancy bitmap: 64 bits per S point (at least 2^20); sparser grids
     // go without it
     L.bm_bits = std::max<uint64_t>(1ull << 20, 64ull * (uint64_t)(nS > 0 ? nS : 1));
@@ -591,6 +963,14 @@ GridLayout grid_layout(int64_t nR, int64_t nS, int64_t out_cap) {
     L.bmpre = carve(4 * (L.bm_bits / 64));
     L.bmsum = carve(4 * (L.bm_bits / 64 / BMB + 1));
     L.runs = carve(4 * (size_t)((nS > 0 ? nS : 1) + 1));
+    // output: per-row counts and offsets, the ids in row order (and a
+    // ping-pong copy for the long-row merge), the list of long rows
+    L.cnt = carve(4 * (size_t)(nR > 0 ? nR : 1));
+    L.off = carve(8 * (size_t)(nR + 1));
+    L.scan_part = carve(8 * (size_t)((nR + SCB - 1) / SCB + 1));
+    L.outj = carve(4 * (size_t)(out_cap > 0 ? out_cap : 1));
+    L.outj2 = carve(4 * (size_t)(out_cap > 0 ? out_cap : 1));
+    L.longrows = carve(4 * (size_t)(nR + 1));  // the count, then the rows
     L.total = (cur + 255) / 256 * 256;
     return L;
 }
@@ -603,8 +983,8 @@ static unsigned blocks_for(int64_t n, int64_t cap) {
 }
 
 int launch_grid_join(const float* R, int64_t nR, const float* S, int64_t nS, int d, double T, bool self_join,
-                     uint8_t* ws, const GridLayout& L, int64_t out_cap, void* pairs, int out_format, uint64_t* stats,
-                     hipStream_t s) {
+                     bool upper, uint8_t* ws, const GridLayout& L, int64_t out_cap, int emit, void* pairs,
+                     uint32_t* counts, uint64_t* stats, hipStream_t s) {
     TJ_CHECK(d >= 1 && d <= 3, TJ_ERR_UNSUPPORTED, "grid join serves d <= 3 (d=%d)", d);
     TJ_CHECK(nR < (1ll << 32) && nS < (1ll << 32), TJ_ERR_UNSUPPORTED, "grid join: point sets beyond 2^32");
     if (self_join) { S = R; nS = nR; }
@@ -616,8 +996,6 @@ int launch_grid_join(const float* R, int64_t nR, const float* S, int64_t nS, int
     float4* srow = (float4*)(ws + L.srow);
     uint64_t* hkey = (uint64_t*)(ws + L.hkey);
     uint2* hval = (uint2*)(ws + L.hval);
-    uint64_t* okeys = (uint64_t*)(ws + L.outj);
-    const int jbits = bits_for(nS > 1 ? nS : 2);
     uint64_t* cR = (uint64_t*)(ws + L.count_nr);
     uint64_t* cS = (uint64_t*)(ws + L.count_ns);
     int rc;
@@ -661,17 +1039,42 @@ int launch_grid_join(const float* R, int64_t nR, const float* S, int64_t nS, int
     hipLaunchKernelGGL(bm_top_scan, dim3(1), dim3(1024), 0, s, gp, bmsum);
     hipLaunchKernelGGL(bm_apply, dim3(nbm), dim3(256), 0, s, gp, bm, bmsum, bmpre);
     hipLaunchKernelGGL(runs_kernel, dim3(blocks_for(nS, gb)), dim3(256), 0, s, nS, gp, skeys, bm, bmpre, runs);
-    // one probe pass appends the pair keys; the key sort orders them
+    // count pass -> offsets -> write pass -> per-row sort and emit
+    uint32_t* cnt = counts ? counts : (uint32_t*)(ws + L.cnt);
+    uint64_t* off = (uint64_t*)(ws + L.off);
+    uint64_t* part = (uint64_t*)(ws + L.scan_part);
+    uint32_t* outj = (uint32_t*)(ws + L.outj);
+    uint32_t* outj2 = (uint32_t*)(ws + L.outj2);
+    uint32_t* nlong = (uint32_t*)(ws + L.longrows);
+    uint32_t* longrows = nlong + 1;
     const unsigned pb = blocks_for(nR, (int64_t)num_cus() * 16);
-    hipLaunchKernelGGL(probe_kernel, dim3(pb), dim3(256), 0, s, rrow, nR, gp, srow, hkey, hval,
-                       (uint64_t)(L.hslots - 1), bm, bmpre, runs, T, (int)self_join, okeys, jbits, out_cap, stats);
+    const uint64_t hmask = (uint64_t)(L.hslots - 1);
+    // (one instantiation per dimension: the distance has no runtime branches)
+    auto probe = [&](auto mode, uint32_t* c, const uint64_t* o, uint32_t* oj) {
+        constexpr int M = decltype(mode)::value;
+        auto kern = d == 1 ? probe_kernel<M, 1> : d == 2 ? probe_kernel<M, 2> : probe_kernel<M, 3>;
+        hipLaunchKernelGGL(kern, dim3(pb), dim3(256), 0, s, rrow, nR, gp, srow, hkey, hval, hmask, bm, bmpre, runs, T,
+                           (int)upper, c, o, oj, out_cap, stats);
+    };
+    probe(std::integral_constant<int, PROBE_COUNT>{}, cnt, (const uint64_t*)nullptr, (uint32_t*)nullptr);
+    const int64_t nsb = (nR + SCB - 1) / SCB;
+    hipLaunchKernelGGL(scan_sums_kernel, dim3((unsigned)nsb), dim3(256), 0, s, cnt, nR, part);
+    hipLaunchKernelGGL(scan_top_kernel, dim3(1), dim3(1024), 0, s, part, nsb, off + nR, stats);
+    hipLaunchKernelGGL(scan_apply_kernel, dim3((unsigned)nsb), dim3(256), 0, s, cnt, nR, part, off);
+    probe(std::integral_constant<int, PROBE_WRITE>{}, (uint32_t*)nullptr, (const uint64_t*)off, outj);
+    if ((rc = launch_fill_u64((uint64_t*)nlong, 1, 0, s))) return rc;  // (clears the count and its neighbour word)
+    const dim3 sg(blocks_for(nR, (int64_t)num_cus() * 8));
+    hipLaunchKernelGGL(segsort_lane_kernel, sg, dim3(256), 0, s, off, nR, outj, out_cap, emit, pairs, longrows, nlong);
+    hipLaunchKernelGGL((midsort_kernel<16, 1>), sg, dim3(256), 0, s, off, nR, outj, out_cap, emit, pairs);
+    hipLaunchKernelGGL((midsort_kernel<16, 2>), sg, dim3(256), 0, s, off, nR, outj, out_cap, emit, pairs);
+    hipLaunchKernelGGL((midsort_kernel<16, 4>), sg, dim3(256), 0, s, off, nR, outj, out_cap, emit, pairs);
+    hipLaunchKernelGGL((midsort_kernel<16, 8>), sg, dim3(256), 0, s, off, nR, outj, out_cap, emit, pairs);
+    hipLaunchKernelGGL((midsort_kernel<32, 8>), sg, dim3(256), 0, s, off, nR, outj, out_cap, emit, pairs);
+    hipLaunchKernelGGL((midsort_kernel<64, 8>), sg, dim3(256), 0, s, off, nR, outj, out_cap, emit, pairs);
+    hipLaunchKernelGGL(longsort_kernel, dim3((unsigned)num_cus()), dim3(256), 0, s, off, longrows, nlong, outj, outj2,
+                       out_cap, emit, pairs);
     TJ_HIP(hipGetLastError());
-    // the keys were written in query (cell) order; one radix sort over all
-    // their bits orders them by (i, j), then they unpack to pairs
-    uint64_t* sorted = okeys;
-    const int keybits = bits_for(nR > 1 ? nR : 2) + jbits;
-    if ((rc = launch_sort(ws + L.sort_tmp, okeys, stats + TJ_ST_OUT, out_cap, keybits, s, &sorted))) return rc;
-    return launch_unpack(sorted, stats + TJ_ST_OUT, out_cap, jbits, pairs, out_format, s);
+    return TJ_OK;
 }
 
 }  // namespace tj

@@ -215,12 +215,22 @@ inline int bits_for(int64_t n) { int b = 0; while (((int64_t)1 << b) < n) ++b; r
 namespace tj {
 // Workspace layout of the grid join (grid.hip), kept in the plan (api.hip).
 struct GridLayout {
-    size_t bounds, bpart, params, rkeys, skeys, sort_tmp, srow, rrow, hkey, hval, outj, count_nr, count_ns, bitmap, bmpre,
-        bmsum, runs, total;
+    size_t bounds, bpart, params, rkeys, skeys, sort_tmp, srow, rrow, hkey, hval, count_nr, count_ns, bitmap, bmpre,
+        bmsum, runs, cnt, off, scan_part, outj, outj2, longrows, total;
     uint64_t hslots;
     uint64_t bm_bits;  // capacity of the cell-occupancy bitmap
     int idxbits;
 };
+// Output of the grid join: (i, j) rows, or the neighbour table of
+// GDS-Join (counts per point + neighbour ids grouped by point).
+enum { GRID_EMIT_I32 = TJ_OUT_I32, GRID_EMIT_I64 = TJ_OUT_I64, GRID_EMIT_TABLE = 16 };
+GridLayout grid_layout(int64_t nR, int64_t nS, int64_t out_cap);
+// self_join: S is R (one index); upper: keep j > i only (the self-join's
+// i < j output).  Table output: counts (uint32 [nR]) and table (uint32,
+// out_cap entries); pairs output: pairs ([out_cap, 2] in emit's format).
+int launch_grid_join(const float* R, int64_t nR, const float* S, int64_t nS, int d, double T, bool self_join,
+                     bool upper, uint8_t* ws, const GridLayout& L, int64_t out_cap, int emit, void* pairs,
+                     uint32_t* counts, uint64_t* stats, hipStream_t s);
 struct ScreenArgs {
     const void* qa_hi; const void* qa_lo; const RowMeta* ma; int64_t na;
     const void* qb_hi; const void* qb_lo; const RowMeta* mb; int64_t nb;

@@ -144,3 +144,98 @@ def test_grid_far_outlier(tj, oracle):
         Y = X[::-1].copy()
         got = tj.join(X, Y, T=T, method="grid", out_dtype=np.int64)
         assert np.array_equal(got, oracle.join(X, Y, T)), name
+
+
+@pytest.mark.parametrize("d", [2, 3, 8])
+def test_gdsjoin_matches_oracle(tj, oracle, d):
+    """GDS-Join's Python call (GDS/gdsjoingpu.py:146-232): counts per point
+    including itself, and every point's neighbours (ascending here)."""
+    rng = np.random.default_rng(60 + d)
+    n = 4000
+    X = rng.random((n, d)).astype(np.float32)
+    X[n // 2:] = (X[: n // 2] + rng.normal(0, 0.003, (n // 2, d))).astype(np.float32)
+    eps = {2: 0.01, 3: 0.03, 8: 0.3}[d]
+    say("gdsjoin", d)
+    counts, table = tj.gdsjoin(X.reshape(-1), eps, d, "float")
+    T = float(np.float32(eps)) ** 2
+    want = oracle.join(X, X, T)  # every directed pair, self pairs included
+    assert counts.dtype == np.uint32 and table.dtype == np.uint32
+    assert np.array_equal(counts, np.bincount(want[:, 0], minlength=n))
+    assert np.array_equal(table.astype(np.int64), want[:, 1])
+    assert int(counts.min()) >= 1
+
+
+def test_gdsjoin_double_input(tj, oracle):
+    rng = np.random.default_rng(66)
+    X = rng.random((500, 3)).astype(np.float32)
+    say("gdsjoin double")
+    c1, t1 = tj.gdsjoin(X.astype(np.float64).ravel(), 0.05, 3, "double")
+    c2, t2 = tj.gdsjoin(X.ravel(), 0.05, 3, "float")
+    want = oracle.join(X, X, 0.05 ** 2)
+    assert np.array_equal(t1.astype(np.int64), want[:, 1])
+    assert np.array_equal(c1, np.bincount(want[:, 0], minlength=500))
+    # float32 EPSILON rounding differs from the double one only at ties
+    assert abs(len(t1) - len(t2)) <= 4
+    with pytest.raises(ValueError):
+        tj.gdsjoin(np.full(6, 0.1), 0.05, 3, "double")  # 0.1 is not a float32
+
+
+def test_grid_long_rows(tj, oracle):
+    """Rows beyond the wave sort (2,048 ids): one workgroup per row sorts
+    chunks of 4,096 ids in LDS and merges them (rows of up to 9,000 ids, so
+    two merge passes)."""
+    rng = np.random.default_rng(70)
+    blob = lambda n: (0.5 + rng.normal(0, 0.001, (n, 3))).astype(np.float32)
+    X = blob(5000)
+    st = check(tj, oracle, X, None, 0.05 ** 2, "one blob self-join (rows of 0..4999 ids)")
+    assert st["out"] == 5000 * 4999 // 2
+    R, S = blob(1500), blob(9000)
+    S[::7] += np.float32(0.2)  # a few S points far away
+    check(tj, oracle, R, S, 0.05 ** 2, "one blob R x S (rows of ~7,700 ids)")
+
+
+def test_grid_capacity_retry(tj, oracle):
+    from tensorjoin_variants_amd.join import Plan, _as_device
+    rng = np.random.default_rng(71)
+    X = rng.random((3000, 2)).astype(np.float32)
+    T = 0.02 ** 2
+    say("grid capacity retry")
+    plan = Plan(3000, None, 2, "grid", "none", out_capacity=16, amb_capacity=0)
+    res = plan.run(_as_device(X), None, T)
+    assert plan.out_capacity >= res.stats["out"] > 16
+    assert np.array_equal(res.numpy().astype(np.int64), oracle.join(X, None, T))
+
+
+def test_grid_stage_abi(tj, oracle):
+    """tj_grid_join through ctypes, its own workspace query, R x S and self."""
+    import ctypes as C
+    import torch
+    from tensorjoin_variants_amd import _lib
+    L = _lib.load()
+    rng = np.random.default_rng(72)
+    R = rng.random((6000, 3)).astype(np.float32)
+    S = rng.random((5000, 3)).astype(np.float32)
+    T = 0.03 ** 2
+    for name, A, B in [("rxs", R, S), ("self", R, None)]:
+        say("grid stage", name)
+        want = oracle.join(A, B, T)
+        cap = len(want) + 10
+        nS = 0 if B is None else B.shape[0]
+        need = C.c_size_t()
+        assert L.tj_grid_workspace_bytes(A.shape[0], nS, cap, C.byref(need)) == 0
+        ws = torch.empty(need.value, dtype=torch.uint8, device="cuda")
+        Ad = torch.from_numpy(A).cuda()
+        Bd = None if B is None else torch.from_numpy(B).cuda()
+        pairs = torch.full((cap, 2), -1, dtype=torch.int64, device="cuda")
+        stats = torch.zeros(_lib.STATS_SLOTS, dtype=torch.int64, device="cuda")
+        _lib.check(L.tj_grid_join(Ad.data_ptr(), A.shape[0], None if Bd is None else Bd.data_ptr(), nS, 3, T,
+                                  ws.data_ptr(), need.value, cap, pairs.data_ptr(), 1, stats.data_ptr(),
+                                  torch.cuda.current_stream().cuda_stream), "tj_grid_join")
+        torch.cuda.synchronize()
+        st = _lib.decode_stats(stats.cpu().numpy().view(np.uint64))
+        assert st["out"] == len(want)
+        assert np.array_equal(pairs[: len(want)].cpu().numpy(), want)
+        assert bool((pairs[len(want):] == -1).all())  # nothing past the output
+        small = C.c_size_t(need.value - 256)
+        assert L.tj_grid_join(Ad.data_ptr(), A.shape[0], None if Bd is None else Bd.data_ptr(), nS, 3, T,
+                              ws.data_ptr(), small.value, cap, pairs.data_ptr(), 1, stats.data_ptr(), None) == -4

@@ -150,3 +150,17 @@ def test_published_thresholds_are_exact_squares():
         assert np.float32(c["T"]) == c["T"], name            # representable in binary32
         assert c["count"] >= vecio.PUBLISHED_ROWS and (c["count"] - vecio.PUBLISHED_ROWS) % 2 == 0
     assert vecio.expected("original")["expected_count"] == 3926078
+
+
+def test_gdsjoin_input_checks():
+    """GDS-Join's flat DATASET layout and DTYPE rules (no GPU needed)."""
+    import numpy as np
+    from tensorjoin_variants_amd.gds import _points
+    x = _points(np.arange(12, dtype=np.float64), 3, "double")
+    assert x.shape == (4, 3) and x.dtype == np.float32
+    with pytest.raises(ValueError):
+        _points(np.arange(10), 3, "float")           # not a multiple of NDIM
+    with pytest.raises(ValueError):
+        _points(np.full(3, 0.1), 3, "double")        # not exact in float32
+    with pytest.raises(ValueError):
+        _points(np.zeros(3), 3, "half")
