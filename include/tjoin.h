@@ -48,6 +48,8 @@ extern "C" {
 #define TJ_METHOD_CERTIFIED 0  /* MFMA screen + FP64 refinement of the ambiguous band */
 #define TJ_METHOD_PLAIN_FP64 1 /* FP64 brute-force tile, no certificate band */
 #define TJ_METHOD_GRID 2       /* uniform grid (d <= 3), FP64 predicate per candidate */
+#define TJ_METHOD_PLAIN_FP32 3 /* control: FP32 GEMM-form tile over every pair, rigorous binary64 interval,
+                                  FP64 for the pairs it cannot decide (G16/src/fp32_gpu_norm_operator.py:107-161) */
 
 /* Output pair formats. */
 #define TJ_OUT_I32 0 /* int32 [P, 2] */

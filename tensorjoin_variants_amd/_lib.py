@@ -18,7 +18,7 @@ TJ_OK = 0
 ERRORS = {-1: "TJ_ERR_INVALID", -2: "TJ_ERR_HIP", -3: "TJ_ERR_UNSUPPORTED", -4: "TJ_ERR_WORKSPACE"}
 
 PREC = {"none": 0, "fp16": 1, "fp16x3": 2, "bf16": 3, "bf16x3": 4, "fp8": 5, "fp8x3": 6}
-METHOD = {"certified": 0, "plain_fp64": 1, "grid": 2}
+METHOD = {"certified": 0, "plain_fp64": 1, "grid": 2, "plain_fp32": 3}
 OUT = {"int32": 0, "int64": 1}
 OUT_U64_CANONICAL = 2
 NOALLOC_SCRATCH_BYTES = 64 << 20

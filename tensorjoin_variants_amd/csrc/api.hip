@@ -169,7 +169,8 @@ int tj::plan_init(tj_plan* p, int64_t nR, int64_t nS, int d, int method, int pre
                   int64_t amb_capacity, unsigned flags) {
     TJ_CHECK(nR >= 0 && nS >= 0 && d >= 1, TJ_ERR_INVALID, "bad shape nR=%lld nS=%lld d=%d", (long long)nR,
              (long long)nS, d);
-    TJ_CHECK(method == TJ_METHOD_CERTIFIED || method == TJ_METHOD_PLAIN_FP64 || method == TJ_METHOD_GRID,
+    TJ_CHECK(method == TJ_METHOD_CERTIFIED || method == TJ_METHOD_PLAIN_FP64 || method == TJ_METHOD_GRID ||
+                 method == TJ_METHOD_PLAIN_FP32,
              TJ_ERR_INVALID, "method %d is not served by tj_join", method);
     TJ_CHECK(method != TJ_METHOD_GRID || d <= 3, TJ_ERR_UNSUPPORTED, "the grid method serves d <= 3 (d=%d)", d);
     TJ_CHECK(out_capacity >= 0 && out_capacity < (1ll << 32), TJ_ERR_INVALID, "out_capacity must be in [0, 2^32)");
@@ -219,6 +220,10 @@ int tj::plan_init(tj_plan* p, int64_t nR, int64_t nS, int d, int method, int pre
         p->col_splits = sl.col_splits;
         p->cols_per_split = sl.cols_per_split;
         p->grid = sl.grid;
+    }
+    if (method == TJ_METHOD_PLAIN_FP32) {
+        p->off_normR = carve(cur, plain32_norm_bytes(nR));
+        p->off_normS = p->self_join ? p->off_normR : carve(cur, plain32_norm_bytes(nS));
     }
     if (method == TJ_METHOD_GRID) {
         p->glay = grid_layout(p->nR, p->nS, p->out_cap);
@@ -376,6 +381,13 @@ int tj::join_enqueue(const JoinCall& c, hipStream_t s) {
         TJ_MARK(3);
         TJ_MARK(4);
         return TJ_OK;
+    } else if (p->method == TJ_METHOD_PLAIN_FP32) {
+        TJ_MARK(1);
+        if ((rc = launch_plain32(R, p->nR, S, p->nS, p->d, p->jbits, T, p->self_join, ws + p->off_normR,
+                                 ws + p->off_normS, out_keys, p->out_cap, stats, s)))
+            return rc;
+        TJ_MARK(2);
+        TJ_MARK(3);
     } else {
         TJ_MARK(1);
         if ((rc = launch_plain(R, p->nR, S, p->nS, p->d, p->jbits, T, p->self_join, out_keys, p->out_cap, stats, s)))

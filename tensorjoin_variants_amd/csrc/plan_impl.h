@@ -15,7 +15,7 @@ struct tj_plan {
     size_t off_grid;
     tj::GridLayout glay;
     size_t off_maxabs, off_params, off_metaR, off_metaS, off_qRh, off_qRl, off_qSh, off_qSl, off_out, off_amb,
-        off_aux, off_sort, total;
+        off_aux, off_sort, off_normR, off_normS, total;
     hipEvent_t ev[5];
     bool timing;
     hipGraphExec_t graph;

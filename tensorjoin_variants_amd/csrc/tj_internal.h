@@ -263,6 +263,9 @@ int launch_certify(const float* R, const float* S, int d, int jbits, const uint6
                    hipStream_t s);
 int launch_plain(const float* R, int64_t nR, const float* S, int64_t nS, int d, int jbits, double T,
                  bool self_join, uint64_t* out_keys, int64_t out_cap, uint64_t* stats, hipStream_t s);
+size_t plain32_norm_bytes(int64_t n);
+int launch_plain32(const float* R, int64_t nR, const float* S, int64_t nS, int d, int jbits, double T, bool self_join,
+                   void* normR, void* normS, uint64_t* out_keys, int64_t out_cap, uint64_t* stats, hipStream_t s);
 size_t sort_temp_bytes(int64_t capacity);
 // result == NULL: the sorted keys end in `keys`; otherwise *result names the
 // buffer holding them (keys or the ping-pong half of temp), saving the copy

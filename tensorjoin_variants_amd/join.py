@@ -170,7 +170,8 @@ def join(R, S=None, eps: float | None = None, *, T: float | None = None, method:
     d2_seq(R_i, S_j) <= T with T = fl64(eps^2) unless T is given.
     method: 'auto' (grid for d <= 3, else certified), 'certified' (MFMA
     screen + FP64 refinement), 'grid' (uniform grid, d <= 3), 'plain_fp64'
-    (FP64 brute force).
+    (FP64 brute force), 'plain_fp32' (the FP32-first control: FP32 GEMM-form
+    tile over every pair, rigorous interval, FP64 for what it leaves).
     """
     if (eps is None) == (T is None):
         raise ValueError("give exactly one of eps and T")
@@ -237,12 +238,15 @@ class Matrix:
     OCP e4m3 with per-row power-of-two scales).  The split (S*) and fused
     (F*) variants of the reference differ only in kernel structure; both map
     to the fused screen here and return the same pairs.  'F32' -> the plain
-    FP64 brute force.
+    FP64 brute force; 'P32' -> the pedantic FP32-first control of
+    G16/src/fp32_gpu_norm_operator.py:107-161 (FP32 GEMM-form tile,
+    rigorous interval, FP64 terminal).
     """
 
     METHODS = {"F16": ("certified", "fp16"), "S16": ("certified", "fp16"),
                "F8": ("certified", "fp8"), "S8": ("certified", "fp8"),
-               "F16X3": ("certified", "fp16x3"), "F32": ("plain_fp64", "none")}
+               "F16X3": ("certified", "fp16x3"), "F32": ("plain_fp64", "none"),
+               "P32": ("plain_fp32", "none")}
 
     def __init__(self, T: float):
         self.T = float(T)

@@ -77,6 +77,45 @@ def test_plain_fp64(tj, oracle, d):
     run_check(tj, oracle, R, None, T, "fp16", method="plain_fp64", name=f"plain self d{d}")
 
 
+@pytest.mark.parametrize("d", [1, 3, 32, 100, 128, 512])
+def test_plain_fp32(tj, oracle, d):
+    """The FP32-first control: FP32 GEMM-form tile, rigorous interval, FP64
+    for the undecided pairs -- bit-exact like every method."""
+    rng = np.random.default_rng(150 + d)
+    R = rng.standard_normal((1100, d)).astype(np.float32)
+    S = rng.standard_normal((900, d)).astype(np.float32)
+    T = sample_T(oracle, R, S, 0.01)
+    st = run_check(tj, oracle, R, S, T, "none", method="plain_fp32", name=f"plain32 rxs d{d}")
+    assert st["cert_accept"] + st["fp64_accept"] == st["out"], st
+    assert st["ambiguous"] == st["fp64_refined"], st
+    run_check(tj, oracle, R, None, T, "none", method="plain_fp32", name=f"plain32 self d{d}")
+
+
+def test_plain_fp32_hard_cases(tj, oracle):
+    """Near ties (partners at exactly T), magnitudes 1e-30..1e30 (the
+    interval widens to FP64 where binary32 products underflow or overflow),
+    exact zeros and duplicates."""
+    rng = np.random.default_rng(160)
+    n, d = 1500, 64
+    R = rng.standard_normal((n, d)).astype(np.float32)
+    step = rng.standard_normal((n, d)).astype(np.float32)
+    step /= np.linalg.norm(step, axis=1, keepdims=True)
+    S = (R + (0.5 * (1.0 + 1e-3 * (rng.random(n) - 0.5)))[:, None].astype(np.float32) * step).astype(np.float32)
+    dd = oracle.d2_pairs(R, S, np.stack([np.arange(n), np.arange(n)], 1))
+    T = float(np.sort(dd)[n // 2])
+    st = run_check(tj, oracle, R, S, T, "none", method="plain_fp32", name="plain32 near ties")
+    assert st["fp64_refined"] > 0, st
+    base = rng.standard_normal((400, 20)).astype(np.float32)
+    for scale in (1e-30, 1e-8, 1e12, 1e30):
+        X = (base * np.float32(scale)).astype(np.float32)
+        run_check(tj, oracle, X, None, sample_T(oracle, X, None, 0.02), "none", method="plain_fp32",
+                  name=f"plain32 scale {scale}")
+    Z = np.zeros((300, 16), np.float32)
+    Z[150:] = rng.standard_normal((150, 16)).astype(np.float32)
+    Z[200:250] = Z[150:200]
+    run_check(tj, oracle, Z, None, 0.0, "none", method="plain_fp32", name="plain32 zeros and duplicates T=0")
+
+
 @pytest.mark.parametrize("prec", PRECS)
 @pytest.mark.parametrize("d", [3, 16, 32, 100, 128, 256])
 def test_certified_random(tj, oracle, prec, d):
@@ -245,7 +284,7 @@ def test_canonical_and_matrix(tj, oracle):
     X = rng.standard_normal((600, 16)).astype(np.float32)
     T = sample_T(oracle, X, None, 0.01)
     want = oracle.canonical_u64(oracle.join(X, None, T), 600)
-    for m in ("F16", "S16", "F8", "S8", "F16X3", "F32"):
+    for m in ("F16", "S16", "F8", "S8", "F16X3", "F32", "P32"):
         say("Matrix.run", m)
         ids, stats = tj.Matrix(T).run(m, X)
         assert np.array_equal(ids, want), m
