@@ -133,6 +133,7 @@ def main() -> int:
                     help="other screen precisions timed on the same join (comma list; '' for none; "
                          "default: the 16-bit hi/lo splits for certified_d128)")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-tune-tile", action="store_true", help="run the screen's default tile (no tile tuning)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-host", action="store_true", help="skip the host-to-host pipeline leg")
     ap.add_argument("--prune", dest="prune", action="store_true", default=None,
@@ -187,6 +188,17 @@ def main() -> int:
     log("first run stats", {k: v for k, v in res.stats.items() if k != "plan"})
     out = torch.empty((plan.out_capacity, 2), dtype=tdt, device="cuda")
 
+    tile = None
+    if not grid and not args.no_tune_tile:
+        # screen tile: each candidate (row blocks per wave x column split
+        # width) timed on this join; the headline runs the fastest
+        from tensorjoin_variants_amd.autotune import Autotuner
+        d0 = plan.describe()
+        rep = Autotuner(cache=None).tune_tile(plan, R, S, w.T, out=out, fmt=fmt)
+        tile = {"default": {"row_blocks": d0["row_blocks"], "cols_per_split": d0["cols_per_split"]},
+                "tuned": rep["winner"], "candidates_ms": rep["ms"]}
+        log("tile", json.dumps(tile["tuned"]), "of", len(rep["ms"]), "candidates")
+
     # ---- timed region: K joins back to back, no host synchronisation inside
     ms_per_step = timed(plan, R, S, w.T, out, fmt, args.steps, args.warmup)
     stats = _lib.decode_stats(plan._stats.cpu().numpy().view(np.uint64))
@@ -233,6 +245,8 @@ def main() -> int:
     }
     if grid:
         result["grid_candidate_evals"] = stats["grid_candidates"]
+    if tile is not None:
+        result["screen_tile"] = tile
 
     if args.variants is None:
         args.variants = "fp16x3,bf16x3" if args.workload == "certified_d128" else ""

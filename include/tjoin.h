@@ -105,6 +105,13 @@ int tj_plan_create(tj_plan** plan, int64_t nR, int64_t nS /* 0 = self-join */, i
                    int method, int precision, int64_t out_capacity, int64_t amb_capacity,
                    unsigned flags);
 int tj_plan_destroy(tj_plan* plan);
+/* Screen tile of a certified plan: row_blocks = 32-row blocks per wave (0 =
+ * the body's default; the four-block R x S FP16 / BF16 bodies of width 64 and
+ * 128 also run with 2), cols_per_split = columns of S per work item (0 = the
+ * default rule, rounded up to 32).  Never changes the output, only the time;
+ * drops a captured graph.  Replaces the fixed BLOCK_M = BLOCK_N = 64 tile of
+ * CEJ/src/run_g2b_tensorjoin_public.py:41-42 (the reference does not tune). */
+int tj_plan_set_tile(tj_plan* plan, int row_blocks, int64_t cols_per_split);
 int tj_plan_workspace_bytes(const tj_plan* plan, size_t* bytes);
 int tj_plan_describe(const tj_plan* plan, char* buf, size_t len); /* host only */
 /* Milliseconds of the stages of the last tj_join on a TJ_FLAG_TIMING plan:

@@ -59,6 +59,7 @@ def load():
         "tj_device_info": (i32, [C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]),
         "tj_plan_create": (i32, [C.POINTER(vp), i64, i64, i32, i32, i32, i64, i64, C.c_uint]),
         "tj_plan_destroy": (i32, [vp]),
+        "tj_plan_set_tile": (i32, [vp, i32, i64]),
         "tj_plan_workspace_bytes": (i32, [vp, C.POINTER(sz)]),
         "tj_plan_describe": (i32, [vp, C.c_char_p, sz]),
         "tj_plan_stage_times": (i32, [vp, C.POINTER(C.c_float), i32]),

@@ -3,7 +3,9 @@
 Every choice is certified (the output never depends on it), so the tuner
 only trades speed.  ``choose_precision`` is the static default; ``Autotuner``
 times candidate precisions on a sample of the actual join with HIP events and
-keeps the winner per (d, size class) in an in-memory table.  The table is
+keeps the winner per (d, size class) in an in-memory table, and times screen
+tiles (row blocks per wave x column split width) on a plan's own join and
+keeps the winner per (d, size class, precision).  The table is
 written to a JSON file only when one is named (the ``cache`` argument or the
 TJ_AUTOTUNE_CACHE environment variable); by default tuning leaves no file.
 """
@@ -93,6 +95,59 @@ class Autotuner(_Table):
         self.table[self.key(R.shape[1], R.shape[0], nS)] = win
         self.save()
         return {"winner": win, "ms": times}
+
+    def tile_key(self, d: int, nR: int, nS: int | None, precision: str) -> str:
+        return self.key(d, nR, nS) + f"_{precision}_tile"
+
+    def lookup_tile(self, d: int, nR: int, nS: int | None, precision: str) -> tuple[int, int] | None:
+        v = self.table.get(self.tile_key(d, nR, nS, precision))
+        return None if v is None else (int(v[0]), int(v[1]))
+
+    @staticmethod
+    def tile_candidates(plan) -> list[tuple[int, int]]:
+        """Row blocks per wave (the body's default and, for the four-block
+        bodies, two) x column split widths (the default rule's, halved,
+        doubled, quartered, and all of S in one split)."""
+        desc = plan.describe()
+        rb0, cps = desc["row_blocks"], desc["cols_per_split"]
+        nS = plan.nS or plan.nR
+        rbs = [rb0] + ([2] if rb0 == 4 else [])
+        widths = sorted({w for w in (cps // 4, cps // 2, cps, 2 * cps, 4 * cps, nS) if 1024 <= w <= nS} | {cps})
+        return [(rb, w) for rb in rbs for w in widths]
+
+    def tune_tile(self, plan, R, S, T: float, candidates=None, reps: int = 2, out=None, fmt: str = "int64") -> dict:
+        """Time each screen tile on ``plan``'s own join (the full R and S it
+        was made for) and keep the fastest per (d, size class, precision).
+        Leaves the winner set on the plan."""
+        from .join import torch
+        t = torch()
+        if plan.method != "certified":
+            raise ValueError("only a certified plan has a screen tile")
+        if candidates is None:
+            plan.set_tile(0, 0)
+            candidates = self.tile_candidates(plan)
+        if out is None:
+            out = t.empty((max(plan.out_capacity, 1), 2), dtype=t.int64 if fmt == "int64" else t.int32,
+                          device=plan.device)
+        times = {}
+        for rb, w in candidates:
+            plan.set_tile(rb, w)
+            plan.enqueue(R, S, T, out, fmt)  # warm
+            best = float("inf")
+            for _ in range(reps):
+                a, b = t.cuda.Event(enable_timing=True), t.cuda.Event(enable_timing=True)
+                a.record()
+                plan.enqueue(R, S, T, out, fmt)
+                b.record()
+                b.synchronize()
+                best = min(best, a.elapsed_time(b))
+            times[(rb, w)] = best
+        win = min(times, key=times.get)
+        plan.set_tile(*win)
+        self.table[self.tile_key(plan.d, plan.nR, plan.nS, plan.precision)] = list(win)
+        self.save()
+        return {"winner": {"row_blocks": win[0], "cols_per_split": win[1]},
+                "ms": [{"row_blocks": rb, "cols_per_split": w, "ms": ms} for (rb, w), ms in times.items()]}
 
 
 def width_class(d: int) -> int:

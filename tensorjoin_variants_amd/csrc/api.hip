@@ -16,12 +16,12 @@
 #include "plan_impl.h"
 
 namespace tj {
-int screen_fp16(int dp, bool self_join, const ScreenArgs& a, int grid, hipStream_t s, int query);
-int screen_fp16x3(int dp, bool self_join, const ScreenArgs& a, int grid, hipStream_t s, int query);
-int screen_bf16(int dp, bool self_join, const ScreenArgs& a, int grid, hipStream_t s, int query);
-int screen_bf16x3(int dp, bool self_join, const ScreenArgs& a, int grid, hipStream_t s, int query);
-int screen_fp8(int dp, bool self_join, const ScreenArgs& a, int grid, hipStream_t s, int query);
-int screen_fp8x3(int dp, bool self_join, const ScreenArgs& a, int grid, hipStream_t s, int query);
+int screen_fp16(int dp, bool self_join, const ScreenArgs& a, int grid, hipStream_t s, int query, int rb);
+int screen_fp16x3(int dp, bool self_join, const ScreenArgs& a, int grid, hipStream_t s, int query, int rb);
+int screen_bf16(int dp, bool self_join, const ScreenArgs& a, int grid, hipStream_t s, int query, int rb);
+int screen_bf16x3(int dp, bool self_join, const ScreenArgs& a, int grid, hipStream_t s, int query, int rb);
+int screen_fp8(int dp, bool self_join, const ScreenArgs& a, int grid, hipStream_t s, int query, int rb);
+int screen_fp8x3(int dp, bool self_join, const ScreenArgs& a, int grid, hipStream_t s, int query, int rb);
 int launch_mirror(const int64_t* pairs, const uint64_t* count, int64_t cap, int64_t n, uint64_t* ids, uint64_t* total,
                   hipStream_t s);
 
@@ -48,38 +48,41 @@ int num_cus() {
     return cus;
 }
 
-static int screen_call(int prec, int dp, bool self_join, const ScreenArgs& a, int grid, hipStream_t s, int query) {
+static int screen_call(int prec, int dp, bool self_join, const ScreenArgs& a, int grid, hipStream_t s, int query,
+                       int rb) {
     switch (prec) {
-        case P_FP16: return screen_fp16(dp, self_join, a, grid, s, query);
-        case P_FP16X3: return screen_fp16x3(dp, self_join, a, grid, s, query);
-        case P_BF16: return screen_bf16(dp, self_join, a, grid, s, query);
-        case P_BF16X3: return screen_bf16x3(dp, self_join, a, grid, s, query);
-        case P_FP8: return screen_fp8(dp, self_join, a, grid, s, query);
-        case P_FP8X3: return screen_fp8x3(dp, self_join, a, grid, s, query);
+        case P_FP16: return screen_fp16(dp, self_join, a, grid, s, query, rb);
+        case P_FP16X3: return screen_fp16x3(dp, self_join, a, grid, s, query, rb);
+        case P_BF16: return screen_bf16(dp, self_join, a, grid, s, query, rb);
+        case P_BF16X3: return screen_bf16x3(dp, self_join, a, grid, s, query, rb);
+        case P_FP8: return screen_fp8(dp, self_join, a, grid, s, query, rb);
+        case P_FP8X3: return screen_fp8x3(dp, self_join, a, grid, s, query, rb);
         default: set_error("unknown precision %d", prec); return TJ_ERR_INVALID;
     }
 }
-int launch_screen(int prec, int dp, bool self_join, const ScreenArgs& a, int grid, hipStream_t s) {
-    return screen_call(prec, dp, self_join, a, grid, s, 0);
+int launch_screen(int prec, int dp, bool self_join, const ScreenArgs& a, int grid, hipStream_t s, int rb) {
+    return screen_call(prec, dp, self_join, a, grid, s, 0, rb);
 }
-int screen_rows_per_wg(int prec, int dp, bool self_join) {
+int screen_rows_per_wg(int prec, int dp, bool self_join, int rb) {
     ScreenArgs dummy{};
-    return screen_call(prec, dp, self_join, dummy, 0, nullptr, 2);
+    return screen_call(prec, dp, self_join, dummy, 0, nullptr, 2, rb);
 }
-int screen_max_grid(int prec, int dp, bool self_join) {
+int screen_max_grid(int prec, int dp, bool self_join, int rb) {
     ScreenArgs dummy{};
-    int occ = screen_call(prec, dp, self_join, dummy, 0, nullptr, 1);
+    int occ = screen_call(prec, dp, self_join, dummy, 0, nullptr, 1, rb);
     if (occ < 1) occ = 1;
     return occ * num_cus();
 }
 
-int screen_layout(int64_t nA, int64_t nB, bool self_join, int dp, int prec, ScreenLayout* L) {
+int screen_layout(int64_t nA, int64_t nB, bool self_join, int dp, int prec, ScreenLayout* L, int rb,
+                  int64_t split_cols) {
     const int es = prec_esize(prec);
     const bool x3 = prec_terms(prec) == 3;
-    L->bm = screen_rows_per_wg(prec, dp, self_join);
-    if (L->bm <= 0) return TJ_ERR_UNSUPPORTED;
+    L->bm = screen_rows_per_wg(prec, dp, self_join, rb);
+    if (L->bm <= 0) return L->bm < 0 ? L->bm : TJ_ERR_UNSUPPORTED;
+    L->rb = rb;
     L->panels = (int)((nA + L->bm - 1) / L->bm);
-    const int maxg = screen_max_grid(prec, dp, self_join);
+    const int maxg = screen_max_grid(prec, dp, self_join, rb);
     const int64_t panels = std::max(L->panels, 1);
     int64_t cs;
     if (self_join) {
@@ -102,7 +105,9 @@ int screen_layout(int64_t nA, int64_t nB, bool self_join, int dp, int prec, Scre
         const int64_t by_width = (nB + 1023) / 1024;  // splits of at least 1024 columns
         cs = std::max<int64_t>(1, std::min(std::max(by_l2, want), by_width));
     }
-    const int64_t per = ((nB + cs - 1) / cs + 31) / 32 * 32;
+    int64_t per = ((nB + cs - 1) / cs + 31) / 32 * 32;
+    if (split_cols > 0) per = (split_cols + 31) / 32 * 32;  // the caller's column split width
+    if (per > nB) per = (nB + 31) / 32 * 32;
     L->cols_per_split = (int)std::max<int64_t>(per, 32);
     L->col_splits = (int)((nB + L->cols_per_split - 1) / L->cols_per_split);
     const int64_t items = (int64_t)L->panels * L->col_splits;
@@ -214,7 +219,7 @@ int tj::plan_init(tj_plan* p, int64_t nR, int64_t nS, int d, int method, int pre
         p->off_aux = carve(cur, 4 * (size_t)(p->amb_cap > 0 ? p->amb_cap : 1));
         ScreenLayout sl;
         int rc;
-        if ((rc = screen_layout(nR, p->nS, p->self_join, p->dp, p->prec, &sl))) return rc;
+        if ((rc = screen_layout(nR, p->nS, p->self_join, p->dp, p->prec, &sl, 0, 0))) return rc;
         p->bm = sl.bm;
         p->panels = sl.panels;
         p->col_splits = sl.col_splits;
@@ -275,6 +280,29 @@ int tj_plan_destroy(tj_plan* plan) {
     return TJ_OK;
 }
 
+int tj_plan_set_tile(tj_plan* p, int row_blocks, int64_t cols_per_split) {
+    TJ_CHECK(p != nullptr, TJ_ERR_INVALID, "plan is NULL");
+    TJ_CHECK(p->method == TJ_METHOD_CERTIFIED, TJ_ERR_INVALID, "only a certified plan has a screen tile");
+    TJ_CHECK(row_blocks >= 0 && cols_per_split >= 0, TJ_ERR_INVALID, "negative tile parameter");
+    TJ_CHECK(cols_per_split <= ((int64_t)1 << 30), TJ_ERR_INVALID, "column split of %lld columns",
+             (long long)cols_per_split);
+    ScreenLayout sl;
+    int rc;
+    if ((rc = screen_layout(p->nR, p->nS, p->self_join, p->dp, p->prec, &sl, row_blocks, cols_per_split))) return rc;
+    if (p->has_graph) {  // a captured graph holds the old launch shape
+        (void)hipGraphExecDestroy(p->graph);
+        p->has_graph = false;
+    }
+    p->tile_rb = row_blocks;
+    p->tile_split = cols_per_split;
+    p->bm = sl.bm;
+    p->panels = sl.panels;
+    p->col_splits = sl.col_splits;
+    p->cols_per_split = sl.cols_per_split;
+    p->grid = sl.grid;
+    return TJ_OK;
+}
+
 int tj_plan_workspace_bytes(const tj_plan* plan, size_t* bytes) {
     TJ_CHECK(plan && bytes, TJ_ERR_INVALID, "NULL argument");
     *bytes = plan->total;
@@ -286,9 +314,9 @@ int tj_plan_describe(const tj_plan* p, char* buf, size_t len) {
     snprintf(buf, len,
              "{\"nR\":%lld,\"nS\":%lld,\"self\":%d,\"d\":%d,\"dp\":%d,\"method\":%d,\"precision\":%d,"
              "\"jbits\":%d,\"keybits\":%d,\"rows_per_wg\":%d,\"panels\":%d,\"col_splits\":%d,"
-             "\"cols_per_split\":%d,\"grid\":%d,\"workspace\":%zu}",
+             "\"cols_per_split\":%d,\"grid\":%d,\"row_blocks\":%d,\"workspace\":%zu}",
              (long long)p->nR, (long long)p->nS, (int)p->self_join, p->d, p->dp, p->method, p->prec, p->jbits,
-             p->keybits, p->bm, p->panels, p->col_splits, p->cols_per_split, p->grid, p->total);
+             p->keybits, p->bm, p->panels, p->col_splits, p->cols_per_split, p->grid, p->bm / 128, p->total);
     return TJ_OK;
 }
 
@@ -362,7 +390,7 @@ int tj::join_enqueue(const JoinCall& c, hipStream_t s) {
             a.jbits = p->jbits;
             a.panels = p->panels; a.col_splits = p->col_splits; a.cols_per_split = p->cols_per_split;
             a.audit = (p->flags & TJ_FLAG_AUDIT) ? 1 : 0;
-            if ((rc = launch_screen(p->prec, p->dp, p->self_join, a, p->grid, s))) return rc;
+            if ((rc = launch_screen(p->prec, p->dp, p->self_join, a, p->grid, s, p->tile_rb))) return rc;
             TJ_MARK(2);
             if ((rc = launch_certify(R, S, p->d, p->jbits, a.amb_keys, a.amb_aux, p->amb_cap, mR, mS, params, out_keys,
                                      p->out_cap, stats, a.audit, s)))
@@ -490,7 +518,7 @@ int tj_screen(int precision, int d, const void* qa_hi, const void* qa_lo, const 
     if (na == 0 || nB == 0) return TJ_OK;
     ScreenLayout sl;
     int rc;
-    if ((rc = screen_layout(na, nB, self_join, dp, precision, &sl))) return rc;
+    if ((rc = screen_layout(na, nB, self_join, dp, precision, &sl, 0, 0))) return rc;
     hipStream_t s = (hipStream_t)stream;
     ScreenArgs a{};
     a.qa_hi = qa_hi; a.qa_lo = x3 ? qa_lo : qa_hi; a.ma = (const RowMeta*)meta_a; a.na = na;
@@ -509,7 +537,7 @@ int tj_screen(int precision, int d, const void* qa_hi, const void* qa_lo, const 
     a.audit = (flags & TJ_FLAG_AUDIT) ? 1 : 0;
     // the persistent grid takes its items from a cursor that starts at zero
     if ((rc = launch_fill_u64(stats + TJ_ST_WORK, 1, 0, s))) return rc;
-    return launch_screen(precision, dp, self_join, a, sl.grid, s);
+    return launch_screen(precision, dp, self_join, a, sl.grid, s, 0);
 }
 
 int tj_certify(const float* R, const float* S, int d, int64_t nS, const uint64_t* cand_keys, const float* cand_s,

@@ -12,6 +12,8 @@ struct tj_plan {
     unsigned flags;
     int jbits, keybits;
     int bm, panels, col_splits, cols_per_split, grid;
+    int tile_rb;           // 32-row blocks per wave of the screen (0 = the body's default)
+    int64_t tile_split;    // column split width of an R x S screen (0 = the default rule)
     size_t off_grid;
     tj::GridLayout glay;
     size_t off_maxabs, off_params, off_metaR, off_metaS, off_qRh, off_qRl, off_qSh, off_qSl, off_out, off_amb,

@@ -759,37 +759,53 @@ constexpr int rb_for() {
     return (!SELF && C::NT == 1 && !C::FP8 && DP >= 64 && DP <= 128) ? 4 : rb_batch<PREC, DP>();
 }
 
+// The alternative row blocking a caller may select for a body (0: none):
+// the four-block bodies can also run as two seeded blocks per wave.
 template <int PREC, int DP, bool SELF>
-int launch_one(const tj::ScreenArgs& a, int grid, hipStream_t s) {
-    constexpr int RB = rb_for<PREC, DP, SELF>();
+constexpr int alt_rb() {
+    return rb_for<PREC, DP, SELF>() == 4 ? 2 : 0;
+}
+
+// query 0: launch; 1: workgroups per CU; 2: rows per workgroup.
+template <int PREC, int DP, bool SELF, int RB>
+int run_rb(int query, const tj::ScreenArgs& a, int grid, hipStream_t s) {
+    if (query == 1) {
+        int blocks = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, screen_kernel<PREC, DP, RB, SELF>, 256, 0) != hipSuccess)
+            return 1;
+        return blocks > 0 ? blocks : 1;
+    }
+    if (query == 2) return 4 * RB * 32;
     hipLaunchKernelGGL((screen_kernel<PREC, DP, RB, SELF>), dim3(grid), dim3(256), 0, s, a);
     return hipGetLastError() == hipSuccess ? TJ_OK : TJ_ERR_HIP;
 }
 
+// rb: 32-row blocks per wave, 0 for the body's default.
 template <int PREC, int DP, bool SELF>
-int occupancy_one() {
-    constexpr int RB = rb_for<PREC, DP, SELF>();
-    int blocks = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, screen_kernel<PREC, DP, RB, SELF>, 256, 0) != hipSuccess) return 1;
-    return blocks > 0 ? blocks : 1;
+int dispatch_rb(int rb, int query, const tj::ScreenArgs& a, int grid, hipStream_t s) {
+    constexpr int R0 = rb_for<PREC, DP, SELF>();
+    constexpr int R1 = alt_rb<PREC, DP, SELF>();
+    if (rb == 0 || rb == R0) return run_rb<PREC, DP, SELF, R0>(query, a, grid, s);
+    if constexpr (R1 > 0) {
+        if (rb == R1) return run_rb<PREC, DP, SELF, R1>(query, a, grid, s);
+    }
+    tj::set_error("screen: %d row blocks per wave are not built for this precision and width", rb);
+    return TJ_ERR_UNSUPPORTED;
 }
 
 }  // namespace tjscreen
 
 // Instantiate the dispatch for one precision over the supported widths.
-#define TJ_SCREEN_DISPATCH(NAME, PREC, ...)                                                       \
-    namespace tj {                                                                                \
-    int NAME(int dp, bool self_join, const ScreenArgs& a, int grid, hipStream_t s, int query) {  \
-        switch (dp) {                                                                             \
-            __VA_ARGS__                                                                           \
-            default: set_error("screen: unsupported padded width %d", dp); return TJ_ERR_UNSUPPORTED; \
-        }                                                                                         \
-    }                                                                                             \
+#define TJ_SCREEN_DISPATCH(NAME, PREC, ...)                                                              \
+    namespace tj {                                                                                       \
+    int NAME(int dp, bool self_join, const ScreenArgs& a, int grid, hipStream_t s, int query, int rb) { \
+        switch (dp) {                                                                                    \
+            __VA_ARGS__                                                                                  \
+            default: set_error("screen: unsupported padded width %d", dp); return TJ_ERR_UNSUPPORTED;   \
+        }                                                                                                \
+    }                                                                                                    \
     }
-#define TJ_CASE(PREC, DPV)                                                                                    \
-    case DPV:                                                                                                 \
-        if (query == 1) return self_join ? tjscreen::occupancy_one<PREC, DPV, true>()                        \
-                                         : tjscreen::occupancy_one<PREC, DPV, false>();                      \
-        if (query == 2) return 4 * (self_join ? tjscreen::rb_for<PREC, DPV, true>()                          \
-                                              : tjscreen::rb_for<PREC, DPV, false>()) * 32;                  \
-        return self_join ? tjscreen::launch_one<PREC, DPV, true>(a, grid, s) : tjscreen::launch_one<PREC, DPV, false>(a, grid, s);
+#define TJ_CASE(PREC, DPV)                                                             \
+    case DPV:                                                                          \
+        return self_join ? tjscreen::dispatch_rb<PREC, DPV, true>(rb, query, a, grid, s) \
+                         : tjscreen::dispatch_rb<PREC, DPV, false>(rb, query, a, grid, s);

@@ -242,13 +242,16 @@ struct ScreenArgs {
     int panels, col_splits, cols_per_split; // work decomposition
     int audit;
 };
-int launch_screen(int prec, int dp, bool self_join, const ScreenArgs& a, int grid, hipStream_t s);
-int screen_rows_per_wg(int prec, int dp, bool self_join);
-int screen_max_grid(int prec, int dp, bool self_join);
+// rb: 32-row blocks per wave of the body, 0 for its default.
+int launch_screen(int prec, int dp, bool self_join, const ScreenArgs& a, int grid, hipStream_t s, int rb);
+int screen_rows_per_wg(int prec, int dp, bool self_join, int rb);
+int screen_max_grid(int prec, int dp, bool self_join, int rb);
 // Work decomposition of the persistent screen over an nA x nB join: row
-// panels of bm rows, column splits, and the persistent grid.
-struct ScreenLayout { int bm, panels, col_splits, cols_per_split, grid; };
-int screen_layout(int64_t nA, int64_t nB, bool self_join, int dp, int prec, ScreenLayout* L);
+// panels of bm rows, column splits, and the persistent grid.  rb and
+// split_cols (column split width) are tile choices, 0 for the defaults.
+struct ScreenLayout { int bm, panels, col_splits, cols_per_split, grid, rb; };
+int screen_layout(int64_t nA, int64_t nB, bool self_join, int dp, int prec, ScreenLayout* L, int rb,
+                  int64_t split_cols);
 
 int launch_scale_scan(const float* x, int64_t n, int d, uint32_t* maxabs, uint64_t* stats, hipStream_t s);
 int launch_params(const uint32_t* maxabs, double T, int d, int dp, int prec, JoinParams* p, uint64_t* stats, hipStream_t s);

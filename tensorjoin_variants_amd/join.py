@@ -76,15 +76,26 @@ class Plan:
 
     def __init__(self, nR: int, nS: int | None, d: int, method: str = "certified",
                  precision: str = "fp16", out_capacity: int = 1 << 20, amb_capacity: int = 1 << 20,
-                 audit: bool = False, device: str = "cuda", timing: bool = False):
+                 audit: bool = False, device: str = "cuda", timing: bool = False,
+                 tile: tuple[int, int] | None = None):
         self.nR, self.nS, self.d = int(nR), (None if nS is None else int(nS)), int(d)
         self.method, self.precision = method, precision
         self.audit = audit
         self.timing = timing
         self.device = device
+        self.tile = None
         self._plan = None
         self._ws = None
         self._make(int(out_capacity), int(amb_capacity))
+        if tile is not None:
+            self.set_tile(*tile)
+
+    def set_tile(self, row_blocks: int, cols_per_split: int) -> None:
+        """Screen tile of a certified plan: 32-row blocks per wave and columns
+        of S per work item (0 = the default of each).  Output-neutral."""
+        _lib.check(_lib.load().tj_plan_set_tile(self._plan, int(row_blocks), int(cols_per_split)),
+                   "tj_plan_set_tile")
+        self.tile = (int(row_blocks), int(cols_per_split))
 
     def _make(self, out_cap: int, amb_cap: int) -> None:
         L = _lib.load()
@@ -96,6 +107,8 @@ class Plan:
                               (_lib.FLAG_AUDIT if self.audit else 0) | (_lib.FLAG_TIMING if self.timing else 0))
         _lib.check(rc, "tj_plan_create")
         self._plan = p
+        if self.tile is not None:  # a regrown plan keeps its tile
+            _lib.check(L.tj_plan_set_tile(p, *self.tile), "tj_plan_set_tile")
         self.out_capacity, self.amb_capacity = out_cap, amb_cap
         nbytes = C.c_size_t()
         _lib.check(L.tj_plan_workspace_bytes(p, C.byref(nbytes)), "tj_plan_workspace_bytes")

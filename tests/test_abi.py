@@ -62,6 +62,49 @@ def test_self_join_plan(L):
     L.tj_plan_destroy(p)
 
 
+def _desc(L, p):
+    buf = C.create_string_buffer(1024)
+    assert L.tj_plan_describe(p, buf, 1024) == 0
+    return json.loads(buf.value.decode())
+
+
+def test_plan_set_tile(L):
+    rc, p = _plan(L, 100000, 262144, 128, 0, _lib.PREC["fp16"])
+    assert rc == 0
+    d0 = _desc(L, p)
+    assert d0["row_blocks"] == 4 and d0["rows_per_wg"] == 512
+    nbytes0 = C.c_size_t()
+    L.tj_plan_workspace_bytes(p, C.byref(nbytes0))
+    # two row blocks per wave, 5000-column splits (rounded up to 5024)
+    assert L.tj_plan_set_tile(p, 2, 5000) == 0
+    d1 = _desc(L, p)
+    assert d1["row_blocks"] == 2 and d1["rows_per_wg"] == 256 and d1["panels"] == (100000 + 255) // 256
+    assert d1["cols_per_split"] == 5024 and d1["col_splits"] == (262144 + 5023) // 5024
+    nbytes1 = C.c_size_t()
+    L.tj_plan_workspace_bytes(p, C.byref(nbytes1))
+    assert nbytes1.value == nbytes0.value  # the workspace does not depend on the tile
+    # a split wider than S is all of S; back to the defaults
+    assert L.tj_plan_set_tile(p, 0, 1 << 30) == 0
+    assert _desc(L, p)["col_splits"] == 1
+    assert L.tj_plan_set_tile(p, 0, 0) == 0
+    assert _desc(L, p) == d0
+    # blockings that are not built, negative values, non-certified plans
+    assert L.tj_plan_set_tile(p, 1, 0) == -3
+    assert L.tj_plan_set_tile(p, 3, 0) == -3
+    assert L.tj_plan_set_tile(p, -1, 0) == -1
+    assert L.tj_plan_set_tile(p, 0, -5) == -1
+    assert L.tj_plan_set_tile(None, 0, 0) == -1
+    assert _desc(L, p) == d0  # a rejected tile leaves the plan as it was
+    L.tj_plan_destroy(p)
+    rc, q = _plan(L, 1000, 1000, 8, 1, 0)  # plain FP64
+    assert rc == 0 and L.tj_plan_set_tile(q, 0, 0) == -1
+    L.tj_plan_destroy(q)
+    rc, q = _plan(L, 1000, 1000, 128, 0, _lib.PREC["fp16x3"])  # a one-block body has no alternative
+    assert rc == 0 and _desc(L, q)["row_blocks"] == 1
+    assert L.tj_plan_set_tile(q, 1, 0) == 0 and L.tj_plan_set_tile(q, 2, 0) == -3
+    L.tj_plan_destroy(q)
+
+
 @pytest.mark.parametrize("args,code", [
     ((10, 10, 0, 0, 1), -1),        # d = 0
     ((10, 10, 8, 7, 1), -1),        # unknown method
