@@ -1,8 +1,9 @@
 // batch.hip -- many independent self-joins of uneven sizes and widths in one
 // pipeline: one scale scan, one parameter pass, one row-metadata pass, one
 // persistent screen launch over every join's panels (one per precision
-// family present: FP16, FP8), one certificate pass, one radix sort and one
-// unpack with per-join offsets.
+// family present: FP16, FP8), one certificate pass, and the output grouped
+// by row (per-row counts, offsets, scatter, per-row sort) with per-join
+// offsets.
 //
 // The reference runs one join at a time and slices its triangular tile list
 // into host batches of 4,096 tiles with a blocking count read after every
@@ -18,8 +19,9 @@
 // consecutively in that order (the batch row space, gbase = first row of a
 // join).  Joins of one (precision, padded width) form a group whose quantized
 // rows and metadata are stored contiguously (grow = first row of a join in
-// its group).  A pair key is (batch row of i) << jbits | (local j), so one
-// sort over all keys orders the output by join, then (i, j).
+// its group).  A pair key is (batch row of i) << jbits | (local j); rows in
+// batch order with their ids sorted are the output ordered by join, then
+// (i, j).
 #include <stdio.h>
 #include <string.h>
 #include <algorithm>
@@ -496,20 +498,176 @@ __global__ void __launch_bounds__(256) batch_prune_kernel(const BatchJoin* __res
     }
 }
 
-// Output keys in permuted row numbers -> the caller's row numbers, i < j.
-__global__ void batch_remap_kernel(uint64_t* __restrict__ keys, const uint64_t* __restrict__ count, int64_t cap,
-                                   int jbits, const int32_t* __restrict__ row_join, const BatchJoin* __restrict__ joins,
-                                   const uint64_t* __restrict__ perm) {
+// ------------------------------------------------------ row-grouped output
+// The output keys (batch row of i, local j) are ordered without a global
+// sort, in three steps:
+//  1. by join: a count pass adds each key to its join's count, an exclusive
+//     scan over the joins gives each join its output range, a scatter pass
+//     moves each key, as its local (i, j) (with tile pruning mapped back
+//     from the permuted rows to the caller's, i < j), into its join's range.
+//     A flush of the screen stages the survivors of one work item together,
+//     so the keys of a join arrive in long runs and a wave adds one count
+//     per run of equal joins among its lanes;
+//  2. by row within a join, one workgroup per join: row counts (LDS
+//     counters for joins of up to ROWS_LDS rows, global ones beyond),
+//     their scan (the row offsets), and a scatter of each j into its row's
+//     range;
+//  3. the per-row sort (rowsort.hip) orders every row and emits the local
+//     (i, j) rows.
+__device__ inline uint64_t shfl_u64(uint64_t v, int src) {
+    const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)v, src);
+    const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(v >> 32), src);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+constexpr int ROWS_LDS = 16384;  // rows of a join whose counters live in LDS (64 KiB)
+constexpr int JR_THREADS = 256;
+
+template <bool SCATTER>
+__global__ void __launch_bounds__(256) batch_bucket_kernel(const uint64_t* __restrict__ keys,
+                                                           const uint64_t* __restrict__ count, int64_t cap, int jbits,
+                                                           const int32_t* __restrict__ row_join,
+                                                           const BatchJoin* __restrict__ joins,
+                                                           const uint64_t* __restrict__ perm,
+                                                           uint32_t* __restrict__ jcnt, const uint64_t* __restrict__ joff,
+                                                           uint64_t* __restrict__ bucket) {
     int64_t cnt = (int64_t)*count;
     if (cnt > cap) cnt = cap;
     const uint64_t jmask = (1ull << jbits) - 1;
-    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < cnt; k += (int64_t)gridDim.x * blockDim.x) {
-        const uint64_t key = keys[k];
-        const int64_t ig = (int64_t)(key >> jbits);
-        const int64_t gb = joins[row_join[ig]].gbase;
-        int64_t a = perm_of(perm, ig, jmask), b = perm_of(perm, gb + (int64_t)(key & jmask), jmask);
-        if (a > b) { const int64_t u = a; a = b; b = u; }
-        keys[k] = ((uint64_t)(gb + a) << jbits) | (uint64_t)b;
+    const int lane = threadIdx.x & 63;
+    // UN groups of 64 consecutive keys per wave step, their loads and table
+    // lookups issued before any is used (overlapping latencies)
+    constexpr int UN = 4;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x * UN;
+    // k0 is wave-uniform: every lane of a wave runs the same iterations
+    for (int64_t k0 = ((int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63)) * UN; k0 < cnt; k0 += stride) {
+        int t[UN];
+        uint64_t local[UN];
+        uint64_t key[UN];
+#pragma unroll
+        for (int u = 0; u < UN; ++u) {
+            const int64_t k = k0 + 64 * u + lane;
+            key[u] = k < cnt ? keys[k] : ~0ull;
+        }
+#pragma unroll
+        for (int u = 0; u < UN; ++u) t[u] = key[u] != ~0ull ? row_join[(int64_t)(key[u] >> jbits)] : -1;
+#pragma unroll
+        for (int u = 0; u < UN; ++u) {
+            local[u] = 0;
+            if (SCATTER && t[u] >= 0) {
+                const int64_t ig = (int64_t)(key[u] >> jbits);
+                int64_t jl = (int64_t)(key[u] & jmask);
+                const int64_t gb = joins[t[u]].gbase;
+                int64_t il = ig - gb;
+                if (perm) {
+                    il = perm_of(perm, ig, jmask);
+                    jl = perm_of(perm, gb + jl, jmask);
+                    if (il > jl) { const int64_t w = il; il = jl; jl = w; }
+                }
+                local[u] = ((uint64_t)il << 32) | (uint64_t)jl;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < UN; ++u) {
+            // runs of equal joins among the lanes (the valid lanes are a prefix)
+            const int tu = t[u];
+            const int prev = __shfl(tu, lane > 0 ? lane - 1 : 0);
+            const bool head = tu >= 0 && (lane == 0 || prev != tu);
+            const uint64_t H = __ballot(head);
+            const int nvalid = __popcll(__ballot(tu >= 0));
+            const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1);  // lanes <= this one
+            const uint64_t hb = H & upto, ha = H & ~upto;
+            const int h = hb ? 63 - __clzll(hb) : 0;          // this lane's run head
+            const int nxt = ha ? __builtin_ctzll(ha) : nvalid;  // the next run's head
+            uint32_t base = 0;
+            if (head) {
+                const uint32_t len = (uint32_t)(nxt - lane);
+                if (SCATTER) base = atomicSub(&jcnt[tu], len) - len;  // the run takes [c - len, c) of the join
+                else atomicAdd(&jcnt[tu], len);
+            }
+            if (SCATTER) {
+                base = (uint32_t)__shfl((int)base, h);
+                if (tu >= 0) bucket[joff[tu] + base + (uint32_t)(lane - h)] = local[u];
+            }
+        }
+    }
+}
+
+// Step 2 for the joins listed in order[0, njl): LDS counters (joins of at
+// most ROWS_LDS rows) or global ones (gcnt, one per batch row).  roff[r] =
+// first output position of batch row r; the first workgroup also writes
+// roff[rows], the total.
+template <bool LDS>
+__global__ void __launch_bounds__(JR_THREADS) batch_join_rows_kernel(const int32_t* __restrict__ order, int njl,
+                                                              const BatchJoin* __restrict__ joins, int nj, int64_t rows,
+                                                              const uint64_t* __restrict__ joff,
+                                                              const uint64_t* __restrict__ bucket,
+                                                              uint32_t* __restrict__ gcnt, uint64_t* __restrict__ roff,
+                                                              uint32_t* __restrict__ outj) {
+    __shared__ uint32_t sc[LDS ? ROWS_LDS : 1];
+    __shared__ uint32_t wsum[JR_THREADS / 64];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (LDS && blockIdx.x == 0 && tid == 0) roff[rows] = joff[nj];
+    for (int q = blockIdx.x; q < njl; q += gridDim.x) {
+        const int t = order[q];
+        const int64_t gb = joins[t].gbase;
+        const int n = joins[t].n;
+        uint32_t* c = LDS ? sc : gcnt + gb;
+        for (int a = tid; a < n; a += JR_THREADS) c[a] = 0;
+        __syncthreads();
+        const uint64_t k0 = joff[t], k1 = joff[t + 1];
+        // UN keys per thread per step, loaded before any is used: a
+        // workgroup per join holds few waves, so the loads must overlap
+        constexpr int UN = 8;
+        for (uint64_t k = k0 + tid; k < k1; k += (uint64_t)JR_THREADS * UN) {
+            uint32_t rows_u[UN];
+#pragma unroll
+            for (int u = 0; u < UN; ++u) {
+                const uint64_t kk = k + (uint64_t)JR_THREADS * u;
+                rows_u[u] = kk < k1 ? (uint32_t)(bucket[kk] >> 32) : 0xFFFFFFFFu;
+            }
+#pragma unroll
+            for (int u = 0; u < UN; ++u)
+                if (rows_u[u] != 0xFFFFFFFFu) atomicAdd(&c[rows_u[u]], 1u);
+        }
+        __syncthreads();
+        // exclusive scan of the row counts, JR_THREADS rows at a time: each
+        // row's first position in the join's range
+        uint32_t run = 0;
+        for (int base = 0; base < n; base += JR_THREADS) {
+            const int a = base + tid;
+            const uint32_t v = a < n ? c[a] : 0;
+            uint32_t x = v;
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t y = (uint32_t)__shfl_up((int)x, o);
+                if (lane >= o) x += y;
+            }
+            if (lane == 63) wsum[wave] = x;
+            __syncthreads();
+            uint32_t before = run;
+            for (int w = 0; w < wave; ++w) before += wsum[w];
+            if (a < n) {
+                c[a] = before + x - v;
+                roff[gb + a] = k0 + before + x - v;
+            }
+            for (int w = 0; w < JR_THREADS / 64; ++w) run += wsum[w];
+            __syncthreads();
+        }
+        for (uint64_t k = k0 + tid; k < k1; k += (uint64_t)JR_THREADS * UN) {
+            uint64_t v[UN];
+#pragma unroll
+            for (int u = 0; u < UN; ++u) {
+                const uint64_t kk = k + (uint64_t)JR_THREADS * u;
+                v[u] = kk < k1 ? bucket[kk] : ~0ull;
+            }
+#pragma unroll
+            for (int u = 0; u < UN; ++u) {
+                if (v[u] == ~0ull) continue;
+                const uint32_t pos = atomicAdd(&c[(uint32_t)(v[u] >> 32)], 1u);
+                outj[k0 + pos] = (uint32_t)v[u];
+            }
+        }
+        __syncthreads();  // the counters are reused by the next join
     }
 }
 
@@ -671,36 +829,6 @@ struct BatchSrc {
     }
 };
 
-// ------------------------------------------------------------------ unpack
-// Sorted keys -> local int32 (i, j) rows, and offsets[t] = first output row
-// of join t (offsets[joins] = total), from the join boundaries in the
-// sorted order.
-__global__ void batch_unpack_kernel(const uint64_t* __restrict__ keys, const uint64_t* __restrict__ count, int64_t cap,
-                                    int jbits, const int32_t* __restrict__ row_join,
-                                    const BatchJoin* __restrict__ joins, int nj, int32_t* __restrict__ pairs,
-                                    int64_t* __restrict__ offsets) {
-    int64_t cnt = (int64_t)*count;
-    if (cnt > cap) cnt = cap;
-    const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    if (cnt == 0) {
-        for (int64_t u = gt; u <= nj; u += stride) offsets[u] = 0;
-        return;
-    }
-    const uint64_t jmask = (1ull << jbits) - 1;
-    for (int64_t k = gt; k < cnt; k += stride) {
-        const uint64_t key = keys[k];
-        const int64_t ig = (int64_t)(key >> jbits);
-        const int t = row_join[ig];
-        pairs[2 * k] = (int32_t)(ig - joins[t].gbase);
-        pairs[2 * k + 1] = (int32_t)(key & jmask);
-        const int tp = k > 0 ? row_join[keys[k - 1] >> jbits] : -1;
-        for (int u = tp + 1; u <= t; ++u) offsets[u] = k;
-        if (k == cnt - 1)
-            for (int u = t + 1; u <= nj; ++u) offsets[u] = cnt;
-    }
-}
-
 }  // namespace tjbatch
 
 using namespace tj;
@@ -736,6 +864,7 @@ struct tj_batch {
     BatchGroup* d_groups;
     int2* d_items;
     int32_t* d_row_join;
+    int32_t* d_row_local;  // row -> its number within its join
     int64_t* d_cmeta;  // row-metadata units before join t
     int64_t* d_cscan;  // scale-scan units before join t
     int64_t units_meta, units_scan;
@@ -746,6 +875,9 @@ struct tj_batch {
     // workspace layout
     size_t off_maxabs, off_cursor, off_params, off_cand, off_aux, off_out, off_sort, total;
     size_t off_pivots, off_perm, off_balls, off_cend, off_rows;
+    size_t off_jcnt, off_jpart, off_bucket, off_rcnt, off_roff, off_long;  // row-grouped output
+    int32_t* d_jorder;  // joins of at most ROWS_LDS rows, then the others, each largest first
+    int64_t n_small;
     hipEvent_t ev[5];
     bool timing;
     hipGraphExec_t graph;
@@ -869,23 +1001,44 @@ static int batch_enqueue(const BatchCall& c, hipStream_t s) {
                            b->cand_cap, out_keys, b->out_cap, stats, (b->flags & TJ_FLAG_AUDIT) ? 1 : 0);
         TJ_HIP(hipGetLastError());
     }
-    if (b->prune && b->out_cap > 0) {
-        int64_t blocks = (b->out_cap + 255) / 256;
-        if (blocks > 2048) blocks = 2048;
-        hipLaunchKernelGGL(batch_remap_kernel, dim3((unsigned)blocks), dim3(256), 0, s, out_keys, stats + TJ_ST_OUT,
-                           b->out_cap, b->jbits, b->d_row_join, b->d_joins, perm);
-        TJ_HIP(hipGetLastError());
-    }
     TJ_MARK(3);
-    uint64_t* sorted = out_keys;
-    if ((rc = launch_sort(ws + b->off_sort, out_keys, stats + TJ_ST_OUT, b->out_cap, b->keybits, s, &sorted))) return rc;
     {
-        int64_t work = std::max<int64_t>(b->out_cap, b->nj + 1);
-        int64_t blocks = (work + 255) / 256;
-        if (blocks > 2048) blocks = 2048;
-        hipLaunchKernelGGL(batch_unpack_kernel, dim3((unsigned)blocks), dim3(256), 0, s, sorted, stats + TJ_ST_OUT,
-                           b->out_cap, b->jbits, b->d_row_join, b->d_joins, b->nj, c.pairs, c.offsets);
+        // output: keys by join (into the bucket), rows per join, per-row
+        // sort + emit; the join offsets are the caller's offsets array
+        uint32_t* jcnt = (uint32_t*)(ws + b->off_jcnt);
+        uint64_t* joff = (uint64_t*)c.offsets;
+        uint64_t* bucket = (uint64_t*)(ws + b->off_bucket);
+        uint64_t* roff = (uint64_t*)(ws + b->off_roff);
+        uint32_t* outj = (uint32_t*)out_keys;  // the keys are in the bucket by then
+        uint32_t* outj2 = outj + b->out_cap;
+        if ((rc = launch_fill_u64((uint64_t*)jcnt, (b->nj + 1) / 2, 0, s))) return rc;
+        int64_t blocks = (b->out_cap + 255) / 256;
+        if (blocks > (int64_t)num_cus() * 16) blocks = (int64_t)num_cus() * 16;
+        if (blocks < 1) blocks = 1;
+        hipLaunchKernelGGL(batch_bucket_kernel<false>, dim3((unsigned)blocks), dim3(256), 0, s, out_keys,
+                           stats + TJ_ST_OUT, b->out_cap, b->jbits, b->d_row_join, b->d_joins, perm, jcnt,
+                           (const uint64_t*)nullptr, (uint64_t*)nullptr);
         TJ_HIP(hipGetLastError());
+        if ((rc = launch_row_offsets(jcnt, b->nj, (uint64_t*)(ws + b->off_jpart), joff, nullptr, s))) return rc;
+        hipLaunchKernelGGL(batch_bucket_kernel<true>, dim3((unsigned)blocks), dim3(256), 0, s, out_keys,
+                           stats + TJ_ST_OUT, b->out_cap, b->jbits, b->d_row_join, b->d_joins, perm, jcnt, joff,
+                           bucket);
+        TJ_HIP(hipGetLastError());
+        const int g0 = (int)std::max<int64_t>(1, std::min<int64_t>(b->n_small, 2 * (int64_t)num_cus()));
+        hipLaunchKernelGGL(batch_join_rows_kernel<true>, dim3((unsigned)g0), dim3(JR_THREADS), 0, s, b->d_jorder,
+                           (int)b->n_small, b->d_joins, b->nj, b->rows, joff, bucket, (uint32_t*)(ws + b->off_rcnt),
+                           roff, outj);
+        TJ_HIP(hipGetLastError());
+        if (b->nj > b->n_small) {
+            const int g1 = (int)std::min<int64_t>(b->nj - b->n_small, 2 * (int64_t)num_cus());
+            hipLaunchKernelGGL(batch_join_rows_kernel<false>, dim3((unsigned)g1), dim3(JR_THREADS), 0, s,
+                               b->d_jorder + b->n_small, (int)(b->nj - b->n_small), b->d_joins, b->nj, b->rows, joff,
+                               bucket, (uint32_t*)(ws + b->off_rcnt), roff, outj);
+            TJ_HIP(hipGetLastError());
+        }
+        if (b->rows > 0 && (rc = launch_row_sort(roff, b->rows, outj, outj2, b->out_cap, GRID_EMIT_I32, c.pairs,
+                                                 b->d_row_local, (uint32_t*)(ws + b->off_long), s)))
+            return rc;
     }
     TJ_MARK(4);
 #undef TJ_MARK
@@ -989,7 +1142,17 @@ int tj_batch_create(tj_batch** out, int joins, const int64_t* n, const int* d, c
     b->off_cand = bcarve(cur, 8 * (size_t)std::max<int64_t>(cand_capacity, 1));
     b->off_aux = bcarve(cur, 4 * (size_t)std::max<int64_t>(cand_capacity, 1));
     b->off_out = bcarve(cur, 8 * (size_t)std::max<int64_t>(out_capacity, 1));
-    b->off_sort = bcarve(cur, sort_temp_bytes(b->prune ? std::max<int64_t>(out_capacity, rows) : out_capacity));
+    // row-grouped output: per-join counts, the keys bucketed by join, the
+    // global row counters of joins beyond ROWS_LDS rows, the row offsets,
+    // the list of long rows.  The ids in row order and the long rows' merge
+    // buffer reuse the output keys (two uint32 per key).
+    b->off_jcnt = bcarve(cur, 4 * (size_t)(joins + 2));
+    b->off_jpart = bcarve(cur, row_scan_part_bytes(joins));
+    b->off_bucket = bcarve(cur, 8 * (size_t)std::max<int64_t>(out_capacity, 1));
+    b->off_rcnt = bcarve(cur, 4 * (size_t)(rows + 2));
+    b->off_roff = bcarve(cur, 8 * (size_t)(rows + 1));
+    b->off_long = bcarve(cur, row_sort_scratch_bytes(rows));
+    b->off_sort = b->prune ? bcarve(cur, sort_temp_bytes(std::max<int64_t>(rows, 1))) : 0;  // the spatial order
     b->off_pivots = b->off_perm = b->off_balls = b->off_cend = b->off_rows = 0;
     if (b->prune) {
         b->off_pivots = bcarve(cur, 4 * (size_t)NPIV * joins);
@@ -1030,8 +1193,20 @@ int tj_batch_create(tj_batch** out, int joins, const int64_t* n, const int* d, c
     std::vector<int2> it(items.size() > 0 ? items.size() : 1);
     for (size_t k = 0; k < items.size(); ++k) it[k] = make_int2(items[k].t, items[k].p);
     std::vector<int32_t> rj(rows > 0 ? rows : 1);
+    std::vector<int32_t> jo(joins);
+    for (int t = 0; t < joins; ++t) jo[t] = t;
+    std::stable_sort(jo.begin(), jo.end(), [&](int x, int y) {
+        const bool sx = J[x].n <= ROWS_LDS, sy = J[y].n <= ROWS_LDS;
+        return sx != sy ? sx : J[x].n > J[y].n;
+    });
+    int64_t n_small = 0;
+    for (int t = 0; t < joins; ++t) n_small += J[t].n <= ROWS_LDS ? 1 : 0;
+    std::vector<int32_t> rl(rows > 0 ? rows : 1);
     for (int t = 0; t < joins; ++t)
-        for (int64_t r = 0; r < J[t].n; ++r) rj[J[t].gbase + r] = t;
+        for (int64_t r = 0; r < J[t].n; ++r) {
+            rj[J[t].gbase + r] = t;
+            rl[J[t].gbase + r] = (int32_t)r;
+        }
     std::vector<int64_t> cmeta(joins + 1, 0), cscan(joins + 1, 0);
     for (int t = 0; t < joins; ++t) {
         const int rpw = 64 / tjmeta::group_width(J[t].dp);
@@ -1052,6 +1227,8 @@ int tj_batch_create(tj_batch** out, int joins, const int64_t* n, const int* d, c
     const size_t o_g = bcarve(tcur, sizeof(BatchGroup) * G.size());
     const size_t o_i = bcarve(tcur, sizeof(int2) * it.size());
     const size_t o_r = bcarve(tcur, sizeof(int32_t) * rj.size());
+    const size_t o_l = bcarve(tcur, sizeof(int32_t) * rl.size());
+    const size_t o_o = bcarve(tcur, sizeof(int32_t) * jo.size());
     const size_t o_cm = bcarve(tcur, sizeof(int64_t) * cmeta.size());
     const size_t o_cs = bcarve(tcur, sizeof(int64_t) * cscan.size());
     hipError_t e = hipMalloc(&b->dev, tcur);
@@ -1064,12 +1241,17 @@ int tj_batch_create(tj_batch** out, int joins, const int64_t* n, const int* d, c
     b->d_groups = (BatchGroup*)(base + o_g);
     b->d_items = (int2*)(base + o_i);
     b->d_row_join = (int32_t*)(base + o_r);
+    b->d_row_local = (int32_t*)(base + o_l);
+    b->d_jorder = (int32_t*)(base + o_o);
+    b->n_small = n_small;
     b->d_cmeta = (int64_t*)(base + o_cm);
     b->d_cscan = (int64_t*)(base + o_cs);
     if ((e = hipMemcpy(b->d_joins, J.data(), sizeof(BatchJoin) * J.size(), hipMemcpyHostToDevice)) != hipSuccess ||
         (e = hipMemcpy(b->d_groups, G.data(), sizeof(BatchGroup) * G.size(), hipMemcpyHostToDevice)) != hipSuccess ||
         (e = hipMemcpy(b->d_items, it.data(), sizeof(int2) * it.size(), hipMemcpyHostToDevice)) != hipSuccess ||
         (e = hipMemcpy(b->d_row_join, rj.data(), sizeof(int32_t) * rj.size(), hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(b->d_row_local, rl.data(), sizeof(int32_t) * rl.size(), hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(b->d_jorder, jo.data(), sizeof(int32_t) * jo.size(), hipMemcpyHostToDevice)) != hipSuccess ||
         (e = hipMemcpy(b->d_cmeta, cmeta.data(), sizeof(int64_t) * cmeta.size(), hipMemcpyHostToDevice)) != hipSuccess ||
         (e = hipMemcpy(b->d_cscan, cscan.data(), sizeof(int64_t) * cscan.size(), hipMemcpyHostToDevice)) != hipSuccess) {
         (void)hipFree(b->dev);
