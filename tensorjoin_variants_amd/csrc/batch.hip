@@ -1037,7 +1037,7 @@ static int batch_enqueue(const BatchCall& c, hipStream_t s) {
             TJ_HIP(hipGetLastError());
         }
         if (b->rows > 0 && (rc = launch_row_sort(roff, b->rows, outj, outj2, b->out_cap, GRID_EMIT_I32, c.pairs,
-                                                 b->d_row_local, (uint32_t*)(ws + b->off_long), s)))
+                                                 b->d_row_local, (uint32_t*)(ws + b->off_long), nullptr, s)))
             return rc;
     }
     TJ_MARK(4);

@@ -5,36 +5,43 @@
 // GDS/GPU.cu:775-872 batches with thrust::sort_by_key of the results).
 // Same index idea -- cells of side >= eps, a query only visits its own and
 // the adjacent cells -- rebuilt for MI355X:
-//  * the index is built on the device: bounds (order-preserving integer
-//    atomics), cell keys, the radix sort by cell, S gathered into cell order
-//    as 16-byte rows, and an open-addressing hash of the non-empty cells
-//    (memory bounded by 2|S| slots however sparse the grid is);
-//  * when the grid has at most 64 |S| cells, no hash at all: a bitmap of
-//    the non-empty cells with a prefix count per 64-bit word ranks every
-//    cell, and runs[rank] is where its S rows start.  The three x-adjacent
-//    cells of a neighbour row are consecutive ids, so their S rows are one
-//    contiguous range [runs[rank(mid - 1)], runs[rank(mid - 1) + occupied]):
-//    a query reads nine bitmap words (shared by the cell-ordered lanes of a
-//    wave), nine prefix counts and 18 run bounds, where hashing cost a probe
-//    sequence of random 64-byte lines per neighbour cell, empty or not;
-//  * queries run in cell order (R is gathered into cell order as 16-byte
-//    rows too; a self-join reuses S's), so the 64 lanes of a wave visit the
-//    same neighbour cells and the same S rows, and read their queries,
-//    counts and offsets contiguously;
-//  * output without a global sort: a count pass stores each query's number
-//    of neighbours, an exclusive scan turns the counts into row offsets, a
-//    write pass stores every neighbour id in its row's final range, and a
-//    per-row sort orders each row (by its lane, its wave or, for rows beyond
-//    512 ids, its workgroup; rowsort.hip) while it emits the (i, j) rows -- the same
-//    shape as GDS-Join's neighbour table (counts per point + ids grouped by
-//    point, GDS/main.cu:507-540), which the table output gives directly;
+//  * the index is built on the device: bounds, cell keys (cell id << index
+//    bits | point index), the radix sort of the keys (the digits wholly
+//    inside the index need no pass: the keys are written in index order),
+//    and the points written in key order -- coordinates widened to binary64
+//    (exact) and the original index, 32 bytes a row, so the probe converts
+//    nothing per candidate; S's non-empty cells go into an occupancy bitmap
+//    with a prefix count per word (grids of at most 64 cells per S point) or
+//    an open-addressing hash of [start, end) runs (sparser grids).  (A
+//    counting sort over the cells, one device atomic per point, measured
+//    slower: scattered device atomics cost more than the sort's passes);
+//  * with the bitmap the three x-adjacent cells of a neighbour row are
+//    consecutive ids and consecutive ranks, so their S rows are one
+//    contiguous range: nine ranges per query, found from one bitmap word,
+//    one prefix count and two run bounds each; sparser grids look every
+//    neighbour cell up in the hash (27 ranges);
+//  * queries run in cell order, so the 64 lanes of a wave visit the same
+//    neighbour cells and the same S rows;
+//  * one probe pass: each wave sums its lanes' candidate ranges (an upper
+//    bound of their neighbours), reserves that much of its workgroup's
+//    slice of a candidate buffer with one atomic, and every lane writes its
+//    neighbour ids into its own segment as it finds them, storing its count
+//    and segment start; an
+//    exclusive scan of the counts gives each row's final range, and the
+//    per-row sort (rowsort.hip) reads each segment, orders it and emits the
+//    (i, j) rows -- or the neighbour table of GDS-Join (counts per point +
+//    ids grouped by point, GDS/main.cu:507-540) -- at the row's final range.
+//    Waves whose candidates overflow that buffer (8 ids per output slot + 2
+//    per query) count their neighbours first and reserve exactly that many
+//    in a region of out_cap ids, so every output that fits its capacity is
+//    written in one run;
 //  * the predicate is the sequential binary64 difference form of the CPU
 //    oracle (d2 <= T; GDS used sqrt(d2) <= eps, GDS/kernel.cu:368-371).
 // The cell side is sqrt(T) * (1 + 2^-10) (or larger when an axis would
-// span more than 2^38 cells or the cell ids would not fit a 64-bit key):
-// every pair with d2_seq <= T has all coordinate differences below one cell
-// side, so it lies in adjacent cells.
-#include <type_traits>
+// span more than 2^38 cells or the sort keys would not fit 63 bits): every
+// pair with d2_seq <= T has all coordinate differences below one cell side,
+// so it lies in adjacent cells.
+#include <algorithm>
 #include "tj_internal.h"
 
 namespace {
@@ -46,7 +53,14 @@ struct GridParams {
     uint64_t ncell[3];
     uint64_t mul[3];  // linear id = sum c_k * mul[k] (mul[0] = 1: x-adjacent cells are adjacent ids)
     uint64_t cells;   // product of ncell
-    int keybits, idxbits, d, use_bm;  // use_bm: cells fit the occupancy bitmap
+    int keybits, d, use_bm, pad;  // use_bm: cells fit the occupancy bitmap
+};
+
+// A point of the index: coordinates widened to binary64 (exact) and the
+// point's index in its input array.
+struct GPoint {
+    double x, y, z;
+    uint32_t idx, pad;
 };
 
 __device__ inline uint32_t f2ord(float f) {
@@ -154,13 +168,14 @@ __global__ void grid_params_kernel(const uint32_t* bounds, double T, int d, int 
         ext[k] = b > a ? b - a : 0.0;
     }
     // Widen the cell until (1) no axis has more than 2^38 cells and (2) the
-    // linear id of every (padded) cell fits beside the index bits.  (1) keeps
-    // cell_of exact enough: its binary64 value (x - lo) / side carries a
-    // relative error below 2^-51, i.e. below 2^-13 cells at 2^38 cells, far
-    // inside the 2^-10 slack of the side, so two points closer than sqrt(T)
-    // on an axis always land in the same or adjacent cells.  Without it one
-    // far outlier stretches an axis to ~2^55 cells, where rounding moves a
-    // point by several cells and the probe misses pairs.
+    // linear id of every (padded) cell fits beside the index bits of the
+    // sort keys.  (1) keeps cell_of
+    // exact enough: its binary64 value (x - lo) / side carries a relative
+    // error below 2^-51, i.e. below 2^-13 cells at 2^38 cells, far inside the
+    // 2^-10 slack of the side, so two points closer than sqrt(T) on an axis
+    // always land in the same or adjacent cells.  Without it one far outlier
+    // stretches an axis to ~2^55 cells, where rounding moves a point by
+    // several cells and the probe misses pairs.
     for (;;) {
         int bits = 0;
         bool wide = false;
@@ -191,10 +206,10 @@ __global__ void grid_params_kernel(const uint32_t* bounds, double T, int d, int 
     g.side = side;
     g.inv_side = 1.0 / side;
     g.keybits = bits;
-    g.idxbits = idxbits;
     g.d = d;
     g.cells = mul;
     g.use_bm = mul <= bm_bits ? 1 : 0;
+    g.pad = 0;
     *gp = g;
 }
 
@@ -206,13 +221,13 @@ __global__ void __launch_bounds__(256) clear_bitmap_kernel(const GridParams* __r
         bm[t] = 0;
 }
 
-__device__ inline void cell_of(const float* x, const GridParams& g, int64_t c[3]) {
+__device__ inline void cell_of(const double* x, const GridParams& g, int64_t c[3]) {
     // unrolled over the three axes (static indices keep c and x in registers)
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         c[k] = 0;
         if (k < g.d) {
-            int64_t v = (int64_t)floor(((double)x[k] - g.lo[k]) * g.inv_side);
+            int64_t v = (int64_t)floor((x[k] - g.lo[k]) * g.inv_side);
             if (v < 1) v = 1;
             if (v > (int64_t)g.ncell[k] - 2) v = (int64_t)g.ncell[k] - 2;
             c[k] = v;
@@ -220,29 +235,32 @@ __device__ inline void cell_of(const float* x, const GridParams& g, int64_t c[3]
     }
 }
 
+// Sort keys: cell id << idxbits | point index.  Three-dimensional points go
+// four at a time as three 16-byte loads and two 16-byte stores.
 __global__ void __launch_bounds__(256) keys_kernel(const float* __restrict__ x, int64_t n, const GridParams* __restrict__ gp,
-                                                   uint64_t* __restrict__ keys) {
+                                                   int idxbits, uint64_t* __restrict__ keys) {
     const GridParams g = *gp;
-    auto key_of = [&](const float* p, int64_t t) {
+    auto key_of = [&](double x0, double x1, double x2, int64_t t) {
+        const double q[3] = {x0, x1, x2};
         int64_t c[3];
-        cell_of(p, g, c);
+        cell_of(q, g, c);
         const uint64_t id = (uint64_t)c[0] * g.mul[0] + (uint64_t)c[1] * g.mul[1] + (uint64_t)c[2] * g.mul[2];
-        return (id << g.idxbits) | (uint64_t)t;
+        return (id << idxbits) | (uint64_t)t;
     };
     const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
-    // three-dimensional points four at a time: three 16-byte loads, two
-    // 16-byte stores
     const int64_t n4 = (g.d == 3 && ((uintptr_t)x & 15) == 0 && ((uintptr_t)keys & 15) == 0) ? n / 4 : 0;
     const float4* xv = (const float4*)x;
     for (int64_t q = tid; q < n4; q += stride) {
         const float4 a = xv[3 * q], b = xv[3 * q + 1], c = xv[3 * q + 2];
-        const float p0[3] = {a.x, a.y, a.z}, p1[3] = {a.w, b.x, b.y}, p2[3] = {b.z, b.w, c.x}, p3[3] = {c.y, c.z, c.w};
         const int64_t t = 4 * q;
         ulonglong2* kv = (ulonglong2*)(keys + t);
-        kv[0] = make_ulonglong2(key_of(p0, t), key_of(p1, t + 1));
-        kv[1] = make_ulonglong2(key_of(p2, t + 2), key_of(p3, t + 3));
+        kv[0] = make_ulonglong2(key_of(a.x, a.y, a.z, t), key_of(a.w, b.x, b.y, t + 1));
+        kv[1] = make_ulonglong2(key_of(b.z, b.w, c.x, t + 2), key_of(c.y, c.z, c.w, t + 3));
     }
-    for (int64_t t = 4 * n4 + tid; t < n; t += stride) keys[t] = key_of(x + t * g.d, t);
+    for (int64_t t = 4 * n4 + tid; t < n; t += stride) {
+        const float* p = x + t * g.d;
+        keys[t] = key_of(p[0], g.d > 1 ? p[1] : 0.0, g.d > 2 ? p[2] : 0.0, t);
+    }
 }
 
 __device__ inline uint64_t hmix(uint64_t k) {
@@ -250,64 +268,60 @@ __device__ inline uint64_t hmix(uint64_t k) {
     return k;
 }
 constexpr uint64_t EMPTY = ~0ull;
+constexpr int GRID_SHARDS = 64;  // slices of the probe's candidate buffer
 
-// R in cell order: 16-byte rows (x, y, z, original index as bits), so the
-// probes read their queries and counts contiguously.
+// The points in sorted-key (cell) order as index rows: coordinates widened
+// to binary64 and the original index.  For S (hash != NULL) also the
+// occupied cells: a bitmap bit per run start, or, on sparse grids, the
+// run's [start, end) inserted into the hash.
 __global__ void __launch_bounds__(256) rows_kernel(const float* __restrict__ x, int64_t n, const GridParams* __restrict__ gp,
-                                                   const uint64_t* __restrict__ keys, float4* __restrict__ rows) {
+                                                   int idxbits, const uint64_t* __restrict__ keys, GPoint* __restrict__ rows,
+                                                   uint64_t* __restrict__ hkey, uint2* __restrict__ hval, uint64_t hmask,
+                                                   uint64_t* __restrict__ bm) {
     const GridParams g = *gp;
-    const uint64_t imask = (1ull << g.idxbits) - 1;
-    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
-        const uint64_t j = keys[p] & imask;
-        float4 r = make_float4(0.f, 0.f, 0.f, __builtin_bit_cast(float, (uint32_t)j));
-        const float* y = x + j * g.d;
-        r.x = y[0];
-        if (g.d > 1) r.y = y[1];
-        if (g.d > 2) r.z = y[2];
-        rows[p] = r;
-    }
-}
-
-// S in cell order: 16-byte rows (x, y, z, original index as bits); run starts
-// of each non-empty cell inserted into the hash (slot: key, start, end).
-__global__ void __launch_bounds__(256) gather_kernel(const float* __restrict__ S, int64_t n, const GridParams* __restrict__ gp,
-                                                     const uint64_t* __restrict__ keys, float4* __restrict__ srow,
-                                                     uint64_t* __restrict__ hkey, uint2* __restrict__ hval, uint64_t hmask,
-                                                     uint64_t* __restrict__ bm) {
-    const GridParams g = *gp;
-    const uint64_t imask = (1ull << g.idxbits) - 1;
+    const uint64_t imask = (1ull << idxbits) - 1;
     for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
         const uint64_t kk = keys[p];
-        const uint64_t j = kk & imask, cell = kk >> g.idxbits;
-        float4 r = make_float4(0.f, 0.f, 0.f, __builtin_bit_cast(float, (uint32_t)j));
-        const float* y = S + j * g.d;
+        const uint64_t j = kk & imask, cell = kk >> idxbits;
+        const float* y = x + j * g.d;
+        GPoint r;
         r.x = y[0];
-        if (g.d > 1) r.y = y[1];
-        if (g.d > 2) r.z = y[2];
-        srow[p] = r;
-        if (p == 0 || (keys[p - 1] >> g.idxbits) != cell) {
-            if (g.use_bm) {
-                atomicOr((unsigned long long*)&bm[cell >> 6], 1ull << (cell & 63));
-                continue;
-            }
-            // end of the run: galloping search over the sorted keys
-            int64_t lo = p, step = 1;  // keys[lo] is in the cell
-            while (lo + step < n && (keys[lo + step] >> g.idxbits) == cell) { lo += step; step <<= 1; }
-            int64_t hi = lo + step < n ? lo + step : n;  // keys[hi] is past the cell (or hi = n)
-            while (hi - lo > 1) {
-                const int64_t mid = lo + (hi - lo) / 2;
-                if ((keys[mid] >> g.idxbits) == cell) lo = mid; else hi = mid;
-            }
-            const int64_t e = hi;
-            uint64_t h = hmix(cell) & hmask;
-            for (;;) {
-                const unsigned long long prev = atomicCAS((unsigned long long*)&hkey[h], (unsigned long long)EMPTY,
-                                                          (unsigned long long)cell);
-                if (prev == EMPTY || prev == cell) break;
-                h = (h + 1) & hmask;
-            }
-            hval[h] = make_uint2((uint32_t)p, (uint32_t)e);
+        r.y = g.d > 1 ? (double)y[1] : 0.0;
+        r.z = g.d > 2 ? (double)y[2] : 0.0;
+        r.idx = (uint32_t)j;
+        r.pad = 0;
+        rows[p] = r;
+        if (!hkey || (p > 0 && (keys[p - 1] >> idxbits) == cell)) continue;
+        if (g.use_bm) {
+            atomicOr((unsigned long long*)&bm[cell >> 6], 1ull << (cell & 63));
+            continue;
         }
+        // end of the run: galloping search over the sorted keys
+        int64_t lo = p, step = 1;  // keys[lo] is in the cell
+        while (lo + step < n && (keys[lo + step] >> idxbits) == cell) { lo += step; step <<= 1; }
+        int64_t hi = lo + step < n ? lo + step : n;  // keys[hi] is past the cell (or hi = n)
+        while (hi - lo > 1) {
+            const int64_t mid = lo + (hi - lo) / 2;
+            if ((keys[mid] >> idxbits) == cell) lo = mid; else hi = mid;
+        }
+        uint64_t h = hmix(cell) & hmask;
+        for (;;) {
+            const unsigned long long prev = atomicCAS((unsigned long long*)&hkey[h], (unsigned long long)EMPTY,
+                                                      (unsigned long long)cell);
+            if (prev == EMPTY || prev == cell) break;
+            h = (h + 1) & hmask;
+        }
+        hval[h] = make_uint2((uint32_t)p, (uint32_t)hi);
+    }
+}
+__device__ inline uint2 lookup(const uint64_t* __restrict__ hkey, const uint2* __restrict__ hval, uint64_t hmask,
+                               uint64_t cell) {
+    uint64_t h = hmix(cell) & hmask;
+    for (;;) {
+        const uint64_t k = hkey[h];
+        if (k == cell) return hval[h];
+        if (k == EMPTY) return make_uint2(0, 0);
+        h = (h + 1) & hmask;
     }
 }
 
@@ -385,292 +399,337 @@ __device__ inline uint32_t cell_rank(const uint64_t* __restrict__ bm, const uint
 }
 // runs[rank(cell)] = first sorted S position of each non-empty cell; the
 // entry after the last run holds n.
-__global__ void __launch_bounds__(256) runs_kernel(int64_t n, const GridParams* __restrict__ gp,
+__global__ void __launch_bounds__(256) runs_kernel(int64_t n, const GridParams* __restrict__ gp, int idxbits,
                                                    const uint64_t* __restrict__ keys, const uint64_t* __restrict__ bm,
                                                    const uint32_t* __restrict__ bmpre, uint32_t* __restrict__ runs) {
     const GridParams g = *gp;
     if (!g.use_bm) return;
     for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
-        const uint64_t cell = keys[p] >> g.idxbits;
-        if (p == 0 || (keys[p - 1] >> g.idxbits) != cell) runs[cell_rank(bm, bmpre, cell)] = (uint32_t)p;
+        const uint64_t cell = keys[p] >> idxbits;
+        if (p == 0 || (keys[p - 1] >> idxbits) != cell) runs[cell_rank(bm, bmpre, cell)] = (uint32_t)p;
         if (p == n - 1) runs[cell_rank(bm, bmpre, cell) + 1] = (uint32_t)n;
-    }
-}
-
-__device__ inline uint2 lookup(const uint64_t* __restrict__ hkey, const uint2* __restrict__ hval, uint64_t hmask, uint64_t cell) {
-    uint64_t h = hmix(cell) & hmask;
-    for (;;) {
-        const uint64_t k = hkey[h];
-        if (k == cell) return hval[h];
-        if (k == EMPTY) return make_uint2(0, 0);
-        h = (h + 1) & hmask;
     }
 }
 
 // The sequential binary64 difference form over D <= 3 coordinates.
 template <int D>
-__device__ inline double d2_row(const float* q, const float4& y) {
+__device__ inline double d2_point(const double* q, const GPoint& y) {
     double s = 0.0, t;
-    t = (double)q[0] - (double)y.x; s = s + t * t;
-    if (D > 1) { t = (double)q[1] - (double)y.y; s = s + t * t; }
-    if (D > 2) { t = (double)q[2] - (double)y.z; s = s + t * t; }
+    t = q[0] - y.x; s = s + t * t;
+    if (D > 1) { t = q[1] - y.y; s = s + t * t; }
+    if (D > 2) { t = q[2] - y.z; s = s + t * t; }
     return s;
 }
+
+// A lane's 64-bit value broadcast to the wave (lane k wave-uniform).
+__device__ inline uint64_t rl_u64(uint64_t v, int k) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, k);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), k);
+    return ((uint64_t)hi << 32) | lo;
+}
+__device__ inline double rl_f64(double v, int k) { return __builtin_bit_cast(double, rl_u64(__builtin_bit_cast(uint64_t, v), k)); }
 
 __device__ inline uint64_t shfl_u64(uint64_t v, int src) {
     const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)v, src);
     const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(v >> 32), src);
     return ((uint64_t)hi << 32) | lo;
 }
+__device__ inline uint64_t shfl_up_u64(uint64_t v, int o) {
+    const uint32_t lo = (uint32_t)__shfl_up((int)(uint32_t)v, o);
+    const uint32_t hi = (uint32_t)__shfl_up((int)(uint32_t)(v >> 32), o);
+    return ((uint64_t)hi << 32) | lo;
+}
 
-// One query per lane, queries in cell order (rrow: x, y, z, original index).
-// The same traversal runs twice: the count pass stores each query's number
-// of neighbours at cnt[i] (and counts candidate evaluations); after an
-// exclusive scan of the counts, the write pass stores each neighbour id j
-// at outj[off[i] + k], k the query's running count, so every row's ids land
-// in their final range without any global atomic.  A row's ids arrive in
-// cell order, not sorted: the row sort kernels order each row afterwards.
-// Positions at or past out_cap are skipped (the caller retries larger).
-enum { PROBE_COUNT = 0, PROBE_WRITE = 1 };
-
-template <int MODE, int D>
-__global__ void __launch_bounds__(256) probe_kernel(const float4* __restrict__ rrow, int64_t nR, const GridParams* __restrict__ gp,
-                                                    const float4* __restrict__ srow,
-                                                    const uint64_t* __restrict__ hkey, const uint2* __restrict__ hval,
-                                                    uint64_t hmask, const uint64_t* __restrict__ bm,
-                                                    const uint32_t* __restrict__ bmpre, const uint32_t* __restrict__ runs,
-                                                    double T, int upper, uint32_t* __restrict__ cnt,
-                                                    const uint64_t* __restrict__ off, uint32_t* __restrict__ outj,
-                                                    int64_t out_cap, uint64_t* __restrict__ stats) {
+// One pass over the queries, in cell order, 64 queries per workgroup step
+// (query l on lane l of every wave) and PW waves sharing them: wave w finds
+// the candidate ranges of neighbour rows w, w + PW, ... and the workgroup's
+// waves split the sweeps (the short ranges of a row by the row's wave, the
+// chunks of the long ranges round robin), so a step over a dense blob, whose
+// 64 queries hold a few thousand candidates each, runs on PW waves at once.
+// Each query's neighbour count so far lives in LDS (sMine, LDS atomics).
+// After the ranges, wave 0 reserves the step's candidate total in its
+// workgroup's slice of the candidate buffer outj (one atomic on
+// cursor[slice]; same-address atomics serialise) and each query's hits are
+// written into its own segment as they are found (ascending within each run
+// of S rows, unordered across runs); cnt[i] and src[i] receive the query's
+// neighbour count and segment start.  The buffer's last out_cap ids (from
+// fast_cap on) take the steps whose candidates no longer fit: they count
+// first and reserve their exact neighbour total there (cursor[GRID_SHARDS]),
+// so any output that fits its capacity is written whatever the candidate
+// total; stats[TJ_ST_GRID_CAND] receives that total.
+constexpr int PW = 4;  // waves per probe workgroup
+template <int D, bool UPPER>
+__global__ void __launch_bounds__(64 * PW) probe_kernel(const GPoint* __restrict__ rrow, int64_t nR,
+                                                       const GridParams* __restrict__ gp, const GPoint* __restrict__ srow,
+                                                       const uint64_t* __restrict__ hkey, const uint2* __restrict__ hval,
+                                                       uint64_t hmask, const uint64_t* __restrict__ bm,
+                                                       const uint32_t* __restrict__ bmpre, const uint32_t* __restrict__ runs,
+                                                       double T, uint32_t* __restrict__ cnt, uint64_t* __restrict__ src,
+                                                       uint32_t* __restrict__ outj, int64_t fast_cap, int64_t cand_cap,
+                                                       uint64_t* __restrict__ cursor, uint64_t* __restrict__ stats) {
+    constexpr int NROWS = D == 3 ? 9 : D == 2 ? 3 : 1;  // neighbour rows (dy, dz)
     const GridParams g = *gp;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    // neighbours found for each lane's query by other lanes (wave-shared sweep)
-    __shared__ uint32_t sCnt[4][64];
-    // each lane's S range per neighbour row
-    __shared__ uint2 sRun[4][9][64];
-    sCnt[wave][lane] = 0;
-    uint64_t cand = 0;
-    // whole waves step together (tb is the wave's first query), so the
-    // shared sweep always has all 64 lanes
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t tb = (int64_t)blockIdx.x * blockDim.x + wave * 64; tb < nR; tb += stride) {
+    __shared__ uint2 sRun[NROWS][64];  // S range of each query per neighbour row (bitmap grids)
+    __shared__ uint32_t sCand[PW][64];  // candidates per query found by each wave
+    __shared__ uint32_t sMine[64];      // neighbours per query so far
+    __shared__ uint64_t sSeg[64];       // segment start per query
+    __shared__ int sPhase;
+    // this workgroup's slice of the fast region (one cursor per slice)
+    const int shard = (int)(blockIdx.x % GRID_SHARDS);
+    const int64_t slice = fast_cap / GRID_SHARDS;
+    uint64_t wcand = 0;  // candidates of this workgroup (one atomic at the end)
+    const int64_t stride = (int64_t)gridDim.x * 64;
+    for (int64_t tb = (int64_t)blockIdx.x * 64; tb < nR; tb += stride) {
         const int64_t t = tb + lane;
         const bool valid = t < nR;
-        const float4 qr = valid ? rrow[t] : make_float4(0.f, 0.f, 0.f, 0.f);
-        const uint32_t i = __builtin_bit_cast(uint32_t, qr.w);
-        const float q[3] = {qr.x, qr.y, qr.z};
-        const uint64_t base = (MODE == PROBE_WRITE && valid) ? off[i] : 0;
-        uint32_t mine = 0;  // neighbours of this lane's query so far
+        GPoint qp;
+        if (valid) qp = rrow[t];
+        else qp = GPoint{0.0, 0.0, 0.0, 0u, 0u};
+        const uint32_t i = qp.idx;
+        const double q[3] = {qp.x, qp.y, qp.z};
         int64_t c[3];
         cell_of(q, g, c);
-        auto visit = [&](const float4& y) {
-            const uint32_t j = __builtin_bit_cast(uint32_t, y.w);
-            if (upper && j <= i) return;
-            if (MODE == PROBE_COUNT) ++cand;
-            if (d2_row<D>(q, y) <= T) {
-                if (MODE == PROBE_WRITE) {
-                    const uint64_t at = base + mine;
-                    if (at < (uint64_t)out_cap) outj[at] = j;
-                }
-                ++mine;
-            }
-        };
-        auto sweep = [&](uint2 run) {
-            // eight rows in flight at a time
-            uint32_t p = run.x;
-            for (; p + 8 <= run.y; p += 8) {
-                float4 y[8];
-#pragma unroll
-                for (int u = 0; u < 8; ++u) y[u] = srow[p + u];
-#pragma unroll
-                for (int u = 0; u < 8; ++u) visit(y[u]);
-            }
-            for (; p < run.y; ++p) visit(srow[p]);
-        };
-        const int r1 = g.d > 1 ? 1 : 0, r2 = g.d > 2 ? 1 : 0;
-        // ids of the x-adjacent cells of neighbour row (dy, dz): mid - 1,
-        // mid, mid + 1 (cell_of keeps c in [1, ncell - 2], so all exist)
-        auto row_mid = [&](int dy, int dz) {
+        // neighbour row rr -> its middle cell (the x-adjacent cells mid - 1,
+        // mid, mid + 1; cell_of keeps c in [1, ncell - 2], so all exist)
+        auto row_mid = [&](int rr) {
+            const int dz = D == 3 ? rr / 3 - 1 : 0, dy = D >= 2 ? rr % 3 - 1 : 0;
             return (uint64_t)c[0] + (uint64_t)(c[1] + dy) * g.mul[1] + (uint64_t)(c[2] + dz) * g.mul[2];
         };
-        if (!g.use_bm) {
-            // sparse grid: one hash run per cell
-            if (valid)
-                for (int dz = -r2; dz <= r2; ++dz)
-                    for (int dy = -r1; dy <= r1; ++dy)
-                        for (int dx = -1; dx <= 1; ++dx)
-                            sweep(lookup(hkey, hval, hmask, row_mid(dy, dz) + (uint64_t)(int64_t)dx));
-            if (MODE == PROBE_COUNT && valid) cnt[i] = mine;
-            continue;
-        }
-        // The S rows of each neighbour row's three cells are one range.  The
-        // ranges of all (up to nine) rows are found first, with their loads
-        // in two independent rounds (bitmap word + its prefix count, then the
-        // run bounds) instead of nine dependent chains, and kept in LDS.
-        {
-            uint64_t w0[9];
-            uint32_t pre[9];
-#pragma unroll
-            for (int r = 0; r < 9; ++r) {
-                const int dz = r / 3 - 1, dy = r % 3 - 1;
-                const bool use = valid && (dz == 0 || r2) && (dy == 0 || r1);
-                const uint64_t b0 = row_mid(use ? dy : 0, use ? dz : 0) - 1;
-                w0[r] = use ? bm[b0 >> 6] : 0ull;
-                pre[r] = use ? bmpre[b0 >> 6] : 0u;
-            }
-#pragma unroll
-            for (int r = 0; r < 9; ++r) {
-                const int dz = r / 3 - 1, dy = r % 3 - 1;
-                const bool use = valid && (dz == 0 || r2) && (dy == 0 || r1);
-                const uint64_t b0 = row_mid(use ? dy : 0, use ? dz : 0) - 1, b2 = b0 + 2;
-                const uint32_t sh = (uint32_t)(b0 & 63);
-                uint32_t occ = use ? (uint32_t)((w0[r] >> sh) & 7) : 0u;
-                if (use && sh > 61 && (b2 >> 6) != (b0 >> 6)) occ |= (uint32_t)(bm[b2 >> 6] << (64 - sh)) & 7;
+        // ---- candidate ranges of this wave's rows and their total
+        uint32_t mycand = 0;
+        for (int rr = wave; rr < NROWS; rr += PW) {
+            if (g.use_bm) {
                 uint2 run = make_uint2(0, 0);
-                if (occ) {
-                    const uint32_t r0 = pre[r] + (uint32_t)__popcll(sh ? (w0[r] & ((1ull << sh) - 1)) : 0ull);
-                    run = make_uint2(runs[r0], runs[r0 + (uint32_t)__popc(occ)]);
+                if (valid) {
+                    const uint64_t b0 = row_mid(rr) - 1, b2 = b0 + 2;
+                    const uint64_t w0 = bm[b0 >> 6];
+                    const uint32_t pre = bmpre[b0 >> 6];
+                    const uint32_t sh = (uint32_t)(b0 & 63);
+                    uint32_t occ = (uint32_t)((w0 >> sh) & 7);
+                    if (sh > 61 && (b2 >> 6) != (b0 >> 6)) occ |= (uint32_t)(bm[b2 >> 6] << (64 - sh)) & 7;
+                    if (occ) {
+                        const uint32_t rk = pre + (uint32_t)__popcll(sh ? (w0 & ((1ull << sh) - 1)) : 0ull);
+                        run = make_uint2(runs[rk], runs[rk + (uint32_t)__popc(occ)]);
+                    }
                 }
-                sRun[wave][r][lane] = run;
+                sRun[rr][lane] = run;
+                mycand += run.y - run.x;
+            } else if (valid) {
+                for (int dx = -1; dx <= 1; ++dx) {
+                    const uint2 run = lookup(hkey, hval, hmask, row_mid(rr) + (uint64_t)(int64_t)dx);
+                    mycand += run.y - run.x;
+                }
             }
         }
-        for (int r = 0; r < 9; ++r) {
-            const int dz = r / 3 - 1, dy = r % 3 - 1;
-            if ((dz != 0 && !r2) || (dy != 0 && !r1)) continue;
-                const uint2 run = sRun[wave][r][lane];
+        sCand[wave][lane] = mycand;
+        __syncthreads();
+        // ---- one reservation per step: the candidate total in the fast
+        // region; phase 2 sweeps and writes, phase 0 only counts
+        if (wave == 0) {
+            uint32_t cand = 0;
+#pragma unroll
+            for (int w = 0; w < PW; ++w) cand += sCand[w][lane];
+            uint64_t inc = cand;
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint64_t v = shfl_up_u64(inc, o);
+                if (lane >= o) inc += v;
+            }
+            const uint64_t W = shfl_u64(inc, 63);
+            uint64_t base = 0;
+            if (lane == 0 && W) base = atomicAdd((unsigned long long*)&cursor[shard], (unsigned long long)W);
+            base = shfl_u64(base, 0);
+            wcand += W;
+            sSeg[lane] = (uint64_t)shard * (uint64_t)slice + base + (inc - cand);
+            sMine[lane] = 0;
+            if (lane == 0) sPhase = base + W <= (uint64_t)slice ? 2 : 0;
+        }
+        __syncthreads();
+        int phase = sPhase;
+        for (;;) {
+            // a hit of query k (own lane or broadcast): its position in the
+            // segment from the LDS count
+            auto sweep_own = [&](uint2 run) {
+                const uint64_t seg = sSeg[lane];
+                for (uint32_t p = run.x; p < run.y; ++p) {
+                    const GPoint y = srow[p];
+                    if (UPPER && y.idx <= i) continue;
+                    if (d2_point<D>(q, y) <= T) {
+                        const uint32_t at = atomicAdd(&sMine[lane], 1u);
+                        if (phase) outj[seg + at] = y.idx;
+                    }
+                }
+            };
+            for (int rr = 0; rr < NROWS; ++rr) {
+                const int own = rr % PW;  // the wave of this row's short ranges
+                if (!g.use_bm) {
+                    // sparse grid: each lane sweeps its own three cells
+                    if (wave == own && valid)
+                        for (int dx = -1; dx <= 1; ++dx)
+                            sweep_own(lookup(hkey, hval, hmask, row_mid(rr) + (uint64_t)(int64_t)dx));
+                    continue;
+                }
+                const uint2 run = sRun[rr][lane];
                 const uint32_t len = run.y - run.x;
-                if (__ballot(len > 32) == 0) {
-                    sweep(run);
-                    continue;
-                }
-                // Long ranges (a dense blob).  Lanes of a wave hold queries of
-                // the same few cells, so their ranges mostly overlap: the wave
-                // walks the union [u0, u1) of its ranges row by row, every lane
-                // reading the same row (one broadcast load) and evaluating it
-                // only if it lies in its own range -- each pair is still met
-                // once, and in ascending row order per lane.
-                uint32_t u0 = len ? run.x : 0xFFFFFFFFu, u1 = len ? run.y : 0u, mx = len;
-                for (int o = 32; o > 0; o >>= 1) {
-                    u0 = min(u0, (uint32_t)__shfl_xor((int)u0, o));
-                    u1 = max(u1, (uint32_t)__shfl_xor((int)u1, o));
-                    mx = max(mx, (uint32_t)__shfl_xor((int)mx, o));
-                }
-                if (u1 - u0 <= 2 * mx + 64) {
-                    const uint32_t a = __builtin_amdgcn_readfirstlane(u0), b = __builtin_amdgcn_readfirstlane(u1);
-                    uint32_t pr = a;
-                    for (; pr + 8 <= b; pr += 8) {
-                        float4 y[8];
-#pragma unroll
-                        for (int u = 0; u < 8; ++u) y[u] = srow[pr + u];
-#pragma unroll
-                        for (int u = 0; u < 8; ++u)
-                            if (pr + u >= run.x && pr + u < run.y) visit(y[u]);
-                    }
-                    for (; pr < b; ++pr) {
-                        const float4 y = srow[pr];
-                        if (pr >= run.x && pr < run.y) visit(y);
-                    }
-                    continue;
-                }
-                // A lane near a dense blob holds hundreds of rows while the
-                // rest hold a few: the wave sweeps the concatenation of its
-                // lanes' ranges together, 64 rows a step (row k belongs to
-                // the lane whose range prefix holds k); hits are counted per
-                // owner in LDS and the owner's next free position is its
-                // own count so far plus that LDS count.
-                uint32_t inc = len;
-                for (int o = 1; o < 64; o <<= 1) {
-                    const uint32_t v = __shfl_up(inc, o);
-                    if (lane >= o) inc += v;
-                }
-                const uint32_t excl = inc - len;
-                const uint32_t total = __shfl(inc, 63);
-                for (uint32_t k0 = 0; k0 < total; k0 += 64) {
-                    const uint32_t k = k0 + (uint32_t)lane;
-                    int o = 0;  // the largest lane whose prefix is <= k: the owner of row k
-                    for (int step = 32; step > 0; step >>= 1)
-                        if (__shfl(excl, o + step) <= k) o += step;
-                    const uint32_t p = __shfl(run.x, o) + (k - __shfl(excl, o));
-                    const float qx = __shfl(qr.x, o), qy = __shfl(qr.y, o), qz = __shfl(qr.z, o);
-                    const uint32_t io = __builtin_bit_cast(uint32_t, __shfl(qr.w, o));
-                    const uint64_t obase = MODE == PROBE_WRITE ? shfl_u64(base, o) + (uint64_t)__shfl(mine, o) : 0;
-                    if (k < total) {
-                        const float4 y = srow[p];
-                        const uint32_t j = __builtin_bit_cast(uint32_t, y.w);
-                        if (!(upper && j <= io)) {
-                            if (MODE == PROBE_COUNT) ++cand;
-                            const float qo[3] = {qx, qy, qz};
-                            if (d2_row<D>(qo, y) <= T) {
-                                const uint32_t slot = atomicAdd(&sCnt[wave][o], 1u);
-                                if (MODE == PROBE_WRITE) {
-                                    const uint64_t at = obase + slot;
-                                    if (at < (uint64_t)out_cap) outj[at] = j;
+                const bool longr = len > 32;
+                if (wave == own && !longr && len) sweep_own(run);
+                if (__ballot(longr) == 0) continue;
+                // Long ranges (a dense blob).  The queries of a step sit in the
+                // same few cells, so their ranges overlap: the long ranges are
+                // walked in chunks of 128 rows, two rows per lane (coalesced
+                // loads), skipping rows no long range holds; chunk t of row rr
+                // belongs to wave (t + rr) % PW.  The queries whose range meets
+                // the chunk are taken one at a time, broadcast from their lane
+                // (readlane), and every lane tests its two rows against the
+                // query; the query's hits in the chunk go to consecutive
+                // positions of its segment (ballot + mbcnt + one LDS atomic):
+                // coalesced stores.  A query whose range covers the whole chunk
+                // needs no range test.
+                uint32_t ch = longr ? run.x : 0xFFFFFFFFu;
+                for (int o = 32; o > 0; o >>= 1) ch = min(ch, (uint32_t)__shfl_xor((int)ch, o));
+                ch = __builtin_amdgcn_readfirstlane(ch);
+                for (int tch = rr; ch != 0xFFFFFFFFu; ++tch) {
+                    uint32_t cend = longr && run.x < ch + 128 && run.y > ch ? min(run.y, ch + 128) : 0u;
+                    for (int o = 32; o > 0; o >>= 1) cend = max(cend, (uint32_t)__shfl_xor((int)cend, o));
+                    cend = __builtin_amdgcn_readfirstlane(cend);
+                    if (tch % PW == wave) {
+                        const uint32_t p0 = ch + (uint32_t)lane, p1 = p0 + 64;
+                        const bool h0 = p0 < cend, h1 = p1 < cend;
+                        GPoint y0, y1;
+                        if (h0) y0 = srow[p0];
+                        else y0 = GPoint{0.0, 0.0, 0.0, 0u, 0u};
+                        if (h1) y1 = srow[p1];
+                        else y1 = GPoint{0.0, 0.0, 0.0, 0u, 0u};
+                        // wait for the rows here: the query loop below then has no
+                        // load in flight, and its stores need no wait
+                        __builtin_amdgcn_s_waitcnt(0);
+                        uint64_t qm = __ballot(longr && run.x < cend && run.y > ch);
+                        while (qm) {
+                            const int k = __builtin_ctzll(qm);
+                            qm &= qm - 1;
+                            const double qk[3] = {rl_f64(q[0], k), rl_f64(q[1], k), rl_f64(q[2], k)};
+                            const uint32_t rx = (uint32_t)__builtin_amdgcn_readlane((int)run.x, k);
+                            const uint32_t ry = (uint32_t)__builtin_amdgcn_readlane((int)run.y, k);
+                            const uint32_t ik = UPPER ? (uint32_t)__builtin_amdgcn_readlane((int)i, k) : 0u;
+                            bool t0, t1;
+                            if (rx <= ch && ry >= cend) {
+                                t0 = h0 && !(UPPER && y0.idx <= ik) && d2_point<D>(qk, y0) <= T;
+                                t1 = h1 && !(UPPER && y1.idx <= ik) && d2_point<D>(qk, y1) <= T;
+                            } else {
+                                t0 = p0 >= rx && p0 < ry && !(UPPER && y0.idx <= ik) && d2_point<D>(qk, y0) <= T;
+                                t1 = p1 >= rx && p1 < ry && !(UPPER && y1.idx <= ik) && d2_point<D>(qk, y1) <= T;
+                            }
+                            const uint64_t m0 = __ballot(t0), m1 = __ballot(t1);
+                            if (m0 | m1) {
+                                const uint32_t n0 = (uint32_t)__popcll(m0), n = n0 + (uint32_t)__popcll(m1);
+                                uint32_t at = 0;
+                                if (lane == 0) at = atomicAdd(&sMine[k], n);
+                                at = (uint32_t)__builtin_amdgcn_readfirstlane((int)at);
+                                if (phase) {
+                                    const uint64_t sk = sSeg[k] + at;
+                                    const uint32_t r0 = __builtin_amdgcn_mbcnt_hi(
+                                        (uint32_t)(m0 >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m0, 0u));
+                                    const uint32_t r1 = __builtin_amdgcn_mbcnt_hi(
+                                        (uint32_t)(m1 >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m1, 0u));
+                                    if (t0) outj[sk + r0] = y0.idx;
+                                    if (t1) outj[sk + n0 + r1] = y1.idx;
                                 }
                             }
                         }
                     }
+                    // the next chunk starts at the first row past this one that a
+                    // long range holds
+                    uint32_t nx = longr && run.y > ch + 128 ? max(run.x, ch + 128) : 0xFFFFFFFFu;
+                    for (int o = 32; o > 0; o >>= 1) nx = min(nx, (uint32_t)__shfl_xor((int)nx, o));
+                    ch = __builtin_amdgcn_readfirstlane(nx);
                 }
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                mine += sCnt[wave][lane];
-                sCnt[wave][lane] = 0;
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             }
-        if (MODE == PROBE_COUNT && valid) cnt[i] = mine;
+            __syncthreads();
+            if (phase) break;
+            // exact reservation of the counted neighbours
+            if (wave == 0) {
+                const uint32_t h = sMine[lane];
+                uint64_t hinc = h;
+                for (int o = 1; o < 64; o <<= 1) {
+                    const uint64_t v = shfl_up_u64(hinc, o);
+                    if (lane >= o) hinc += v;
+                }
+                const uint64_t H = shfl_u64(hinc, 63);
+                uint64_t b2 = 0;
+                if (lane == 0 && H) b2 = atomicAdd((unsigned long long*)&cursor[GRID_SHARDS], (unsigned long long)H);
+                b2 = shfl_u64(b2, 0);
+                const bool fits = b2 + H <= (uint64_t)(cand_cap - fast_cap);
+                // when the output passes its capacity the counts stand and
+                // nothing is written (segment 0 keeps the row sort's reads in
+                // bounds)
+                sSeg[lane] = fits ? (uint64_t)fast_cap + b2 + (hinc - h) : 0;
+                sMine[lane] = fits ? 0 : h;
+                if (lane == 0) sPhase = fits ? 2 : 1;
+            }
+            __syncthreads();
+            phase = sPhase;
+            if (phase == 1) break;
+        }
+        if (wave == 0 && valid) {
+            cnt[i] = sMine[lane];
+            src[i] = sSeg[lane];
+        }
+        __syncthreads();
     }
-    if (MODE == PROBE_COUNT) {
-        for (int s = 32; s > 0; s >>= 1) cand += __shfl_xor(cand, s);
-        if (lane == 0 && cand) atomicAdd((unsigned long long*)&stats[TJ_ST_GRID_CAND], (unsigned long long)cand);
-    }
+    if (threadIdx.x == 0 && wcand) atomicAdd((unsigned long long*)&stats[TJ_ST_GRID_CAND], (unsigned long long)wcand);
 }
 
 }  // namespace
 
 namespace tj {
 
-
 // Blocks of one bounds pass (a partial per block, no atomics): four per CU.
 static int64_t bounds_blocks() { return 4 * (int64_t)num_cus(); }
+
+// The candidate buffer: a fast region of 8 ids per output slot + 2 per
+// query, then out_cap ids for the waves that reserve exactly.
+int64_t grid_cand_capacity(int64_t nR, int64_t out_cap) { return 9 * out_cap + 2 * nR + 1024; }
 
 GridLayout grid_layout(int64_t nR, int64_t nS, int64_t out_cap) {
     GridLayout L{};
     size_t cur = 0;
     auto carve = [&](size_t bytes) { size_t at = (cur + 255) / 256 * 256; cur = at + bytes; return at; };
-    const int64_t nmax = nR > nS ? nR : nS;
+    const int64_t nmax = std::max<int64_t>(std::max<int64_t>(nR, nS), 1);
     uint64_t hs = 1;
-    while (hs < (uint64_t)(2 * (nS > 0 ? nS : 1))) hs <<= 1;
+    while (hs < (uint64_t)(2 * std::max<int64_t>(nS, 1))) hs <<= 1;
     L.hslots = hs;
     L.idxbits = bits_for(nmax > 1 ? nmax : 2);
+    // occupancy bitmap of S: 64 bits per S point (at least 2^20); sparser
+    // grids go without it
+    L.bm_bits = std::max<uint64_t>(1ull << 20, 64ull * (uint64_t)std::max<int64_t>(nS, 1));
     L.bounds = carve(64);
     L.bpart = carve(4 * 6 * 2 * (size_t)bounds_blocks());
     L.params = carve(sizeof(GridParams));
     L.count_nr = carve(8);
     L.count_ns = carve(8);
-    L.rkeys = carve(8 * (size_t)(nR > 0 ? nR : 1));
-    L.skeys = carve(8 * (size_t)(nS > 0 ? nS : 1));
-    L.sort_tmp = carve(sort_temp_bytes(std::max<int64_t>(nmax, 1)));
-    L.srow = carve(16 * (size_t)(nS > 0 ? nS : 1));
-    L.rrow = carve(16 * (size_t)(nR > 0 ? nR : 1));
+    L.rkeys = carve(8 * (size_t)std::max<int64_t>(nR, 1));
+    L.skeys = carve(8 * (size_t)std::max<int64_t>(nS, 1));
+    L.sort_tmp = carve(sort_temp_bytes(nmax));
+    L.srow = carve(sizeof(GPoint) * (size_t)std::max<int64_t>(nS, 1));
+    L.rrow = carve(sizeof(GPoint) * (size_t)std::max<int64_t>(nR, 1));
     L.hkey = carve(8 * hs);
     L.hval = carve(8 * hs);
-    // occupancy bitmap: 64 bits per S point (at least 2^20); sparser grids
-    // go without it
-    L.bm_bits = std::max<uint64_t>(1ull << 20, 64ull * (uint64_t)(nS > 0 ? nS : 1));
     L.bitmap = carve(L.bm_bits / 8);
     L.bmpre = carve(4 * (L.bm_bits / 64));
     L.bmsum = carve(4 * (L.bm_bits / 64 / BMB + 1));
-    L.runs = carve(4 * (size_t)((nS > 0 ? nS : 1) + 1));
-    // output: per-row counts and offsets, the ids in row order (and a
-    // ping-pong copy for the long-row merge), the list of long rows
-    L.cnt = carve(4 * (size_t)(nR > 0 ? nR : 1));
+    L.runs = carve(4 * (size_t)(std::max<int64_t>(nS, 1) + 1));
+    // output: the slice cursors, per-query counts, segment starts and final
+    // offsets, the candidate buffer, the long rows' merge buffer, the row
+    // sort's scratch
+    L.cand_cap = grid_cand_capacity(nR, out_cap);
+    L.cursor = carve(8 * (GRID_SHARDS + 1));
+    L.cnt = carve(4 * (size_t)std::max<int64_t>(nR, 1));
+    L.src = carve(8 * (size_t)std::max<int64_t>(nR, 1));
     L.off = carve(8 * (size_t)(nR + 1));
     L.scan_part = carve(row_scan_part_bytes(nR));
-    L.outj = carve(4 * (size_t)(out_cap > 0 ? out_cap : 1));
+    L.outj = carve(4 * (size_t)L.cand_cap);
     L.outj2 = carve(4 * (size_t)(out_cap > 0 ? out_cap : 1));
-    L.longrows = carve(row_sort_scratch_bytes(nR));  // the long rows' count and list, the row classes
+    L.longrows = carve(row_sort_scratch_bytes(nR));
     L.total = (cur + 255) / 256 * 256;
     return L;
 }
@@ -693,11 +752,12 @@ int launch_grid_join(const float* R, int64_t nR, const float* S, int64_t nS, int
     GridParams* gp = (GridParams*)(ws + L.params);
     uint64_t* rkeys = (uint64_t*)(ws + L.rkeys);
     uint64_t* skeys = (uint64_t*)(ws + L.skeys);
-    float4* srow = (float4*)(ws + L.srow);
+    GPoint* srow = (GPoint*)(ws + L.srow);
     uint64_t* hkey = (uint64_t*)(ws + L.hkey);
     uint2* hval = (uint2*)(ws + L.hval);
     uint64_t* cR = (uint64_t*)(ws + L.count_nr);
     uint64_t* cS = (uint64_t*)(ws + L.count_ns);
+    const uint64_t hmask = L.hslots - 1;
     int rc;
     const unsigned gb = (unsigned)(num_cus() * 4);
     uint32_t* bpart = (uint32_t*)(ws + L.bpart);
@@ -720,16 +780,17 @@ int launch_grid_join(const float* R, int64_t nR, const float* S, int64_t nS, int
     // sorts once and reads its queries from S's rows.  The keys are written
     // in index order, so the digits wholly inside the index need no pass.
     const int lo_digit = L.idxbits / 8 * 8;
-    float4* rrow = self_join ? srow : (float4*)(ws + L.rrow);
+    GPoint* rrow = self_join ? srow : (GPoint*)(ws + L.rrow);
     if (!self_join) {
-        hipLaunchKernelGGL(keys_kernel, dim3(blocks_for((nR + 3) / 4, gb)), dim3(256), 0, s, R, nR, gp, rkeys);
+        hipLaunchKernelGGL(keys_kernel, dim3(blocks_for((nR + 3) / 4, gb)), dim3(256), 0, s, R, nR, gp, L.idxbits, rkeys);
         if ((rc = launch_sort(ws + L.sort_tmp, rkeys, cR, nR, 63, s, nullptr, lo_digit))) return rc;
-        hipLaunchKernelGGL(rows_kernel, dim3(blocks_for(nR, gb)), dim3(256), 0, s, R, nR, gp, rkeys, rrow);
+        hipLaunchKernelGGL(rows_kernel, dim3(blocks_for(nR, gb)), dim3(256), 0, s, R, nR, gp, L.idxbits,
+                           (const uint64_t*)rkeys, rrow, (uint64_t*)nullptr, (uint2*)nullptr, hmask, (uint64_t*)nullptr);
     }
-    hipLaunchKernelGGL(keys_kernel, dim3(blocks_for((nS + 3) / 4, gb)), dim3(256), 0, s, S, nS, gp, skeys);
+    hipLaunchKernelGGL(keys_kernel, dim3(blocks_for((nS + 3) / 4, gb)), dim3(256), 0, s, S, nS, gp, L.idxbits, skeys);
     if ((rc = launch_sort(ws + L.sort_tmp, skeys, cS, nS, 63, s, nullptr, lo_digit))) return rc;
-    hipLaunchKernelGGL(gather_kernel, dim3(blocks_for(nS, gb)), dim3(256), 0, s, S, nS, gp, skeys, srow, hkey, hval,
-                       (uint64_t)(L.hslots - 1), bm);
+    hipLaunchKernelGGL(rows_kernel, dim3(blocks_for(nS, gb)), dim3(256), 0, s, S, nS, gp, L.idxbits,
+                       (const uint64_t*)skeys, srow, hkey, hval, hmask, bm);
     // rank structure of the bitmap (its kernels return at once without one)
     uint32_t* bmpre = (uint32_t*)(ws + L.bmpre);
     uint32_t* bmsum = (uint32_t*)(ws + L.bmsum);
@@ -738,26 +799,26 @@ int launch_grid_join(const float* R, int64_t nR, const float* S, int64_t nS, int
     hipLaunchKernelGGL(bm_block_sums, dim3(nbm), dim3(256), 0, s, gp, bm, bmsum);
     hipLaunchKernelGGL(bm_top_scan, dim3(1), dim3(1024), 0, s, gp, bmsum);
     hipLaunchKernelGGL(bm_apply, dim3(nbm), dim3(256), 0, s, gp, bm, bmsum, bmpre);
-    hipLaunchKernelGGL(runs_kernel, dim3(blocks_for(nS, gb)), dim3(256), 0, s, nS, gp, skeys, bm, bmpre, runs);
-    // count pass -> offsets -> write pass -> per-row sort and emit
+    hipLaunchKernelGGL(runs_kernel, dim3(blocks_for(nS, gb)), dim3(256), 0, s, nS, gp, L.idxbits, skeys, bm, bmpre, runs);
+    // probe -> offsets -> per-row sort and emit
     uint32_t* cnt = counts ? counts : (uint32_t*)(ws + L.cnt);
+    uint64_t* src = (uint64_t*)(ws + L.src);
     uint64_t* off = (uint64_t*)(ws + L.off);
-    uint64_t* part = (uint64_t*)(ws + L.scan_part);
     uint32_t* outj = (uint32_t*)(ws + L.outj);
     uint32_t* outj2 = (uint32_t*)(ws + L.outj2);
-    const unsigned pb = blocks_for(nR, (int64_t)num_cus() * 16);
-    const uint64_t hmask = (uint64_t)(L.hslots - 1);
-    // (one instantiation per dimension: the distance has no runtime branches)
-    auto probe = [&](auto mode, uint32_t* c, const uint64_t* o, uint32_t* oj) {
-        constexpr int M = decltype(mode)::value;
-        auto kern = d == 1 ? probe_kernel<M, 1> : d == 2 ? probe_kernel<M, 2> : probe_kernel<M, 3>;
-        hipLaunchKernelGGL(kern, dim3(pb), dim3(256), 0, s, rrow, nR, gp, srow, hkey, hval, hmask, bm, bmpre, runs, T,
-                           (int)upper, c, o, oj, out_cap, stats);
-    };
-    probe(std::integral_constant<int, PROBE_COUNT>{}, cnt, (const uint64_t*)nullptr, (uint32_t*)nullptr);
-    if ((rc = launch_row_offsets(cnt, nR, part, off, stats + TJ_ST_OUT, s))) return rc;
-    probe(std::integral_constant<int, PROBE_WRITE>{}, (uint32_t*)nullptr, (const uint64_t*)off, outj);
-    if ((rc = launch_row_sort(off, nR, outj, outj2, out_cap, emit, pairs, nullptr, (uint32_t*)(ws + L.longrows), s)))
+    uint64_t* cursor = (uint64_t*)(ws + L.cursor);
+    if ((rc = launch_fill_u64(cursor, GRID_SHARDS + 1, 0, s))) return rc;
+    const unsigned pb = (unsigned)std::max<int64_t>(1, std::min<int64_t>((nR + 63) / 64, (int64_t)num_cus() * 16));
+    // (one instantiation per dimension and triangle flag: the distance has
+    // no runtime branches)
+    auto kern = upper ? (d == 1 ? probe_kernel<1, true> : d == 2 ? probe_kernel<2, true> : probe_kernel<3, true>)
+                      : (d == 1 ? probe_kernel<1, false> : d == 2 ? probe_kernel<2, false> : probe_kernel<3, false>);
+    hipLaunchKernelGGL(kern, dim3(pb), dim3(64 * PW), 0, s, (const GPoint*)rrow, nR, gp, (const GPoint*)srow,
+                       (const uint64_t*)hkey, (const uint2*)hval, hmask, (const uint64_t*)bm, (const uint32_t*)bmpre,
+                       (const uint32_t*)runs, T, cnt, src, outj, (int64_t)(L.cand_cap - out_cap), (int64_t)L.cand_cap,
+                       cursor, stats);
+    if ((rc = launch_row_offsets(cnt, nR, (uint64_t*)(ws + L.scan_part), off, stats + TJ_ST_OUT, s))) return rc;
+    if ((rc = launch_row_sort(off, nR, outj, outj2, out_cap, emit, pairs, nullptr, (uint32_t*)(ws + L.longrows), src, s)))
         return rc;
     TJ_HIP(hipGetLastError());
     return TJ_OK;

@@ -142,7 +142,8 @@ __global__ void __launch_bounds__(256) segsort_lane_kernel(const uint64_t* __res
                                                            const uint32_t* __restrict__ outj, int64_t out_cap, int emit,
                                                            void* __restrict__ out, const int32_t* __restrict__ ilocal,
                                                            uint32_t* __restrict__ longrows,
-                                                           uint32_t* __restrict__ nlong, uint8_t* __restrict__ codes) {
+                                                           uint32_t* __restrict__ nlong, uint8_t* __restrict__ codes,
+                                                           const uint64_t* __restrict__ src) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nR; i += stride) {
         uint64_t o0 = off[i], o1 = off[i + 1];
@@ -153,9 +154,10 @@ __global__ void __launch_bounds__(256) segsort_lane_kernel(const uint64_t* __res
         codes[i] = code;
         if (code == LONG_CODE) longrows[atomicAdd(nlong, 1u)] = (uint32_t)i;
         if (n > (uint32_t)LANE_MAX || n == 0) continue;
+        const uint64_t r0 = src ? src[i] : o0;
         uint32_t v[LANE_MAX];
 #pragma unroll
-        for (int k = 0; k < LANE_MAX; ++k) v[k] = (uint32_t)k < n ? outj[o0 + k] : 0xFFFFFFFFu;
+        for (int k = 0; k < LANE_MAX; ++k) v[k] = (uint32_t)k < n ? outj[r0 + k] : 0xFFFFFFFFu;
 #pragma unroll
         for (int kk = 2; kk <= LANE_MAX; kk <<= 1)
 #pragma unroll
@@ -179,7 +181,7 @@ template <int G, int E>
 __global__ void __launch_bounds__(256) midsort_kernel(const uint64_t* __restrict__ off, int64_t nR,
                                                       const uint32_t* __restrict__ outj, int64_t out_cap, int emit,
                                                       void* __restrict__ out, const int32_t* __restrict__ ilocal,
-                                                      const uint8_t* __restrict__ codes) {
+                                                      const uint8_t* __restrict__ codes, const uint64_t* __restrict__ src) {
     constexpr int P = G * E;
     constexpr int NG = 64 / G;  // rows per wave step
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -190,28 +192,29 @@ __global__ void __launch_bounds__(256) midsort_kernel(const uint64_t* __restrict
         uint64_t mine = __ballot(i < nR && codes[i] == code_of<P>());
         while (mine) {
             // the next NG rows of the class, one per group
-            int src = -1;
+            int row = -1;
 #pragma unroll
             for (int q = 0; q < NG; ++q) {
                 const int b = mine ? __builtin_ctzll(mine) : -1;
                 if (b >= 0) mine &= mine - 1;
-                if (q == g) src = b;
+                if (q == g) row = b;
             }
-            const int s = src < 0 ? 0 : src;  // tb + s < nR
-            uint64_t r0 = 0;
+            const int s = row < 0 ? 0 : row;  // tb + s < nR
+            uint64_t r0 = 0, w0 = 0;
             uint32_t rn = 0;
-            if (src >= 0) {
+            if (row >= 0) {
                 uint64_t o0 = off[tb + s], o1 = off[tb + s + 1];
                 if (o1 > (uint64_t)out_cap) o1 = (uint64_t)out_cap;
                 if (o0 > o1) o0 = o1;
                 r0 = o0;
+                w0 = src ? src[tb + s] : o0;
                 rn = (uint32_t)(o1 - o0);
             }
             uint32_t v[E];
 #pragma unroll
             for (int e = 0; e < E; ++e) {
                 const uint32_t x = (uint32_t)(G * e + l);
-                v[e] = x < rn ? outj[r0 + x] : 0xFFFFFFFFu;
+                v[e] = x < rn ? outj[w0 + x] : 0xFFFFFFFFu;
             }
 #pragma unroll
             for (int kk = 2; kk <= P; kk <<= 1)
@@ -259,7 +262,8 @@ constexpr int CHUNK = 4096;
 __global__ void __launch_bounds__(256) longsort_kernel(const uint64_t* __restrict__ off, const uint32_t* __restrict__ longrows,
                                                        const uint32_t* __restrict__ nlong, uint32_t* __restrict__ outj,
                                                        uint32_t* __restrict__ outj2, int64_t out_cap, int emit,
-                                                       void* __restrict__ out, const int32_t* __restrict__ ilocal) {
+                                                       void* __restrict__ out, const int32_t* __restrict__ ilocal,
+                                                       const uint64_t* __restrict__ rsrc) {
     __shared__ uint32_t S[CHUNK];
     const uint32_t count = *nlong;
     for (uint32_t r = blockIdx.x; r < count; r += gridDim.x) {
@@ -268,7 +272,9 @@ __global__ void __launch_bounds__(256) longsort_kernel(const uint64_t* __restric
         if (o1 > (uint64_t)out_cap) o1 = (uint64_t)out_cap;
         if (o0 > o1) o0 = o1;
         const uint64_t n = o1 - o0;
-        uint32_t* src = outj + o0;
+        // the row's ids ping-pong between its segment of outj and its final
+        // range of outj2
+        uint32_t* src = outj + (rsrc ? rsrc[i] : o0);
         uint32_t* dst = outj2 + o0;
         // sorted chunks; a row shorter than CHUNK runs the network of the
         // next power of two at or above its length
@@ -357,7 +363,7 @@ int launch_row_offsets(const uint32_t* cnt, int64_t n, uint64_t* part, uint64_t*
 }
 
 int launch_row_sort(const uint64_t* off, int64_t n, uint32_t* outj, uint32_t* outj2, int64_t out_cap, int emit,
-                    void* out, const int32_t* ilocal, uint32_t* longlist, hipStream_t s) {
+                    void* out, const int32_t* ilocal, uint32_t* longlist, const uint64_t* src, hipStream_t s) {
     if (n <= 0) return TJ_OK;
     uint32_t* nlong = longlist;
     uint32_t* longrows = longlist + 1;
@@ -366,21 +372,21 @@ int launch_row_sort(const uint64_t* off, int64_t n, uint32_t* outj, uint32_t* ou
     if ((rc = launch_fill_u64((uint64_t*)nlong, 1, 0, s))) return rc;  // (clears the count and its neighbour word)
     const dim3 sg(row_blocks(n, (int64_t)num_cus() * 8));
     hipLaunchKernelGGL(segsort_lane_kernel, sg, dim3(256), 0, s, off, n, outj, out_cap, emit, out, ilocal, longrows,
-                       nlong, codes);
+                       nlong, codes, src);
     hipLaunchKernelGGL((midsort_kernel<16, 1>), sg, dim3(256), 0, s, off, n, outj, out_cap, emit, out, ilocal,
-                       (const uint8_t*)codes);
+                       (const uint8_t*)codes, src);
     hipLaunchKernelGGL((midsort_kernel<16, 2>), sg, dim3(256), 0, s, off, n, outj, out_cap, emit, out, ilocal,
-                       (const uint8_t*)codes);
+                       (const uint8_t*)codes, src);
     hipLaunchKernelGGL((midsort_kernel<16, 4>), sg, dim3(256), 0, s, off, n, outj, out_cap, emit, out, ilocal,
-                       (const uint8_t*)codes);
+                       (const uint8_t*)codes, src);
     hipLaunchKernelGGL((midsort_kernel<16, 8>), sg, dim3(256), 0, s, off, n, outj, out_cap, emit, out, ilocal,
-                       (const uint8_t*)codes);
+                       (const uint8_t*)codes, src);
     hipLaunchKernelGGL((midsort_kernel<32, 8>), sg, dim3(256), 0, s, off, n, outj, out_cap, emit, out, ilocal,
-                       (const uint8_t*)codes);
+                       (const uint8_t*)codes, src);
     hipLaunchKernelGGL((midsort_kernel<64, 8>), sg, dim3(256), 0, s, off, n, outj, out_cap, emit, out, ilocal,
-                       (const uint8_t*)codes);
+                       (const uint8_t*)codes, src);
     hipLaunchKernelGGL(longsort_kernel, dim3((unsigned)num_cus()), dim3(256), 0, s, off, longrows, nlong, outj, outj2,
-                       out_cap, emit, out, ilocal);
+                       out_cap, emit, out, ilocal, src);
     TJ_HIP(hipGetLastError());
     return TJ_OK;
 }

@@ -215,8 +215,9 @@ inline int bits_for(int64_t n) { int b = 0; while (((int64_t)1 << b) < n) ++b; r
 namespace tj {
 // Workspace layout of the grid join (grid.hip), kept in the plan (api.hip).
 struct GridLayout {
-    size_t bounds, bpart, params, rkeys, skeys, sort_tmp, srow, rrow, hkey, hval, count_nr, count_ns, bitmap, bmpre,
-        bmsum, runs, cnt, off, scan_part, outj, outj2, longrows, total;
+    size_t bounds, bpart, params, count_nr, count_ns, rkeys, skeys, sort_tmp, srow, rrow, hkey, hval, bitmap, bmpre,
+        bmsum, runs, cursor, cnt, src, off, scan_part, outj, outj2, longrows, total;
+    int64_t cand_cap;  // ids the probe's candidate buffer holds
     uint64_t hslots;
     uint64_t bm_bits;  // capacity of the cell-occupancy bitmap
     int idxbits;
@@ -225,20 +226,23 @@ struct GridLayout {
 // GDS-Join (counts per point + neighbour ids grouped by point).
 enum { GRID_EMIT_I32 = TJ_OUT_I32, GRID_EMIT_I64 = TJ_OUT_I64, GRID_EMIT_TABLE = 16 };
 GridLayout grid_layout(int64_t nR, int64_t nS, int64_t out_cap);
+// Candidate buffer of a grid join with nR queries and out_cap output slots.
+int64_t grid_cand_capacity(int64_t nR, int64_t out_cap);
 // Row-grouped output (rowsort.hip).  launch_row_offsets: off[r] = counts of
 // rows before r, off[n] = the total (also to *total when not NULL); part is
 // row_scan_part_bytes(n) of scratch.  launch_row_sort: the ids of row r sit
-// in outj[off[r], off[r + 1]) (clamped to out_cap); each row is sorted and
-// emitted to out at the same positions, as (ilocal ? ilocal[r] : r, id)
-// rows in emit's format or bare ids (GRID_EMIT_TABLE); outj2 (out_cap ids)
-// is the long rows' merge buffer, longlist row_sort_scratch_bytes(n) of
-// scratch (256-byte aligned).
+// in outj at src[r] (or at off[r] when src is NULL), off[r + 1] - off[r] of
+// them (clamped to out_cap); each row is sorted and emitted to out at
+// positions [off[r], off[r + 1]), as (ilocal ? ilocal[r] : r, id) rows in
+// emit's format or bare ids (GRID_EMIT_TABLE); outj2 (out_cap ids) is the
+// long rows' merge buffer, longlist row_sort_scratch_bytes(n) of scratch
+// (256-byte aligned).
 size_t row_scan_part_bytes(int64_t n);
 size_t row_sort_scratch_bytes(int64_t n);
 int launch_row_offsets(const uint32_t* cnt, int64_t n, uint64_t* part, uint64_t* off, uint64_t* total,
                        hipStream_t s);
 int launch_row_sort(const uint64_t* off, int64_t n, uint32_t* outj, uint32_t* outj2, int64_t out_cap, int emit,
-                    void* out, const int32_t* ilocal, uint32_t* longlist, hipStream_t s);
+                    void* out, const int32_t* ilocal, uint32_t* longlist, const uint64_t* src, hipStream_t s);
 // self_join: S is R (one index); upper: keep j > i only (the self-join's
 // i < j output).  Table output: counts (uint32 [nR]) and table (uint32,
 // out_cap entries); pairs output: pairs ([out_cap, 2] in emit's format).
