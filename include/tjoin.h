@@ -76,7 +76,7 @@ extern "C" {
 #define TJ_ST_VIOLATIONS 7    /* refined pairs outside their certified interval (must be 0) */
 #define TJ_ST_WORK 8          /* work-queue cursor of the persistent screen */
 #define TJ_ST_INVALID 9       /* non-finite input coordinates seen by the row metadata pass */
-#define TJ_ST_GRID_CAND 10    /* candidate evaluations of the grid join */
+#define TJ_ST_GRID_CAND 10    /* grid join: candidates, the points in the cells adjacent to each query */
 #define TJ_ST_PAIRS 11        /* pair evaluations the call covers */
 #define TJ_ST_SCALE_EXP 12    /* join-unit exponent G (x_join = x * 2^-G), as int64 */
 #define TJ_ST_VIOL_KEY 13      /* diagnostic: key of a refined pair outside its interval (flag bit 63 = certified) */
@@ -270,7 +270,9 @@ int tj_plain_fp64(const float* R, int64_t nR, const float* S, int64_t nS, int d,
                   void* stream);
 
 /* The uniform-grid join (d <= 3) as one stage, with its own workspace
- * (tj_grid_workspace_bytes; capacity = rows of the output).  tj_grid_join:
+ * (tj_grid_workspace_bytes; capacity = rows of the output; the workspace
+ * holds a candidate buffer of 9 capacity + 2 nR + 1024 ids, so any output
+ * that fits the capacity is complete in one call).  tj_grid_join:
  * R x S (nS == 0: self-join of R, i < j) -> sorted (i, j) rows in
  * out_format, count stats[TJ_ST_OUT] (rows past capacity counted, not
  * written); stats is cleared first and receives TJ_ST_OUT, TJ_ST_GRID_CAND,
