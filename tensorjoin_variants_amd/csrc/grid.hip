@@ -11,7 +11,8 @@
 //    and the points written in key order -- coordinates widened to binary64
 //    (exact) and the original index, 32 bytes a row, so the probe converts
 //    nothing per candidate; S's non-empty cells go into an occupancy bitmap
-//    with a prefix count per word (grids of at most 64 cells per S point) or
+//    with a prefix count per word (grids of at most 4096 cells per S point
+//    and 2^28 cells) or
 //    an open-addressing hash of [start, end) runs (sparser grids).  (A
 //    counting sort over the cells, one device atomic per point, measured
 //    slower: scattered device atomics cost more than the sort's passes);
@@ -699,9 +700,11 @@ GridLayout grid_layout(int64_t nR, int64_t nS, int64_t out_cap) {
     while (hs < (uint64_t)(2 * std::max<int64_t>(nS, 1))) hs <<= 1;
     L.hslots = hs;
     L.idxbits = bits_for(nmax > 1 ? nmax : 2);
-    // occupancy bitmap of S: 64 bits per S point (at least 2^20); sparser
-    // grids go without it
-    L.bm_bits = std::max<uint64_t>(1ull << 20, 64ull * (uint64_t)std::max<int64_t>(nS, 1));
+    // occupancy bitmap of S: up to 4096 cells per S point, at least 2^20
+    // and at most 2^28 cells (32 MiB; only the words the grid uses are
+    // cleared and scanned); sparser grids go without it
+    L.bm_bits = std::max<uint64_t>(1ull << 20,
+                                   std::min<uint64_t>(1ull << 28, 4096ull * (uint64_t)std::max<int64_t>(nS, 1)));
     L.bounds = carve(64);
     L.bpart = carve(4 * 6 * 2 * (size_t)bounds_blocks());
     L.params = carve(sizeof(GridParams));

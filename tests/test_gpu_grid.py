@@ -239,3 +239,33 @@ def test_grid_stage_abi(tj, oracle):
         small = C.c_size_t(need.value - 256)
         assert L.tj_grid_join(Ad.data_ptr(), A.shape[0], None if Bd is None else Bd.data_ptr(), nS, 3, T,
                               ws.data_ptr(), small.value, cap, pairs.data_ptr(), 1, stats.data_ptr(), None) == -4
+
+
+def test_grid_plan_graph_replay(tj):
+    """A grid plan captured into a hipGraph (tj_plan_capture) and replayed
+    three times, eager work in between: every replay gives the eager pairs
+    (the grid's clears are fill kernels, its counts device-side)."""
+    import torch
+    from tensorjoin_variants_amd import _lib
+    from tensorjoin_variants_amd import workloads
+    from tensorjoin_variants_amd.join import Plan, _as_device, _ptr
+    w = workloads.grid_rxs(n=65_536)
+    Rd, Sd = _as_device(w.R), _as_device(w.S)
+    plan = Plan(w.R.shape[0], w.S.shape[0], 3, "grid", "none", out_capacity=1 << 20, amb_capacity=0)
+    eager = plan.run(Rd, Sd, w.T, np.int64).numpy()
+    pairs = torch.empty((plan.out_capacity, 2), dtype=torch.int64, device="cuda")
+    L = _lib.load()
+    say("grid plan graph capture")
+    _lib.check(L.tj_plan_capture(plan._plan, _ptr(Rd), _ptr(Sd), w.T, _ptr(plan._ws), _ptr(pairs), 1,
+                                 _ptr(plan._stats)), "tj_plan_capture")
+    for rep in range(3):
+        pairs.fill_(-1)
+        junk = torch.randn(1 << 20, device="cuda").sum()  # eager work between replays
+        torch.cuda.synchronize()
+        say("grid plan graph replay", rep)
+        _lib.check(L.tj_plan_replay(plan._plan, torch.cuda.current_stream().cuda_stream), "tj_plan_replay")
+        torch.cuda.synchronize()
+        n = int(plan._stats[0].item())
+        assert n == eager.shape[0]
+        assert np.array_equal(pairs[:n].cpu().numpy(), eager)
+        del junk
