@@ -54,7 +54,8 @@ struct GridParams {
     uint64_t ncell[3];
     uint64_t mul[3];  // linear id = sum c_k * mul[k] (mul[0] = 1: x-adjacent cells are adjacent ids)
     uint64_t cells;   // product of ncell
-    int keybits, d, use_bm, pad;  // use_bm: cells fit the occupancy bitmap
+    int keybits, d, use_bm;  // use_bm: cells fit the occupancy bitmap
+    int xb;  // bits of the sub-cell x position in S's sort keys (rows of a cell ordered by x)
 };
 
 // A point of the index: coordinates widened to binary64 (exact) and the
@@ -210,7 +211,9 @@ __global__ void grid_params_kernel(const uint32_t* bounds, double T, int d, int 
     g.d = d;
     g.cells = mul;
     g.use_bm = mul <= bm_bits ? 1 : 0;
-    g.pad = 0;
+    // S's keys order each cell's rows by their x position, quantized to xb
+    // bits (as many as fit, up to 8)
+    g.xb = 63 - bits - idxbits < 8 ? 63 - bits - idxbits : 8;
     *gp = g;
 }
 
@@ -236,17 +239,34 @@ __device__ inline void cell_of(const double* x, const GridParams& g, int64_t c[3
     }
 }
 
-// Sort keys: cell id << idxbits | point index.  Three-dimensional points go
+// Global quantized x of a point: its x cell << xb | its position within
+// the cell in 2^xb steps -- the same binary64 expression as cell_of, so it
+// is a non-decreasing function of x, and S's rows are ordered by it within
+// every row of cells.
+__device__ inline uint64_t xpos(double x, const GridParams& g) {
+    const double v = (x - g.lo[0]) * g.inv_side;
+    int64_t c = (int64_t)floor(v);
+    if (c < 1) c = 1;
+    if (c > (int64_t)g.ncell[0] - 2) c = (int64_t)g.ncell[0] - 2;
+    const double f = floor((v - (double)c) * (double)(1 << g.xb));
+    const int64_t q = f < 0.0 ? 0 : (f > (double)((1 << g.xb) - 1) ? (int64_t)((1 << g.xb) - 1) : (int64_t)f);
+    return ((uint64_t)c << g.xb) | (uint64_t)q;
+}
+
+// Sort keys: cell id << idxbits | point index; with sub (S's keys) the cell's
+// x position sits between them, so each cell's rows are ordered by x.  Three-dimensional points go
 // four at a time as three 16-byte loads and two 16-byte stores.
 __global__ void __launch_bounds__(256) keys_kernel(const float* __restrict__ x, int64_t n, const GridParams* __restrict__ gp,
-                                                   int idxbits, uint64_t* __restrict__ keys) {
+                                                   int idxbits, int sub, uint64_t* __restrict__ keys) {
     const GridParams g = *gp;
+    const int xb = sub ? g.xb : 0;
     auto key_of = [&](double x0, double x1, double x2, int64_t t) {
         const double q[3] = {x0, x1, x2};
         int64_t c[3];
         cell_of(q, g, c);
         const uint64_t id = (uint64_t)c[0] * g.mul[0] + (uint64_t)c[1] * g.mul[1] + (uint64_t)c[2] * g.mul[2];
-        return (id << idxbits) | (uint64_t)t;
+        const uint64_t xs = xb ? xpos(x0, g) & ((1ull << xb) - 1) : 0;
+        return (((id << xb) | xs) << idxbits) | (uint64_t)t;
     };
     const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
     const int64_t n4 = (g.d == 3 && ((uintptr_t)x & 15) == 0 && ((uintptr_t)keys & 15) == 0) ? n / 4 : 0;
@@ -281,9 +301,10 @@ __global__ void __launch_bounds__(256) rows_kernel(const float* __restrict__ x, 
                                                    uint64_t* __restrict__ bm) {
     const GridParams g = *gp;
     const uint64_t imask = (1ull << idxbits) - 1;
+    const int csh = idxbits + (hkey ? g.xb : 0);  // cell id = key >> csh (S's keys carry x positions)
     for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
         const uint64_t kk = keys[p];
-        const uint64_t j = kk & imask, cell = kk >> idxbits;
+        const uint64_t j = kk & imask, cell = kk >> csh;
         const float* y = x + j * g.d;
         GPoint r;
         r.x = y[0];
@@ -292,18 +313,18 @@ __global__ void __launch_bounds__(256) rows_kernel(const float* __restrict__ x, 
         r.idx = (uint32_t)j;
         r.pad = 0;
         rows[p] = r;
-        if (!hkey || (p > 0 && (keys[p - 1] >> idxbits) == cell)) continue;
+        if (!hkey || (p > 0 && (keys[p - 1] >> csh) == cell)) continue;
         if (g.use_bm) {
             atomicOr((unsigned long long*)&bm[cell >> 6], 1ull << (cell & 63));
             continue;
         }
         // end of the run: galloping search over the sorted keys
         int64_t lo = p, step = 1;  // keys[lo] is in the cell
-        while (lo + step < n && (keys[lo + step] >> idxbits) == cell) { lo += step; step <<= 1; }
+        while (lo + step < n && (keys[lo + step] >> csh) == cell) { lo += step; step <<= 1; }
         int64_t hi = lo + step < n ? lo + step : n;  // keys[hi] is past the cell (or hi = n)
         while (hi - lo > 1) {
             const int64_t mid = lo + (hi - lo) / 2;
-            if ((keys[mid] >> idxbits) == cell) lo = mid; else hi = mid;
+            if ((keys[mid] >> csh) == cell) lo = mid; else hi = mid;
         }
         uint64_t h = hmix(cell) & hmask;
         for (;;) {
@@ -405,9 +426,10 @@ __global__ void __launch_bounds__(256) runs_kernel(int64_t n, const GridParams* 
                                                    const uint32_t* __restrict__ bmpre, uint32_t* __restrict__ runs) {
     const GridParams g = *gp;
     if (!g.use_bm) return;
+    const int csh = idxbits + g.xb;
     for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
-        const uint64_t cell = keys[p] >> idxbits;
-        if (p == 0 || (keys[p - 1] >> idxbits) != cell) runs[cell_rank(bm, bmpre, cell)] = (uint32_t)p;
+        const uint64_t cell = keys[p] >> csh;
+        if (p == 0 || (keys[p - 1] >> csh) != cell) runs[cell_rank(bm, bmpre, cell)] = (uint32_t)p;
         if (p == n - 1) runs[cell_rank(bm, bmpre, cell) + 1] = (uint32_t)n;
     }
 }
@@ -497,6 +519,49 @@ __global__ void __launch_bounds__(64 * PW) probe_kernel(const GPoint* __restrict
             const int dz = D == 3 ? rr / 3 - 1 : 0, dy = D >= 2 ? rr % 3 - 1 : 0;
             return (uint64_t)c[0] + (uint64_t)(c[1] + dy) * g.mul[1] + (uint64_t)(c[2] + dz) * g.mul[2];
         };
+        // Reach of neighbour row rr: dd <= the squared yz distance from the
+        // query to any point of the row's cells (the cell faces, pulled in by
+        // 2^-12 of a side: more than the binary64 cell assignment can move a
+        // point).  A row with dd > Tw holds no neighbour (Tw exceeds T by far
+        // more than the gap between d2_seq and the exact d2); otherwise a long
+        // range keeps only the rows whose x lies within w = sqrt(Tw - dd) of
+        // the query's (a cell's S rows are ordered by x position, xpos): the
+        // rest are at least w apart in x alone.
+        const double mg = g.side * 0x1p-12, Tw = T * (1.0 + 0x1p-30);
+        auto trim = [&](int rr, uint2& run) {
+            const int dz = D == 3 ? rr / 3 - 1 : 0, dy = D >= 2 ? rr % 3 - 1 : 0;
+            double dd = 0.0;
+            if (dy) {
+                const double f = g.lo[1] + (double)(c[1] + (dy > 0 ? 1 : 0)) * g.side;
+                const double t = (dy > 0 ? f - q[1] : q[1] - f) - mg;
+                if (t > 0.0) dd += t * t;
+            }
+            if (dz) {
+                const double f = g.lo[2] + (double)(c[2] + (dz > 0 ? 1 : 0)) * g.side;
+                const double t = (dz > 0 ? f - q[2] : q[2] - f) - mg;
+                if (t > 0.0) dd += t * t;
+            }
+            if (dd > Tw) {
+                run = make_uint2(0, 0);
+                return;
+            }
+            if (!g.xb || run.y - run.x <= 32) return;
+            const double w = sqrt(Tw - dd) + mg;
+            const uint64_t xl = xpos(q[0] - w, g), xh = xpos(q[0] + w, g);
+            uint32_t lo = run.x, hi = run.y;  // first row with xpos >= xl
+            while (lo < hi) {
+                const uint32_t m = lo + (hi - lo) / 2;
+                if (xpos(srow[m].x, g) < xl) lo = m + 1;
+                else hi = m;
+            }
+            uint32_t lo2 = lo, hi2 = run.y;  // first row with xpos > xh
+            while (lo2 < hi2) {
+                const uint32_t m = lo2 + (hi2 - lo2) / 2;
+                if (xpos(srow[m].x, g) <= xh) lo2 = m + 1;
+                else hi2 = m;
+            }
+            run = make_uint2(lo, lo2);
+        };
         // ---- candidate ranges of this wave's rows and their total
         uint32_t mycand = 0;
         for (int rr = wave; rr < NROWS; rr += PW) {
@@ -512,6 +577,7 @@ __global__ void __launch_bounds__(64 * PW) probe_kernel(const GPoint* __restrict
                     if (occ) {
                         const uint32_t rk = pre + (uint32_t)__popcll(sh ? (w0 & ((1ull << sh) - 1)) : 0ull);
                         run = make_uint2(runs[rk], runs[rk + (uint32_t)__popc(occ)]);
+                        trim(rr, run);
                     }
                 }
                 sRun[rr][lane] = run;
@@ -785,12 +851,12 @@ int launch_grid_join(const float* R, int64_t nR, const float* S, int64_t nS, int
     const int lo_digit = L.idxbits / 8 * 8;
     GPoint* rrow = self_join ? srow : (GPoint*)(ws + L.rrow);
     if (!self_join) {
-        hipLaunchKernelGGL(keys_kernel, dim3(blocks_for((nR + 3) / 4, gb)), dim3(256), 0, s, R, nR, gp, L.idxbits, rkeys);
+        hipLaunchKernelGGL(keys_kernel, dim3(blocks_for((nR + 3) / 4, gb)), dim3(256), 0, s, R, nR, gp, L.idxbits, 0, rkeys);
         if ((rc = launch_sort(ws + L.sort_tmp, rkeys, cR, nR, 63, s, nullptr, lo_digit))) return rc;
         hipLaunchKernelGGL(rows_kernel, dim3(blocks_for(nR, gb)), dim3(256), 0, s, R, nR, gp, L.idxbits,
                            (const uint64_t*)rkeys, rrow, (uint64_t*)nullptr, (uint2*)nullptr, hmask, (uint64_t*)nullptr);
     }
-    hipLaunchKernelGGL(keys_kernel, dim3(blocks_for((nS + 3) / 4, gb)), dim3(256), 0, s, S, nS, gp, L.idxbits, skeys);
+    hipLaunchKernelGGL(keys_kernel, dim3(blocks_for((nS + 3) / 4, gb)), dim3(256), 0, s, S, nS, gp, L.idxbits, 1, skeys);
     if ((rc = launch_sort(ws + L.sort_tmp, skeys, cS, nS, 63, s, nullptr, lo_digit))) return rc;
     hipLaunchKernelGGL(rows_kernel, dim3(blocks_for(nS, gb)), dim3(256), 0, s, S, nS, gp, L.idxbits,
                        (const uint64_t*)skeys, srow, hkey, hval, hmask, bm);
