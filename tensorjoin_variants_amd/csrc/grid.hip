@@ -479,7 +479,7 @@ __device__ inline uint64_t shfl_up_u64(uint64_t v, int o) {
 // fast_cap on) take the steps whose candidates no longer fit: they count
 // first and reserve their exact neighbour total there (cursor[GRID_SHARDS]),
 // so any output that fits its capacity is written whatever the candidate
-// total; stats[TJ_ST_GRID_CAND] receives that total.
+// total; cand_total_kernel adds the candidate total to stats[TJ_ST_GRID_CAND].
 constexpr int PW = 4;  // waves per probe workgroup
 template <int D, bool UPPER>
 __global__ void __launch_bounds__(64 * PW) probe_kernel(const GPoint* __restrict__ rrow, int64_t nR,
@@ -501,7 +501,6 @@ __global__ void __launch_bounds__(64 * PW) probe_kernel(const GPoint* __restrict
     // this workgroup's slice of the fast region (one cursor per slice)
     const int shard = (int)(blockIdx.x % GRID_SHARDS);
     const int64_t slice = fast_cap / GRID_SHARDS;
-    uint64_t wcand = 0;  // candidates of this workgroup (one atomic at the end)
     const int64_t stride = (int64_t)gridDim.x * 64;
     for (int64_t tb = (int64_t)blockIdx.x * 64; tb < nR; tb += stride) {
         const int64_t t = tb + lane;
@@ -564,30 +563,50 @@ __global__ void __launch_bounds__(64 * PW) probe_kernel(const GPoint* __restrict
         };
         // ---- candidate ranges of this wave's rows and their total
         uint32_t mycand = 0;
-        for (int rr = wave; rr < NROWS; rr += PW) {
-            if (g.use_bm) {
-                uint2 run = make_uint2(0, 0);
-                if (valid) {
-                    const uint64_t b0 = row_mid(rr) - 1, b2 = b0 + 2;
-                    const uint64_t w0 = bm[b0 >> 6];
-                    const uint32_t pre = bmpre[b0 >> 6];
-                    const uint32_t sh = (uint32_t)(b0 & 63);
-                    uint32_t occ = (uint32_t)((w0 >> sh) & 7);
-                    if (sh > 61 && (b2 >> 6) != (b0 >> 6)) occ |= (uint32_t)(bm[b2 >> 6] << (64 - sh)) & 7;
-                    if (occ) {
-                        const uint32_t rk = pre + (uint32_t)__popcll(sh ? (w0 & ((1ull << sh) - 1)) : 0ull);
-                        run = make_uint2(runs[rk], runs[rk + (uint32_t)__popc(occ)]);
-                        trim(rr, run);
-                    }
+        // (bitmap grids: this wave's rows -- at most three -- in three
+        // rounds of independent loads: bitmap words and prefix counts, run
+        // bounds, then the x-window trims)
+        constexpr int MYR = (NROWS + PW - 1) / PW;
+        if (g.use_bm) {
+            uint64_t w0[MYR], b0[MYR];
+            uint32_t pre[MYR];
+#pragma unroll
+            for (int u = 0; u < MYR; ++u) {
+                const int rr = wave + u * PW;
+                const bool use = valid && rr < NROWS;
+                b0[u] = row_mid(use ? rr : 0) - 1;
+                w0[u] = use ? bm[b0[u] >> 6] : 0ull;
+                pre[u] = use ? bmpre[b0[u] >> 6] : 0u;
+            }
+            uint2 run[MYR];
+#pragma unroll
+            for (int u = 0; u < MYR; ++u) {
+                const int rr = wave + u * PW;
+                const bool use = valid && rr < NROWS;
+                const uint64_t b2 = b0[u] + 2;
+                const uint32_t sh = (uint32_t)(b0[u] & 63);
+                uint32_t occ = use ? (uint32_t)((w0[u] >> sh) & 7) : 0u;
+                if (use && sh > 61 && (b2 >> 6) != (b0[u] >> 6)) occ |= (uint32_t)(bm[b2 >> 6] << (64 - sh)) & 7;
+                run[u] = make_uint2(0, 0);
+                if (occ) {
+                    const uint32_t rk = pre[u] + (uint32_t)__popcll(sh ? (w0[u] & ((1ull << sh) - 1)) : 0ull);
+                    run[u] = make_uint2(runs[rk], runs[rk + (uint32_t)__popc(occ)]);
                 }
-                sRun[rr][lane] = run;
-                mycand += run.y - run.x;
-            } else if (valid) {
+            }
+#pragma unroll
+            for (int u = 0; u < MYR; ++u) {
+                const int rr = wave + u * PW;
+                if (rr >= NROWS) break;
+                if (run[u].y > run[u].x) trim(rr, run[u]);
+                sRun[rr][lane] = run[u];
+                mycand += run[u].y - run[u].x;
+            }
+        } else if (valid) {
+            for (int rr = wave; rr < NROWS; rr += PW)
                 for (int dx = -1; dx <= 1; ++dx) {
                     const uint2 run = lookup(hkey, hval, hmask, row_mid(rr) + (uint64_t)(int64_t)dx);
                     mycand += run.y - run.x;
                 }
-            }
         }
         sCand[wave][lane] = mycand;
         __syncthreads();
@@ -606,7 +625,6 @@ __global__ void __launch_bounds__(64 * PW) probe_kernel(const GPoint* __restrict
             uint64_t base = 0;
             if (lane == 0 && W) base = atomicAdd((unsigned long long*)&cursor[shard], (unsigned long long)W);
             base = shfl_u64(base, 0);
-            wcand += W;
             sSeg[lane] = (uint64_t)shard * (uint64_t)slice + base + (inc - cand);
             sMine[lane] = 0;
             if (lane == 0) sPhase = base + W <= (uint64_t)slice ? 2 : 0;
@@ -743,7 +761,16 @@ __global__ void __launch_bounds__(64 * PW) probe_kernel(const GPoint* __restrict
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0 && wcand) atomicAdd((unsigned long long*)&stats[TJ_ST_GRID_CAND], (unsigned long long)wcand);
+}
+
+// stats[TJ_ST_GRID_CAND] = the candidates of all steps: every step adds its
+// total to its slice's cursor, whether it fits or not (one atomic per step
+// on a shared counter would serialise).
+__global__ void cand_total_kernel(const uint64_t* __restrict__ cursor, uint64_t* __restrict__ stats) {
+    if (threadIdx.x != 0) return;
+    uint64_t t = 0;
+    for (int s = 0; s < GRID_SHARDS; ++s) t += cursor[s];
+    stats[TJ_ST_GRID_CAND] += t;
 }
 
 }  // namespace
@@ -877,15 +904,20 @@ int launch_grid_join(const float* R, int64_t nR, const float* S, int64_t nS, int
     uint32_t* outj2 = (uint32_t*)(ws + L.outj2);
     uint64_t* cursor = (uint64_t*)(ws + L.cursor);
     if ((rc = launch_fill_u64(cursor, GRID_SHARDS + 1, 0, s))) return rc;
-    const unsigned pb = (unsigned)std::max<int64_t>(1, std::min<int64_t>((nR + 63) / 64, (int64_t)num_cus() * 16));
     // (one instantiation per dimension and triangle flag: the distance has
     // no runtime branches)
     auto kern = upper ? (d == 1 ? probe_kernel<1, true> : d == 2 ? probe_kernel<2, true> : probe_kernel<3, true>)
                       : (d == 1 ? probe_kernel<1, false> : d == 2 ? probe_kernel<2, false> : probe_kernel<3, false>);
+    // one workgroup per group of 64 queries (up to 2^20 workgroups, then
+    // they stride): the groups' work is very uneven (a blob core's queries
+    // hold thousands of candidates, the uniform field's about one), and the
+    // dispatcher hands each freed CU slot the next group
+    const unsigned pb = (unsigned)std::max<int64_t>(1, std::min<int64_t>((nR + 63) / 64, (int64_t)1 << 20));
     hipLaunchKernelGGL(kern, dim3(pb), dim3(64 * PW), 0, s, (const GPoint*)rrow, nR, gp, (const GPoint*)srow,
                        (const uint64_t*)hkey, (const uint2*)hval, hmask, (const uint64_t*)bm, (const uint32_t*)bmpre,
                        (const uint32_t*)runs, T, cnt, src, outj, (int64_t)(L.cand_cap - out_cap), (int64_t)L.cand_cap,
                        cursor, stats);
+    hipLaunchKernelGGL(cand_total_kernel, dim3(1), dim3(64), 0, s, (const uint64_t*)cursor, stats);
     if ((rc = launch_row_offsets(cnt, nR, (uint64_t*)(ws + L.scan_part), off, stats + TJ_ST_OUT, s))) return rc;
     if ((rc = launch_row_sort(off, nR, outj, outj2, out_cap, emit, pairs, nullptr, (uint32_t*)(ws + L.longrows), src, s)))
         return rc;
