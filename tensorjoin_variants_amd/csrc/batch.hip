@@ -770,7 +770,7 @@ __global__ void __launch_bounds__(256, 2) batch_screen_kernel(const ScreenBatchA
         const int64_t ce = rg ? J.grow + (int64_t)uni(rg[2 + 2 * r]) : J.grow + (int64_t)J.n;
 #define X(ID, PR, DPV)                                                                                          \
     case ID:                                                                                                    \
-        screen_panel<PR, DPV, rb_batch<PR, DPV>(), true, CCAP_BATCH>(io, A.co, r0, cb, ce, sB, sNp, sW, sSc, sCK[wave], \
+        screen_panel<PR, DPV, rb_batch<PR, DPV>(), true, CCAP_BATCH, true>(io, A.co, r0, cb, ce, sB, sNp, sW, sSc, sCK[wave], \
                                                        sCP[wave], sT[wave], nC, nAcc, A.fast_accept);                     \
         break;
         if constexpr (FAM == 0) {

@@ -319,7 +319,7 @@ constexpr int ring_bytes() { return Geo<PREC, DP>::NSTAGE * Geo<PREC, DP>::BLKB;
 // as rigorously as the seeded chain's output does.  Per 32 x 32 tile a lane
 // first tests fl(max_r p_r + max_r t_r) >= v_j (p_r + t_r is at most
 // max p + max t), and only tiles that pass form the 16 sums.
-template <int PREC, int DP, int RB, bool SELF, int CC = CCAP>
+template <int PREC, int DP, int RB, bool SELF, int CC = CCAP, bool GROUPED = false>
 __device__ __forceinline__ void screen_panel(const PanelIO& io, const CandOut& co, int64_t r0, int64_t cb, int64_t ce,
                                              uint8_t* sB, float* sNp, float* sW, uint32_t* sSc, uint64_t* sCK,
                                              float* sCP, float* sT, int& nC, uint32_t& nAcc, int fast_accept) {
@@ -637,8 +637,88 @@ __device__ __forceinline__ void screen_panel(const PanelIO& io, const CandOut& c
                     nC = uni(nC + k);
                 }
             };
-            if (SELF && rw0 + rb * 32 + 31 >= c0) stage(std::true_type{});
-            else stage(std::false_type{});
+            // Grouped staging (the batched screens, whose self-joins leave a
+            // few percent of a tile's pairs as survivors, spread over most
+            // of a lane's 16 registers): the registers go in groups of
+            // GR = CC / 64 (at most CC survivors a group); a group's masks,
+            // their counts and running offsets are formed first (compares,
+            // then scalar bit counts and adds), one capacity test covers the
+            // group, and every register's survivors are then written at
+            // positions that depend on no other register.  The single-join
+            // bodies keep the per-register form above: their survivors are
+            // rare, and the grouped form's code changes the schedule of
+            // their hot loop (the d = 128 body ran 8 % slower with it).
+            auto gstage = [&](auto diag) {
+                constexpr bool DIAG = decltype(diag)::value;
+                int e = 64;
+                if constexpr (DIAG) {
+                    const int df = (int)(c0 - (rw0 + rb * 32)) + c32 - 4 * hh;
+                    e = df < 0 ? 0 : (df > 64 ? 64 : df);
+                }
+                int js = io.jbits;
+                __asm__ volatile("" : "+s"(js));
+                constexpr int GR = CC / 64 >= 16 ? 16 : CC / 64;
+                static_assert(GR >= 4 && 16 % GR == 0, "register groups tile the 16 registers in fours");
+#pragma unroll
+                for (int g0 = 0; g0 < 16; g0 += GR) {
+                    bool pr[GR];
+                    float sv[GR];
+                    uint64_t ms[GR];
+                    int base[GR];
+                    int tot = 0;
+                    // unseeded: the row levels t of the group's registers
+                    // (register r holds row 8 (r / 4) + 4 hh + r % 4 of the
+                    // block: four registers are one 16-byte read)
+                    float tr[SEEDED ? 1 : GR];
+                    if constexpr (!SEEDED) {
+#pragma unroll
+                        for (int q = 0; q < GR; q += 4) {
+                            const float4 tv = *(const float4*)(sT + rb * 32 + 8 * ((g0 + q) >> 2) + 4 * hh);
+                            tr[q] = tv.x; tr[q + 1] = tv.y; tr[q + 2] = tv.z; tr[q + 3] = tv.w;
+                        }
+                    }
+#pragma unroll
+                    for (int q = 0; q < GR; ++q) {
+                        const int r = g0 + q;
+                        const int off = 8 * (r >> 2) + (r & 3);
+                        sv[q] = acc[rb][r];
+                        bool p = acc[rb][r] >= (SEEDED ? nj : thr);
+                        if constexpr (!SEEDED) {
+                            sv[q] += tr[q];
+                            p = p && sv[q] >= nj;
+                        }
+                        if constexpr (DIAG) p = p && off < e;
+                        pr[q] = p;
+                        ms[q] = __builtin_amdgcn_ballot_w64(p);
+                        base[q] = tot;
+                        tot += __popcll(ms[q]);
+                    }
+                    tot = uni(tot);
+                    if (tot == 0) continue;
+                    nC = uni(nC);  // wave-uniform: keeps the capacity test a scalar branch
+                    if (__builtin_expect(nC + tot > CC, 0)) cand_flush(co, sCK, sCP, nC, nAcc, lane);  // out of line
+#pragma unroll
+                    for (int q = 0; q < GR; ++q) {
+                        if (ms[q] == 0) continue;
+                        if (pr[q]) {
+                            const int off = 8 * ((g0 + q) >> 2) + ((g0 + q) & 3);
+                            const int pos = nC + base[q] + lanes_below(ms[q]);
+                            uint64_t key = kb + ((uint64_t)off << js);
+                            if (sv[q] >= uj) key |= ACCEPT_BIT;
+                            sCK[pos] = key;
+                            sCP[pos] = sv[q];
+                        }
+                    }
+                    nC = uni(nC + tot);
+                }
+            };
+            if constexpr (GROUPED) {
+                if (SELF && rw0 + rb * 32 + 31 >= c0) gstage(std::true_type{});
+                else gstage(std::false_type{});
+            } else {
+                if (SELF && rw0 + rb * 32 + 31 >= c0) stage(std::true_type{});
+                else stage(std::false_type{});
+            }
         }
     };
 
