@@ -521,7 +521,7 @@ __device__ inline uint64_t shfl_u64(uint64_t v, int src) {
 }
 
 constexpr int ROWS_LDS = 16384;  // rows of a join whose counters live in LDS (64 KiB)
-constexpr int JR_THREADS = 256;
+constexpr int JR_THREADS = 1024;
 
 template <bool SCATTER>
 __global__ void __launch_bounds__(256) batch_bucket_kernel(const uint64_t* __restrict__ keys,
@@ -593,6 +593,27 @@ __global__ void __launch_bounds__(256) batch_bucket_kernel(const uint64_t* __res
     }
 }
 
+// Runs of equal values among a wave's lanes: whether this lane starts one,
+// the run's first lane h and (at the head) its length.
+struct RowRun {
+    bool head;
+    int h;
+    uint32_t len;
+};
+__device__ inline RowRun row_run(uint32_t v, int lane) {
+    const uint32_t prev = (uint32_t)__shfl((int)v, lane > 0 ? lane - 1 : 0);
+    const bool head = lane == 0 || prev != v;
+    const uint64_t H = __ballot(head);
+    const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1);  // lanes <= this one
+    const uint64_t hb = H & upto, ha = H & ~upto;
+    RowRun r;
+    r.head = head;
+    r.h = 63 - __clzll(hb);                       // lane 0 is always a head
+    const int nxt = ha ? __builtin_ctzll(ha) : 64;  // the next run's head
+    r.len = (uint32_t)(nxt - lane);
+    return r;
+}
+
 // Step 2 for the joins listed in order[0, njl): LDS counters (joins of at
 // most ROWS_LDS rows) or global ones (gcnt, one per batch row).  roff[r] =
 // first output position of batch row r; the first workgroup also writes
@@ -616,19 +637,27 @@ __global__ void __launch_bounds__(JR_THREADS) batch_join_rows_kernel(const int32
         for (int a = tid; a < n; a += JR_THREADS) c[a] = 0;
         __syncthreads();
         const uint64_t k0 = joff[t], k1 = joff[t + 1];
-        // UN keys per thread per step, loaded before any is used: a
-        // workgroup per join holds few waves, so the loads must overlap
+        // UN keys per thread per step, loaded before any is used (their
+        // latencies overlap)
         constexpr int UN = 8;
-        for (uint64_t k = k0 + tid; k < k1; k += (uint64_t)JR_THREADS * UN) {
+        // (k - lane is wave-uniform: every lane of a wave runs the same
+        // iterations, as the run detection below needs)
+        for (uint64_t k = k0 + tid; k - (uint64_t)lane < k1; k += (uint64_t)JR_THREADS * UN) {
             uint32_t rows_u[UN];
 #pragma unroll
             for (int u = 0; u < UN; ++u) {
                 const uint64_t kk = k + (uint64_t)JR_THREADS * u;
                 rows_u[u] = kk < k1 ? (uint32_t)(bucket[kk] >> 32) : 0xFFFFFFFFu;
             }
+            // one LDS atomic per run of equal rows among a wave's lanes (the
+            // survivors of one register share their row: same-address
+            // atomics of a wave would serialise)
 #pragma unroll
-            for (int u = 0; u < UN; ++u)
-                if (rows_u[u] != 0xFFFFFFFFu) atomicAdd(&c[rows_u[u]], 1u);
+            for (int u = 0; u < UN; ++u) {
+                const uint32_t rw = rows_u[u];
+                const RowRun rr = row_run(rw, lane);
+                if (rr.head && rw != 0xFFFFFFFFu) atomicAdd(&c[rw], rr.len);
+            }
         }
         __syncthreads();
         // exclusive scan of the row counts, JR_THREADS rows at a time: each
@@ -653,7 +682,7 @@ __global__ void __launch_bounds__(JR_THREADS) batch_join_rows_kernel(const int32
             for (int w = 0; w < JR_THREADS / 64; ++w) run += wsum[w];
             __syncthreads();
         }
-        for (uint64_t k = k0 + tid; k < k1; k += (uint64_t)JR_THREADS * UN) {
+        for (uint64_t k = k0 + tid; k - (uint64_t)lane < k1; k += (uint64_t)JR_THREADS * UN) {
             uint64_t v[UN];
 #pragma unroll
             for (int u = 0; u < UN; ++u) {
@@ -662,9 +691,12 @@ __global__ void __launch_bounds__(JR_THREADS) batch_join_rows_kernel(const int32
             }
 #pragma unroll
             for (int u = 0; u < UN; ++u) {
-                if (v[u] == ~0ull) continue;
-                const uint32_t pos = atomicAdd(&c[(uint32_t)(v[u] >> 32)], 1u);
-                outj[k0 + pos] = (uint32_t)v[u];
+                const uint32_t rw = v[u] == ~0ull ? 0xFFFFFFFFu : (uint32_t)(v[u] >> 32);
+                const RowRun rr = row_run(rw, lane);
+                uint32_t pos = 0;
+                if (rr.head && rw != 0xFFFFFFFFu) pos = atomicAdd(&c[rw], rr.len);
+                pos = (uint32_t)__shfl((int)pos, rr.h) + (uint32_t)(lane - rr.h);
+                if (rw != 0xFFFFFFFFu) outj[k0 + pos] = (uint32_t)v[u];
             }
         }
         __syncthreads();  // the counters are reused by the next join
