@@ -146,6 +146,19 @@ class Plan:
                        _lib.OUT[out_format], _ptr(self._stats), _stream_ptr(stream))
         _lib.check(rc, "tj_join")
 
+    def capture(self, R, S, T: float, pairs, out_format: str) -> None:
+        """Record one join with these buffers into the plan's hipGraph
+        (tj_plan_capture, synchronous); replay() launches it.  A later
+        capture replaces the graph; capacities must not change in between."""
+        L = _lib.load()
+        rc = L.tj_plan_capture(self._plan, _ptr(R), _ptr(S), float(T), _ptr(self._ws), _ptr(pairs),
+                               _lib.OUT[out_format], _ptr(self._stats))
+        _lib.check(rc, "tj_plan_capture")
+
+    def replay(self, stream=None) -> None:
+        """Launch the captured join on the stream (no synchronisation)."""
+        _lib.check(_lib.load().tj_plan_replay(self._plan, _stream_ptr(stream)), "tj_plan_replay")
+
     def run(self, R, S, T: float, out_dtype=np.int64, max_retries: int = 3) -> JoinResult:
         """Join with capacity retry; returns device pairs and the decoded stats."""
         t = torch()
