@@ -214,10 +214,12 @@ struct Geo {
 };
 
 // Per-wave staged candidates: 256 for the single-join screens (a larger
-// buffer slows the FP8 d = 512 body by a fifth), 512 for the batched
-// screens, whose dense-survivor joins flush half as often.
+// buffer slows the FP8 d = 512 body by a fifth), 896 for the batched
+// screens, whose dense-survivor joins then flush (two atomics on the two
+// shared cursors, a wait for their return) 1.75 times less often; 896 is
+// what the batched kernel's LDS leaves for two workgroups per CU.
 constexpr int CCAP = 256;
-constexpr int CCAP_BATCH = 512;
+constexpr int CCAP_BATCH = 896;
 
 // Operands of one panel sweep.  Row indices are positions in the quantized
 // arrays; the emitted key of pair (i, j) is key_pack(i + koff_a, j + koff_b),
@@ -657,8 +659,8 @@ __device__ __forceinline__ void screen_panel(const PanelIO& io, const CandOut& c
                 }
                 int js = io.jbits;
                 __asm__ volatile("" : "+s"(js));
-                constexpr int GR = CC / 64 >= 16 ? 16 : CC / 64;
-                static_assert(GR >= 4 && 16 % GR == 0, "register groups tile the 16 registers in fours");
+                constexpr int GR = CC >= 1024 ? 16 : (CC >= 512 ? 8 : 4);
+                static_assert(CC >= 256, "a group of GR registers fits the staging buffer");
 #pragma unroll
                 for (int g0 = 0; g0 < 16; g0 += GR) {
                     bool pr[GR];
