@@ -442,58 +442,84 @@ __global__ void __launch_bounds__(256) batch_prune_kernel(const BatchJoin* __res
                                                           const JoinParams* __restrict__ params,
                                                           const float* __restrict__ balls,
                                                           int32_t* __restrict__ ranges, uint64_t* __restrict__ stats) {
-    const int64_t w = (int64_t)blockIdx.x * 4 + tjscreen::uni((int)(threadIdx.x >> 6));
-    if (w >= n_items) return;
-    const int lane = threadIdx.x & 63;
-    const int2 it = items[w];
-    const BatchJoin J = joins[it.x];
-    const JoinParams& P = params[it.x];
-    const int d = J.d;
-    const int panels = (J.n + J.bm - 1) / J.bm;
-    const float* bp = balls + J.ball_off + (int64_t)it.y * (d + 1);
-    const double rp = (double)bp[d];
-    const double lim = P.T * (1.0 + 2.0 * P.g64) * (1.0 + 0x1p-50);
-    int nr = 0;
-    int rb[MAXR], re[MAXR];  // runs in panel units [rb, re)
-    for (int q = it.y; q < panels; ++q) {
-        bool keep = q == it.y;
-        if (!keep) {
-            const float* bq = balls + J.ball_off + (int64_t)q * (d + 1);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // the skipped pairs (and their work) of this wave's items, added to the
+    // stats by one atomic pair per workgroup: an atomic per item on the
+    // same two counters serialised and cost as much as the pruning
+    uint64_t skip_pairs = 0, skip_work = 0;
+    for (int64_t w = (int64_t)blockIdx.x * 4 + tjscreen::uni(wave); w < n_items; w += (int64_t)gridDim.x * 4) {
+        const int2 it = items[w];
+        const BatchJoin J = joins[it.x];
+        const JoinParams& P = params[it.x];
+        const int d = J.d;
+        const int panels = (J.n + J.bm - 1) / J.bm;
+        const float* bp = balls + J.ball_off + (int64_t)it.y * (d + 1);
+        const double rp = (double)bp[d];
+        const double lim = P.T * (1.0 + 2.0 * P.g64) * (1.0 + 0x1p-50);
+        int nr = 0;
+        int rb[MAXR], re[MAXR];  // runs in panel units [rb, re)
+        // 64 / GW panels q at a time, a group of GW lanes per panel (GW grows with
+        // d so that a lane sums at most 16 of the squared differences, in
+        // binary64: the bound below holds for any summation order), then the
+        // kept panels in order from the wave's mask
+        const int GW = d <= 16 ? 1 : d <= 64 ? 4 : d <= 256 ? 16 : d <= 512 ? 32 : 64;
+        const int QPS = 64 / GW;
+        const int sub = lane % GW;
+        for (int q0 = it.y; q0 < panels; q0 += QPS) {
+            const int q = q0 + lane / GW;
+            bool keep = q == it.y;
             double a = 0.0;
-            for (int k = lane; k < d; k += 64) {
+            const float* bq = balls + J.ball_off + (int64_t)(q < panels ? q : it.y) * (d + 1);
+            for (int k = sub; k < d; k += GW) {
                 const double u = (double)bp[k] - (double)bq[k];
                 a += u * u;
             }
-            for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o);
-            // |c_p - c_q| from below, then the gap from below
-            const double dist = sqrt(a * (1.0 - (double)(d + 4) * 0x1p-52)) * (1.0 - 0x1p-50);
-            const double rs = rp + (double)bq[d];
-            double gap = dist - rs;
-            gap -= (dist + rs) * 0x1p-50;
-            keep = !(gap > 0.0 && gap * gap * (1.0 - 0x1p-50) > lim);
+            for (int o = GW >> 1; o > 0; o >>= 1) a += __shfl_xor(a, o);
+            if (q < panels && !keep) {
+                // |c_p - c_q| from below, then the gap from below
+                const double dist = sqrt(a * (1.0 - (double)(d + 4) * 0x1p-52)) * (1.0 - 0x1p-50);
+                const double rs = rp + (double)bq[d];
+                double gap = dist - rs;
+                gap -= (dist + rs) * 0x1p-50;
+                keep = !(gap > 0.0 && gap * gap * (1.0 - 0x1p-50) > lim);
+            }
+            const uint64_t kb = __ballot(q < panels && keep && sub == 0);  // one bit per group: lane GW g
+            for (int g = 0; g < QPS; ++g) {
+                if (((kb >> (g * GW)) & 1ull) == 0) continue;
+                const int qk = q0 + g;
+                if (nr > 0 && re[nr - 1] == qk) re[nr - 1] = qk + 1;   // extends the current run
+                else if (nr < MAXR) { rb[nr] = qk; re[nr] = qk + 1; ++nr; }
+                else re[nr - 1] = qk + 1;                              // merged into the last run
+            }
         }
-        if (!keep) continue;
-        if (nr > 0 && re[nr - 1] == q) re[nr - 1] = q + 1;   // extends the current run
-        else if (nr < MAXR) { rb[nr] = q; re[nr] = q + 1; ++nr; }
-        else re[nr - 1] = q + 1;                             // merged into the last run
+        if (lane == 0) {
+            int32_t* out = ranges + w * RSTRIDE;
+            out[0] = nr;
+            int64_t swept = 0;
+            for (int r = 0; r < nr; ++r) {
+                const int64_t c0 = (int64_t)rb[r] * J.bm;
+                const int64_t c1 = (int64_t)re[r] * J.bm < J.n ? (int64_t)re[r] * J.bm : J.n;
+                out[1 + 2 * r] = (int32_t)c0;
+                out[2 + 2 * r] = (int32_t)c1;
+                swept += c1 - c0;
+            }
+            const int64_t r0 = (int64_t)it.y * J.bm;
+            const int64_t rows = r0 + J.bm < J.n ? (int64_t)J.bm : J.n - r0;
+            const int64_t skipped = rows * (J.n - r0 - swept);
+            if (skipped > 0) {
+                skip_pairs += (uint64_t)skipped;
+                skip_work += (uint64_t)skipped * (uint64_t)d;
+            }
+        }
     }
-    if (lane == 0) {
-        int32_t* out = ranges + w * RSTRIDE;
-        out[0] = nr;
-        int64_t swept = 0;
-        for (int r = 0; r < nr; ++r) {
-            const int64_t c0 = (int64_t)rb[r] * J.bm;
-            const int64_t c1 = (int64_t)re[r] * J.bm < J.n ? (int64_t)re[r] * J.bm : J.n;
-            out[1 + 2 * r] = (int32_t)c0;
-            out[2 + 2 * r] = (int32_t)c1;
-            swept += c1 - c0;
-        }
-        const int64_t r0 = (int64_t)it.y * J.bm;
-        const int64_t rows = r0 + J.bm < J.n ? (int64_t)J.bm : J.n - r0;
-        const int64_t skipped = rows * (J.n - r0 - swept);
-        if (skipped > 0) {
-            atomicAdd((unsigned long long*)&stats[TJ_ST_PRUNED], (unsigned long long)skipped);
-            atomicAdd((unsigned long long*)&stats[TJ_ST_PRUNED_WORK], (unsigned long long)(skipped * d));
+    __shared__ uint64_t sp[4], sw[4];
+    if (lane == 0) { sp[wave] = skip_pairs; sw[wave] = skip_work; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint64_t a = sp[0] + sp[1] + sp[2] + sp[3], b = sw[0] + sw[1] + sw[2] + sw[3];
+        if (a) {
+            atomicAdd((unsigned long long*)&stats[TJ_ST_PRUNED], (unsigned long long)a);
+            atomicAdd((unsigned long long*)&stats[TJ_ST_PRUNED_WORK], (unsigned long long)b);
         }
     }
 }
@@ -994,7 +1020,7 @@ static int batch_enqueue(const BatchCall& c, hipStream_t s) {
         hipLaunchKernelGGL(batch_ball_kernel, dim3((unsigned)b->n_items), dim3(256), 0, s, c.x, b->d_joins,
                            b->d_items, perm, b->jbits, balls);
         TJ_HIP(hipGetLastError());
-        hipLaunchKernelGGL(batch_prune_kernel, dim3((unsigned)((b->n_items + 3) / 4)), dim3(256), 0, s, b->d_joins,
+        hipLaunchKernelGGL(batch_prune_kernel, dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>((b->n_items + 3) / 4, (int64_t)num_cus() * 8))), dim3(256), 0, s, b->d_joins,
                            b->d_items, b->n_items, params, balls, ranges, stats);
         TJ_HIP(hipGetLastError());
     }
