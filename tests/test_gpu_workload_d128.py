@@ -88,3 +88,30 @@ def test_d128_workload_full_size(tj, oracle):
     got = cert[:n_k].cpu().numpy().astype(np.int64)
     assert np.array_equal(got, want)
     assert len(want) > 0
+
+
+def test_d128_workload_graph_replay(tj, small):
+    """The timed form of the metric join: the whole join captured once into
+    a hipGraph (Plan.capture) and replayed, eager work between replays, on
+    the reduced workload; every replay gives the oracle's pairs."""
+    import torch
+    from tensorjoin_variants_amd.join import Plan, _as_device
+    w, want = small
+    R, S = _as_device(w.R), _as_device(w.S)
+    plan = Plan(w.R.shape[0], w.S.shape[0], 128, "certified", "fp16", out_capacity=1 << 17, amb_capacity=1 << 17)
+    res = plan.run(R, S, w.T, np.int32)
+    assert np.array_equal(res.numpy().astype(np.int64), want)
+    pairs = torch.empty((plan.out_capacity, 2), dtype=torch.int32, device="cuda")
+    say("d128 workload graph capture")
+    plan.capture(R, S, w.T, pairs, "int32")
+    for rep in range(3):
+        pairs.fill_(-1)
+        junk = torch.randn(1 << 20, device="cuda").sum()  # eager work between replays
+        torch.cuda.synchronize()
+        say("d128 workload graph replay", rep)
+        plan.replay()
+        torch.cuda.synchronize()
+        n = int(plan._stats[0].item())
+        assert n == len(want)
+        assert np.array_equal(pairs[:n].cpu().numpy().astype(np.int64), want)
+        del junk
