@@ -66,6 +66,25 @@ def test_grid_boundaries(tj, oracle):
     check(tj, oracle, V[:1500], V[1500:], 0.25, "sparse grid rxs")
 
 
+def test_grid_xwindow_lattice(tj, oracle):
+    """Dense lattice blocks (512 points per cell): every neighbour row is a
+    long range, so each query trims it to its x window by binary search on
+    the S rows' sub-cell x position and skips rows beyond its y/z reach.
+    Lattice coordinates (multiples of 2^-9, centred on 0) make d2 exact, with
+    many pairs at exactly d2 = T on the window edges and cell faces.  The
+    second set adds two far points, which leave fewer key bits for the x
+    position (a coarser window)."""
+    g = (np.arange(24, dtype=np.float64) - 12.0) * 2.0 ** -9
+    X = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3).astype(np.float32)
+    T = 64.0 * 2.0 ** -18  # eps = 8 lattice steps
+    check(tj, oracle, X, None, T, "lattice blocks self")
+    check(tj, oracle, X[::7].copy(), X, T, "lattice blocks rxs")
+    far = np.array([[500.0, 500.0, 500.0], [-500.0, 499.0, -498.0]], dtype=np.float32)
+    Xf = np.concatenate([X, far]).astype(np.float32)
+    check(tj, oracle, Xf, None, T, "lattice blocks + far points self")
+    check(tj, oracle, Xf[::5].copy(), Xf, T, "lattice blocks + far points rxs")
+
+
 def test_grid_edge_sizes(tj, oracle):
     rng = np.random.default_rng(9)
     X = rng.random((70, 3)).astype(np.float32)
