@@ -309,7 +309,8 @@ __device__ inline void order_unit(const float* x, const BatchJoin& J, int t, con
         for (int k = 0; k < NPIV; ++k) {
             const float u = xv - pv[k];
             float acc = u * u;
-            for (int o = OGW / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+            for (int o = OGW / 2; o > 0; o >>= 1)
+                acc += __builtin_bit_cast(float, tj::xor_lane(__builtin_bit_cast(uint32_t, acc), o, lane));
             if (acc < best) { best = acc; bk = k; }  // NaN rows stay in bucket 0
         }
         if (act && lg == 0)

@@ -117,27 +117,6 @@ __device__ inline void emit_one(int emit, void* out, uint64_t at, uint32_t i, ui
 constexpr int LANE_MAX = 8;
 constexpr int GROUP_MAX = 512;  // 16 lanes x 32 ids
 
-// v of lane (lane ^ jj): DPP moves within a 16-lane row for jj < 16 (quad
-// permutations for 1 and 2, row rotations for 4 and 8: no LDS round trip),
-// a bpermute for 16 and 32 (the gfx950 row / half swaps
-// v_permlane16/32_swap measured slower there: midsort<64, 8> 0.39 -> 0.44 ms).
-// jj is a compile-time constant after unrolling, so the switch folds away.
-__device__ __forceinline__ uint32_t xor_lane(uint32_t v, int jj, int lane) {
-    const int x = (int)v;
-    switch (jj) {
-        case 1: return (uint32_t)__builtin_amdgcn_mov_dpp(x, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
-        case 2: return (uint32_t)__builtin_amdgcn_mov_dpp(x, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
-        case 4: {
-            // row_ror:4 brings lane l - 4, row_ror:12 lane l + 4 (mod 16)
-            const int a = __builtin_amdgcn_mov_dpp(x, 0x124, 0xF, 0xF, false);
-            const int b = __builtin_amdgcn_mov_dpp(x, 0x12C, 0xF, 0xF, false);
-            return (uint32_t)((lane & 4) ? a : b);
-        }
-        case 8: return (uint32_t)__builtin_amdgcn_mov_dpp(x, 0x128, 0xF, 0xF, false);  // row_ror:8
-        default: return (uint32_t)__shfl_xor(x, jj);
-    }
-}
-
 __device__ __forceinline__ void cx(uint32_t& a, uint32_t& b, bool asc) {
     const uint32_t lo = min(a, b), hi = max(a, b);
     a = asc ? lo : hi;
@@ -254,7 +233,7 @@ __global__ void __launch_bounds__(256) midsort_kernel(const uint64_t* __restrict
                         // of an ascending pair (or the upper of a descending one)
 #pragma unroll
                         for (int e = 0; e < E; ++e) {
-                            const uint32_t p = xor_lane(v[e], jj, lane);
+                            const uint32_t p = tj::xor_lane(v[e], jj, lane);
                             const int x = G * e + l;
                             const bool asc = (x & kk) == 0;
                             const bool lower = (l & jj) == 0;

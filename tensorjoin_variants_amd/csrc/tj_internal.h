@@ -211,8 +211,32 @@ __device__ inline double d2_seq(const float* __restrict__ x, const float* __rest
 __host__ __device__ inline uint64_t key_pack(uint64_t i, uint64_t j, int jbits) { return (i << jbits) | j; }
 inline int bits_for(int64_t n) { int b = 0; while (((int64_t)1 << b) < n) ++b; return b; }
 
-// ------------------------------------------------------------- launchers
 namespace tj {
+
+// ------------------------------------------------------ cross-lane moves
+// v of lane (lane ^ jj): DPP moves within a 16-lane row for jj < 16 (quad
+// permutations for 1 and 2, row rotations for 4 and 8: no LDS round trip),
+// a bpermute for 16 and 32 (the gfx950 row / half swaps
+// v_permlane16/32_swap measured slower there: midsort<64, 8> 0.39 -> 0.44 ms).
+// jj is a compile-time constant after unrolling, so the switch folds away.
+__device__ __forceinline__ uint32_t xor_lane(uint32_t v, int jj, int lane) {
+    const int x = (int)v;
+    switch (jj) {
+        case 1: return (uint32_t)__builtin_amdgcn_mov_dpp(x, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
+        case 2: return (uint32_t)__builtin_amdgcn_mov_dpp(x, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
+        case 4: {
+            // row_ror:4 brings lane l - 4, row_ror:12 lane l + 4 (mod 16)
+            const int a = __builtin_amdgcn_mov_dpp(x, 0x124, 0xF, 0xF, false);
+            const int b = __builtin_amdgcn_mov_dpp(x, 0x12C, 0xF, 0xF, false);
+            return (uint32_t)((lane & 4) ? a : b);
+        }
+        case 8: return (uint32_t)__builtin_amdgcn_mov_dpp(x, 0x128, 0xF, 0xF, false);  // row_ror:8
+        default: return (uint32_t)__shfl_xor(x, jj);
+    }
+}
+
+// ------------------------------------------------------------- launchers
+
 // Workspace layout of the grid join (grid.hip), kept in the plan (api.hip).
 struct GridLayout {
     size_t bounds, bpart, params, count_nr, count_ns, rkeys, skeys, sort_tmp, srow, rrow, hkey, hval, bitmap, bmpre,
