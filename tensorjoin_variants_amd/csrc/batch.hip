@@ -202,18 +202,22 @@ __device__ inline int64_t perm_of(const uint64_t* perm, int64_t k, uint64_t jmas
 
 // One workgroup per join: farthest-point sampling over m = min(n, 64,
 // 8192 / d) sample rows held in LDS (binary32 distances: only the order's
-// quality depends on them).
+// quality depends on them).  Sample rows sit d + 1 floats apart (with a
+// stride of d, the 64 threads of a distance pass hit one LDS bank when d is
+// a multiple of 64); the farthest sample is a wave reduction; the pivot
+// chain's 8 x 8 distances are formed by 64 threads at once.
 __global__ void __launch_bounds__(256) batch_pivot_kernel(const float* const* __restrict__ xs,
                                                           const BatchJoin* __restrict__ joins,
                                                           int32_t* __restrict__ pivots) {
-    __shared__ float sx[8192];
+    __shared__ float sx[8192 + 64];  // m (d + 1) <= 8192 + m
     __shared__ float smin[64];
+    __shared__ float sdm[NPIV][NPIV];
     __shared__ int spick;
     __shared__ int spiv[NPIV];
     const int t = blockIdx.x;
     const BatchJoin J = joins[t];
     const int tid = threadIdx.x;
-    const int n = J.n, d = J.d;
+    const int n = J.n, d = J.d, ds = d + 1;
     int m = n < 64 ? n : 64;
     if (m > 8192 / d) m = 8192 / d;
     if (m < 1) m = 1;
@@ -224,9 +228,9 @@ __global__ void __launch_bounds__(256) batch_pivot_kernel(const float* const* __
     // (m >= 1 samples; every pivot below is a sample index < m)
     const float* x = xs[t];
     for (int e = tid; e < m * d; e += 256) {
-        const int s = e / d, k = e - s * d;
-        const int64_t row = (int64_t)s * n / m;
-        sx[e] = x[row * d + k];
+        const int s2 = e / d, k = e - s2 * d;
+        const int64_t row = (int64_t)s2 * n / m;
+        sx[s2 * ds + k] = x[row * d + k];
     }
     if (tid < 64) smin[tid] = INFINITY;
     if (tid == 0) spick = 0;
@@ -237,24 +241,36 @@ __global__ void __launch_bounds__(256) batch_pivot_kernel(const float* const* __
         if (tid < m) {
             float acc = 0.0f;
             for (int c = 0; c < d; ++c) {
-                const float u = sx[tid * d + c] - sx[pk * d + c];
+                const float u = sx[tid * ds + c] - sx[pk * ds + c];
                 acc += u * u;
             }
             smin[tid] = fminf(smin[tid], acc);
         }
         __syncthreads();
-        if (tid == 0) {
-            int best = 0;
-            float bv = -1.0f;
-            for (int s2 = 0; s2 < m; ++s2)
-                if (smin[s2] > bv) { bv = smin[s2]; best = s2; }
-            spick = best;
+        if (tid < 64) {
+            // the first sample of largest distance (NaN distances never win;
+            // none valid: sample 0)
+            const float v = tid < m ? smin[tid] : -1.0f;
+            float mx = v;
+            for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+            const uint64_t hit = __ballot(v == mx && mx > -1.0f);
+            if (tid == 0) spick = hit ? __builtin_ctzll(hit) : 0;
         }
         __syncthreads();
     }
     // Chain the pivots greedily (each next = the nearest one not yet taken),
     // so that pivots of one cluster get adjacent buckets and a cluster's rows
     // stay one contiguous run of the order.
+    if (tid < NPIV * NPIV) {
+        const int a2 = tid / NPIV, b2 = tid % NPIV;
+        float acc = 0.0f;
+        for (int c = 0; c < d; ++c) {
+            const float u = sx[spiv[a2] * ds + c] - sx[spiv[b2] * ds + c];
+            acc += u * u;
+        }
+        sdm[a2][b2] = acc;
+    }
+    __syncthreads();
     if (tid == 0) {
         bool used[NPIV] = {};
         int cur = 0;
@@ -265,11 +281,7 @@ __global__ void __launch_bounds__(256) batch_pivot_kernel(const float* const* __
             float bv = INFINITY;
             for (int k = 0; k < NPIV; ++k) {
                 if (used[k]) continue;
-                float acc = 0.0f;
-                for (int c = 0; c < d; ++c) {
-                    const float u = sx[spiv[cur] * d + c] - sx[spiv[k] * d + c];
-                    acc += u * u;
-                }
+                const float acc = sdm[cur][k];
                 if (best < 0 || acc < bv) { bv = acc; best = k; }
             }
             used[best] = true;
