@@ -417,10 +417,8 @@ __global__ void __launch_bounds__(256) batch_ball_kernel(const float* const* __r
                 a += u * u;
             }
         }
-        a += __shfl_xor(a, 8);
-        a += __shfl_xor(a, 4);
-        a += __shfl_xor(a, 2);
-        a += __shfl_xor(a, 1);
+#pragma unroll
+        for (int o = 8; o > 0; o >>= 1) a += __builtin_bit_cast(float, tj::xor_lane(__builtin_bit_cast(uint32_t, a), o, lane));
         if (act) {
             bad |= !(a < INFINITY);
             rmax = fmaxf(rmax, a);
