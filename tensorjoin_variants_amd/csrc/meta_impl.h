@@ -62,14 +62,19 @@ __device__ inline JoinParams make_params(uint32_t maxabs_bits, double T, int d, 
 // Lane t of the group handles coordinates t, t+GW, ... of the padded row.
 // Sums of non-negative binary64 terms, any order: relative error
 // <= (n+1) 2^-53.
+// (butterflies: DPP moves for partners within 16 lanes)
 template <int GW>
 __device__ inline double group_sum(double x) {
-    for (int o = GW / 2; o > 0; o >>= 1) x += __shfl_xor(x, o);
+    const int lane = (int)__lane_id();
+#pragma unroll
+    for (int o = GW / 2; o > 0; o >>= 1) x += tj::xor_lane_f64(x, o, lane);
     return x;
 }
 template <int GW>
 __device__ inline double group_max(double x) {
-    for (int o = GW / 2; o > 0; o >>= 1) x = fmax(x, __shfl_xor(x, o));
+    const int lane = (int)__lane_id();
+#pragma unroll
+    for (int o = GW / 2; o > 0; o >>= 1) x = fmax(x, tj::xor_lane_f64(x, o, lane));
     return x;
 }
 

@@ -234,6 +234,12 @@ __device__ __forceinline__ uint32_t xor_lane(uint32_t v, int jj, int lane) {
         default: return (uint32_t)__shfl_xor(x, jj);
     }
 }
+// The same for a binary64 value (its two halves).
+__device__ __forceinline__ double xor_lane_f64(double v, int jj, int lane) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = xor_lane((uint32_t)u, jj, lane), hi = xor_lane((uint32_t)(u >> 32), jj, lane);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
 
 // ------------------------------------------------------------- launchers
 
