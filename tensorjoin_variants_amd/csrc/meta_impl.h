@@ -252,7 +252,9 @@ __device__ inline void rowmeta_wave(Src src, int64_t n, int64_t r0, int d, int d
         const bool act = r < n;
         const int64_t rc = act ? r : n - 1;
         const RowSums s = rowmeta_sums<PREC, GW>(src(rc), base + rc, d, dp, P, lg, qhi, qlo, act);
-        const RowSums t = shfl_sums(s, (lane % RPW) * GW);  // lane L <- the group of row r0 + L
+        // lane L <- the group of row r0 + L (one group per step: the
+        // butterfly sums already sit in every lane)
+        const RowSums t = RPW == 1 ? s : shfl_sums(s, (lane % RPW) * GW);
         if (lane / RPW == step) mine = t;
     }
     const int64_t my = r0 + lane;
