@@ -51,6 +51,7 @@ def load():
     if not LIB_PATH.exists():
         raise TJError(f"{LIB_PATH} is missing: run tensorjoin_variants_amd.build.build() "
                       "(hipcc --offload-arch=gfx950); there is no CPU fallback")
+    _init_torch_device()  # before the library (and its HIP runtime) is loaded
     L = C.CDLL(str(LIB_PATH))
     vp, i64, i32, f64, sz = C.c_void_p, C.c_int64, C.c_int, C.c_double, C.c_size_t
     sig = {
@@ -103,6 +104,25 @@ def load():
         fn.argtypes = args
     _lib = L
     return L
+
+
+def _init_torch_device() -> None:
+    """Initialise PyTorch's GPU runtime before libtjoin.so is loaded.
+
+    The device buffers come from PyTorch, whose wheel carries its own HIP and
+    HSA runtime beside the system ROCm one libtjoin.so links.  Measured on the
+    box: with PyTorch's runtime initialised first, both work in one process;
+    with libtjoin.so loaded first, whichever runtime opens the device second
+    finds no GPU ("No HIP GPUs are available" from PyTorch after a batched
+    join as a process's first call; "no ROCm-capable device" from the
+    library's hipMalloc when PyTorch initialised after the load).  Without a
+    GPU (the CPU tests only inspect the symbols) this is a no-op."""
+    try:
+        import torch
+    except ImportError:
+        return
+    if torch.cuda.is_available():
+        torch.cuda.init()
 
 
 def check(rc: int, what: str = "") -> None:
