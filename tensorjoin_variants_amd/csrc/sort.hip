@@ -36,7 +36,18 @@ __global__ void __launch_bounds__(SB) hist_kernel(const uint64_t* __restrict__ k
     const int64_t chunk = ((n + nb - 1) / nb + SB - 1) / SB * SB;
     const int64_t b0 = (int64_t)blockIdx.x * chunk;
     const int64_t b1 = b0 + chunk < n ? b0 + chunk : n;
-    for (int64_t t = b0 + threadIdx.x; t < b1; t += SB) atomicAdd(&h[(keys[t] >> shift) & 255u], 1u);
+    // eight keys per thread in flight (one load at a time left the pass
+    // waiting on memory latency)
+    constexpr int U = 8;
+    int64_t t = b0 + threadIdx.x;
+    for (; t + (int64_t)(U - 1) * SB < b1; t += (int64_t)U * SB) {
+        uint32_t dg[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) dg[u] = (uint32_t)(keys[t + (int64_t)u * SB] >> shift) & 255u;
+#pragma unroll
+        for (int u = 0; u < U; ++u) atomicAdd(&h[dg[u]], 1u);
+    }
+    for (; t < b1; t += SB) atomicAdd(&h[(keys[t] >> shift) & 255u], 1u);
     __syncthreads();
     hist[(int64_t)threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
 }
