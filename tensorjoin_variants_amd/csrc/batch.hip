@@ -418,7 +418,8 @@ __global__ void __launch_bounds__(256) batch_ball_kernel(const float* const* __r
             }
         }
 #pragma unroll
-        for (int o = 8; o > 0; o >>= 1) a += __builtin_bit_cast(float, tj::xor_lane(__builtin_bit_cast(uint32_t, a), o, lane));
+        for (int o = 8; o > 0; o >>= 1)
+            a += __builtin_bit_cast(float, tj::xor_lane(__builtin_bit_cast(uint32_t, a), o, lane));
         if (act) {
             bad |= !(a < INFINITY);
             rmax = fmaxf(rmax, a);
@@ -1031,8 +1032,11 @@ static int batch_enqueue(const BatchCall& c, hipStream_t s) {
         hipLaunchKernelGGL(batch_ball_kernel, dim3((unsigned)b->n_items), dim3(256), 0, s, c.x, b->d_joins,
                            b->d_items, perm, b->jbits, balls);
         TJ_HIP(hipGetLastError());
-        hipLaunchKernelGGL(batch_prune_kernel, dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>((b->n_items + 3) / 4, (int64_t)num_cus() * 8))), dim3(256), 0, s, b->d_joins,
-                           b->d_items, b->n_items, params, balls, ranges, stats);
+        // grid-stride over the items, a few workgroups per CU (one stats
+        // atomic pair each)
+        const int64_t pgrid = std::max<int64_t>(1, std::min<int64_t>((b->n_items + 3) / 4, (int64_t)num_cus() * 8));
+        hipLaunchKernelGGL(batch_prune_kernel, dim3((unsigned)pgrid), dim3(256), 0, s, b->d_joins, b->d_items,
+                           b->n_items, params, balls, ranges, stats);
         TJ_HIP(hipGetLastError());
     }
     TJ_MARK(1);
