@@ -103,9 +103,13 @@ __device__ inline RowSums shfl_sums(const RowSums& s, int src) {
 }
 
 // Quantize one row (its GW lanes; lane t of the group handles coordinates
-// t, t+GW, ... of the padded row), write its codes at index `row`, and return
-// its sums.  act == false: the group takes part in the shuffles but stores
-// nothing (a group past the last row).
+// t, t+GW, ... of the padded row -- or, for rows of whole 16-byte quads
+// (d % 4 == 0, the row 16-byte aligned, dp a multiple of 4 GW), the quads
+// t, t+GW, ...: one 16-byte load and one 8- or 4-byte code store per four
+// coordinates), write its codes at index `row`, and return its sums (binary64
+// sums of non-negative terms: their bounds hold in any order).  act == false:
+// the group takes part in the shuffles but stores nothing (a group past the
+// last row).
 template <int PREC, int GW = 64>
 __device__ inline RowSums rowmeta_sums(const float* __restrict__ xr, int64_t row, int d, int dp, const JoinParams& P,
                                        int lane, uint8_t* __restrict__ qhi, uint8_t* __restrict__ qlo, bool act) {
@@ -116,9 +120,18 @@ __device__ inline RowSums rowmeta_sums(const float* __restrict__ xr, int64_t row
 
     // FP8: the per-row scale needs the row max first.
     int e_hi = 0, e_lo = 0;
+    const bool quads = (d & 3) == 0 && dp % (4 * GW) == 0 && ((uintptr_t)xr & 15) == 0;
     if (FP8) {
         double m = 0.0;
-        for (int k = lane; k < d; k += GW) m = fmax(m, fabs((double)xr[k] * sc));
+        if (quads) {
+            for (int k = 4 * lane; k < d; k += 4 * GW) {
+                const float4 q = *(const float4*)(xr + k);
+                m = fmax(m, fmax(fmax(fabs((double)q.x * sc), fabs((double)q.y * sc)),
+                                 fmax(fabs((double)q.z * sc), fabs((double)q.w * sc))));
+            }
+        } else {
+            for (int k = lane; k < d; k += GW) m = fmax(m, fabs((double)xr[k] * sc));
+        }
         m = group_max<GW>(m);
         e_hi = fp8_row_exp(m);
         if (TERMS == 3) {
@@ -136,31 +149,59 @@ __device__ inline RowSums rowmeta_sums(const float* __restrict__ xr, int64_t row
     }
 
     double nx = 0.0, nh = 0.0, nl = 0.0, nr = 0.0;
-    for (int k = lane; k < dp; k += GW) {
-        double v = k < d ? (double)xr[k] * sc : 0.0;
+    // one coordinate: its codes (hi, lo) and its terms of the four sums
+    auto one = [&](double v, uint32_t& ch, uint32_t& cl) {
         double bh = 0.0, bl = 0.0;
+        cl = 0;
         if (FP8) {
-            uint8_t ch = enc_e4m3(ldexp(v, -e_hi), &bh);
+            ch = enc_e4m3(ldexp(v, -e_hi), &bh);
             bh = ldexp(bh, e_hi);
-            if (act) qhi[row * dp + k] = ch;
             if (TERMS == 3) {
-                uint8_t cl = enc_e4m3(ldexp(v - bh, -e_lo), &bl);
+                cl = enc_e4m3(ldexp(v - bh, -e_lo), &bl);
                 bl = ldexp(bl, e_lo);
-                if (act) qlo[row * dp + k] = cl;
             }
         } else {
-            uint16_t ch = BF16 ? enc_bf16(v, &bh) : enc_fp16(v, &bh);
-            if (act) ((uint16_t*)qhi)[row * dp + k] = ch;
-            if (TERMS == 3) {
-                uint16_t cl = BF16 ? enc_bf16(v - bh, &bl) : enc_fp16(v - bh, &bl);
-                if (act) ((uint16_t*)qlo)[row * dp + k] = cl;
-            }
+            ch = BF16 ? enc_bf16(v, &bh) : enc_fp16(v, &bh);
+            if (TERMS == 3) cl = BF16 ? enc_bf16(v - bh, &bl) : enc_fp16(v - bh, &bl);
         }
-        double r = v - bh - bl;  // exact: the codes are roundings of v and of v - bh
+        const double r = v - bh - bl;  // exact: the codes are roundings of v and of v - bh
         nx += v * v;
         nh += bh * bh;
         nl += bl * bl;
         nr += r * r;
+    };
+    if (quads) {
+        for (int k = 4 * lane; k < dp; k += 4 * GW) {
+            // a quad lies wholly inside the row or wholly in its padding
+            const float4 q = k < d ? *(const float4*)(xr + k) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            uint32_t h[4], l[4];
+            one((double)q.x * sc, h[0], l[0]);
+            one((double)q.y * sc, h[1], l[1]);
+            one((double)q.z * sc, h[2], l[2]);
+            one((double)q.w * sc, h[3], l[3]);
+            if (!act) continue;
+            if (FP8) {
+                *(uint32_t*)(qhi + row * dp + k) = h[0] | (h[1] << 8) | (h[2] << 16) | (h[3] << 24);
+                if (TERMS == 3) *(uint32_t*)(qlo + row * dp + k) = l[0] | (l[1] << 8) | (l[2] << 16) | (l[3] << 24);
+            } else {
+                *(uint2*)((uint16_t*)qhi + row * dp + k) = make_uint2(h[0] | (h[1] << 16), h[2] | (h[3] << 16));
+                if (TERMS == 3)
+                    *(uint2*)((uint16_t*)qlo + row * dp + k) = make_uint2(l[0] | (l[1] << 16), l[2] | (l[3] << 16));
+            }
+        }
+    } else {
+        for (int k = lane; k < dp; k += GW) {
+            uint32_t h, l;
+            one(k < d ? (double)xr[k] * sc : 0.0, h, l);
+            if (!act) continue;
+            if (FP8) {
+                qhi[row * dp + k] = (uint8_t)h;
+                if (TERMS == 3) qlo[row * dp + k] = (uint8_t)l;
+            } else {
+                ((uint16_t*)qhi)[row * dp + k] = (uint16_t)h;
+                if (TERMS == 3) ((uint16_t*)qlo)[row * dp + k] = (uint16_t)l;
+            }
+        }
     }
     RowSums out;
     out.nx = group_sum<GW>(nx);
