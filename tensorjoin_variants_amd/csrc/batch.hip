@@ -386,23 +386,32 @@ __global__ void __launch_bounds__(256) batch_ball_kernel(const float* const* __r
     __syncthreads();
     if (tid < d) sc[tid] = ((spart[0][tid] + spart[1][tid]) + (spart[2][tid] + spart[3][tid])) / (float)ns;
     __syncthreads();
-    // radius: 16 lanes per row (4 rows per wave at a time), binary32 sums.
-    // Every difference and square of binary32 values is rounded once and
-    // all terms are non-negative, so the exact |x - c|^2 <= (S + (d+2)
+    // radius: binary32 sums, LPR lanes per row (64 / LPR rows per wave at a
+    // time).  Every difference and square of binary32 values is rounded once
+    // and all terms are non-negative, so the exact |x - c|^2 <= (S + (d+2)
     // 2^-125) / (1 - (d+2) 2^-24) whatever the summation order.  A NaN or
     // infinite sum (centre overflow) gives an infinite radius: no pruning.
-    const int g16 = lane >> 4, l16 = lane & 15;
+    // 16-byte loads when every row starts on a 16-byte boundary; then LPR is
+    // the row's 16-byte count rounded up to a power of two (at most 16), so
+    // narrow rows keep every lane busy (d = 16: 16 rows per wave, not 4)
+    const bool vec = (d & 3) == 0 && ((uintptr_t)x & 15) == 0;
+    int lpr = 16;
+    if (vec) {
+        lpr = 1;
+        while (lpr < (d >> 2) && lpr < 16) lpr <<= 1;
+    }
+    const int lg = __builtin_ctz(lpr);
+    const int g16 = lane >> lg, l16 = lane & (lpr - 1);
+    const int rpw = 64 >> lg;
     float rmax = 0.0f;
     bool bad = false;
-    // 16-byte loads when every row starts on a 16-byte boundary
-    const bool vec = (d & 3) == 0 && ((uintptr_t)x & 15) == 0;
-    for (int64_t rb = r0 + wave * 4; rb < r1; rb += 16) {
+    for (int64_t rb = r0 + wave * rpw; rb < r1; rb += 4 * rpw) {
         const int64_t r = rb + g16;
         const bool act = r < r1;
         const float* xr = x + perm_of(perm, J.gbase + (act ? r : r0), jmask) * d;
         float a = 0.0f;
         if (vec) {
-            for (int k = 4 * l16; k < d; k += 64) {
+            for (int k = 4 * l16; k < d; k += 4 * lpr) {
                 const float4 v = *(const float4*)(xr + k);
                 const float4 cc = *(const float4*)(sc + k);
                 const float u0 = v.x - cc.x, u1 = v.y - cc.y, u2 = v.z - cc.z, u3 = v.w - cc.w;
@@ -417,8 +426,7 @@ __global__ void __launch_bounds__(256) batch_ball_kernel(const float* const* __r
                 a += u * u;
             }
         }
-#pragma unroll
-        for (int o = 8; o > 0; o >>= 1)
+        for (int o = lpr >> 1; o > 0; o >>= 1)
             a += __builtin_bit_cast(float, tj::xor_lane(__builtin_bit_cast(uint32_t, a), o, lane));
         if (act) {
             bad |= !(a < INFINITY);
