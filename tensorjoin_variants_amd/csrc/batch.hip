@@ -369,16 +369,31 @@ __global__ void __launch_bounds__(256) batch_ball_kernel(const float* const* __r
     const float* x = xs[it.x];
     const int d = J.d;  // <= 256: at most four coordinates per lane
     // centre: the mean of up to 32 rows spread over the panel (any centre is
-    // valid; the radius below is taken over every row), 8 per wave
+    // valid; the radius below is taken over every row), 8 per wave.  The
+    // rows' order positions are read first, then their coordinates, so a
+    // wave has its loads in flight together (a loop of dependent pairs of
+    // loads, one row after another, was most of this kernel's time).
     const int64_t nr = r1 - r0;
     const int ns = nr < 32 ? (int)nr : 32;
-    float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-    for (int sidx = wave; sidx < ns; sidx += 4) {
-        const int64_t r = r0 + (int64_t)sidx * nr / ns;
-        const float* xr = x + perm_of(perm, J.gbase + r, jmask) * d;
+    const int wv = tjscreen::uni(wave);
+    int64_t ro[8];
 #pragma unroll
-        for (int m = 0; m < 4; ++m)
-            if (lane + 64 * m < d) acc[m] += xr[lane + 64 * m];
+    for (int u = 0; u < 8; ++u) {
+        const int sidx = wv + 4 * u < ns ? wv + 4 * u : ns - 1;
+        ro[u] = perm_of(perm, J.gbase + r0 + (int64_t)sidx * nr / ns, jmask) * d;
+    }
+    float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        if (64 * m >= d) break;
+        // lanes past d read coordinate d - 1 and store nothing below
+        const int k = lane + 64 * m < d ? lane + 64 * m : d - 1;
+        float v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = x[ro[u] + k];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (wv + 4 * u < ns) acc[m] += v[u];
     }
 #pragma unroll
     for (int m = 0; m < 4; ++m)
@@ -405,10 +420,17 @@ __global__ void __launch_bounds__(256) batch_ball_kernel(const float* const* __r
     const int rpw = 64 >> lg;
     float rmax = 0.0f;
     bool bad = false;
-    for (int64_t rb = r0 + wave * rpw; rb < r1; rb += 4 * rpw) {
+    // (the next step's order positions are read one step ahead)
+    int64_t rb = r0 + wv * rpw;
+    int64_t pnext = 0;
+    if (rb < r1) pnext = perm_of(perm, J.gbase + (rb + g16 < r1 ? rb + g16 : r0), jmask);
+    for (; rb < r1; rb += 4 * rpw) {
         const int64_t r = rb + g16;
         const bool act = r < r1;
-        const float* xr = x + perm_of(perm, J.gbase + (act ? r : r0), jmask) * d;
+        const int64_t pcur = pnext;
+        const int64_t rn = rb + 4 * rpw + g16;
+        if (rb + 4 * rpw < r1) pnext = perm_of(perm, J.gbase + (rn < r1 ? rn : r0), jmask);
+        const float* xr = x + pcur * d;
         float a = 0.0f;
         if (vec) {
             for (int k = 4 * l16; k < d; k += 4 * lpr) {
