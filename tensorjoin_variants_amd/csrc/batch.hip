@@ -294,41 +294,15 @@ __global__ void __launch_bounds__(256) batch_pivot_kernel(const float* const* __
 // Order keys, one per row: (join << (PIV_BITS + jbits)) | (nearest pivot <<
 // jbits) | local row.  Sorted, position k of the batch row space holds the
 // key of the row placed there (the joins stay contiguous and in order).
-// One scan unit (SCAN_SUB groups of 64 / GW rows of one join) per wave: the
-// pivots' coordinates of a lane are loaded once per unit.  The nearest pivot
-// is taken on the first ORDER_DIMS coordinates only (the order only steers
-// how much the tile bounds can skip; any order is correct), so a row costs
-// one 64-byte read however wide it is.
+// The nearest pivot is taken on the first ORDER_DIMS coordinates only (the
+// order only steers how much the tile bounds can skip; any order is
+// correct), so a row costs one 64-byte read however wide it is.  One wave
+// per 64 consecutive rows of a join, one row per lane: the lane holds its
+// row's leading coordinates and forms the NPIV distances itself, the pivot
+// coordinates being wave-uniform (scalar) loads -- no cross-lane step (16
+// lanes per row with a butterfly sum per pivot took 4.6 times the VALU work).
 constexpr int ORDER_DIMS = 16;
-// OGW = 16 lanes per row, one coordinate each; `rows` rows from `first`.
-template <int OGW>
-__device__ inline void order_unit(const float* x, const BatchJoin& J, int t, const int32_t* piv, int64_t first,
-                                  int rows, int lane, int jbits, uint64_t* keys) {
-    constexpr int RPS = 64 / OGW;  // rows per step
-    const int dd = J.d < ORDER_DIMS ? J.d : ORDER_DIMS;
-    const int lg = lane % OGW, g = lane / OGW;
-    float pv[NPIV];
-#pragma unroll
-    for (int k = 0; k < NPIV; ++k) pv[k] = lg < dd ? x[(int64_t)piv[k] * J.d + lg] : 0.0f;
-    for (int step = 0; step * RPS < rows; ++step) {
-        const int64_t local = first + step * RPS + g;
-        const bool act = step * RPS + g < rows && local < J.n;
-        const int64_t lc = act ? local : J.n - 1;
-        const float xv = lg < dd ? x[lc * J.d + lg] : 0.0f;
-        float best = INFINITY;
-        int bk = 0;
-#pragma unroll
-        for (int k = 0; k < NPIV; ++k) {
-            const float u = xv - pv[k];
-            float acc = u * u;
-            for (int o = OGW / 2; o > 0; o >>= 1)
-                acc += __builtin_bit_cast(float, tj::xor_lane(__builtin_bit_cast(uint32_t, acc), o, lane));
-            if (acc < best) { best = acc; bk = k; }  // NaN rows stay in bucket 0
-        }
-        if (act && lg == 0)
-            keys[J.gbase + lc] = ((uint64_t)t << (PIV_BITS + jbits)) | ((uint64_t)bk << jbits) | (uint64_t)lc;
-    }
-}
+constexpr int ORDER_ROWS = 64;  // rows per order unit (one wave)
 
 __global__ void __launch_bounds__(256) batch_order_kernel(const float* const* __restrict__ xs,
                                                           const BatchJoin* __restrict__ joins,
@@ -338,13 +312,54 @@ __global__ void __launch_bounds__(256) batch_order_kernel(const float* const* __
     const int64_t c = (int64_t)blockIdx.x * 4 + tjscreen::uni((int)(threadIdx.x >> 6));
     if (c >= cstart[nj]) return;
     const int lane = threadIdx.x & 63;
-    const int t = find_join(cstart, nj, c);
+    const int t = tjscreen::uni(find_join(cstart, nj, c));
     const BatchJoin J = joins[t];
-    const int gw = tjmeta::group_width(J.dp);
-    const int rows = SCAN_SUB * (64 / gw);  // the scan unit's rows
-    const int64_t first = (c - cstart[t]) * rows;
+    const int dd = J.d < ORDER_DIMS ? J.d : ORDER_DIMS;
+    const float* x = xs[t];
+    const int64_t local = (c - cstart[t]) * ORDER_ROWS + lane;
+    const bool act = local < J.n;
+    const int64_t lc = act ? local : J.n - 1;
+    // (the pivot rows below are read through a global-space pointer: their
+    // wave-uniform loads then become scalar loads)
+    typedef const __attribute__((address_space(1))) float gfloat;
+    const float* xr = x + lc * (int64_t)J.d;
+    float xv[ORDER_DIMS];
+    if ((J.d & 3) == 0 && ((uintptr_t)x & 15) == 0) {
+        // dd is a multiple of 4 here (16, or d < 16)
+#pragma unroll
+        for (int q = 0; q < ORDER_DIMS / 4; ++q) {
+            const float4 v = 4 * q < dd ? *(const float4*)(xr + 4 * q) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            xv[4 * q] = v.x; xv[4 * q + 1] = v.y; xv[4 * q + 2] = v.z; xv[4 * q + 3] = v.w;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < ORDER_DIMS; ++k) xv[k] = k < dd ? xr[k < dd ? k : 0] : 0.0f;
+    }
     const int32_t* piv = pivots + t * NPIV;
-    order_unit<ORDER_DIMS>(xs[t], J, t, piv, first, rows, lane, jbits, keys);
+    float best = INFINITY;
+    int bk = 0;
+#pragma unroll
+    for (int p = 0; p < NPIV; ++p) {
+        const gfloat* pr = (const gfloat*)(x + (int64_t)piv[p] * J.d);
+        float acc = 0.0f;
+        if (dd == ORDER_DIMS) {
+#pragma unroll
+            for (int k = 0; k < ORDER_DIMS; ++k) {
+                const float u = xv[k] - pr[k];
+                acc = fmaf(u, u, acc);
+            }
+        } else {
+            // narrow rows (d < ORDER_DIMS): coordinates past d are 0 in xv
+            // and are not read from the pivot
+#pragma unroll
+            for (int k = 0; k < ORDER_DIMS; ++k) {
+                const float u = xv[k] - (k < dd ? pr[k < dd ? k : 0] : 0.0f);
+                acc = fmaf(u, u, acc);
+            }
+        }
+        if (acc < best) { best = acc; bk = p; }  // NaN rows stay in bucket 0
+    }
+    if (act) keys[J.gbase + lc] = ((uint64_t)t << (PIV_BITS + jbits)) | ((uint64_t)bk << jbits) | (uint64_t)lc;
 }
 
 // One workgroup per work item (join t, panel p): the panel's bounding ball
@@ -967,7 +982,8 @@ struct tj_batch {
     int32_t* d_row_local;  // row -> its number within its join
     int64_t* d_cmeta;  // row-metadata units before join t
     int64_t* d_cscan;  // scale-scan units before join t
-    int64_t units_meta, units_scan;
+    int64_t* d_corder;  // spatial-order units (ORDER_ROWS rows) before join t
+    int64_t units_meta, units_scan, units_order;
     // tile pruning (off unless prune)
     bool prune;
     int order_bits;
@@ -1045,8 +1061,8 @@ static int batch_enqueue(const BatchCall& c, hipStream_t s) {
         int32_t* pivots = (int32_t*)(ws + b->off_pivots);
         hipLaunchKernelGGL(batch_pivot_kernel, dim3((unsigned)b->nj), dim3(256), 0, s, c.x, b->d_joins, pivots);
         TJ_HIP(hipGetLastError());
-        hipLaunchKernelGGL(batch_order_kernel, dim3((unsigned)((b->units_scan + 3) / 4)), dim3(256), 0, s, c.x,
-                           b->d_joins, b->d_cscan, b->nj, pivots, b->jbits, (uint64_t*)perm);
+        hipLaunchKernelGGL(batch_order_kernel, dim3((unsigned)((b->units_order + 3) / 4)), dim3(256), 0, s, c.x,
+                           b->d_joins, b->d_corder, b->nj, pivots, b->jbits, (uint64_t*)perm);
         TJ_HIP(hipGetLastError());
         if ((rc = launch_sort(ws + b->off_sort, (uint64_t*)perm, (const uint64_t*)(ws + b->off_rows), b->rows,
                               b->order_bits, s)))
@@ -1310,15 +1326,17 @@ int tj_batch_create(tj_batch** out, int joins, const int64_t* n, const int* d, c
             rj[J[t].gbase + r] = t;
             rl[J[t].gbase + r] = (int32_t)r;
         }
-    std::vector<int64_t> cmeta(joins + 1, 0), cscan(joins + 1, 0);
+    std::vector<int64_t> cmeta(joins + 1, 0), cscan(joins + 1, 0), corder(joins + 1, 0);
     for (int t = 0; t < joins; ++t) {
         const int rpw = 64 / tjmeta::group_width(J[t].dp);
         const int64_t mrows = meta_unit_rows(J[t].dp);
         cmeta[t + 1] = cmeta[t] + (J[t].n + mrows - 1) / mrows;
         cscan[t + 1] = cscan[t] + (J[t].n + (int64_t)SCAN_SUB * rpw - 1) / ((int64_t)SCAN_SUB * rpw);
+        corder[t + 1] = corder[t] + (J[t].n + ORDER_ROWS - 1) / ORDER_ROWS;
     }
     b->units_meta = cmeta[joins];
     b->units_scan = cscan[joins];
+    b->units_order = corder[joins];
     for (int f = 0; f < N_FAMILIES; ++f) {
         const int64_t nf = b->fam_begin[f + 1] - b->fam_begin[f];
         b->fam_grid[f] = (int)std::max<int64_t>(1, std::min<int64_t>(nf, (int64_t)screen_batch_occupancy(f) * num_cus()));
@@ -1334,6 +1352,7 @@ int tj_batch_create(tj_batch** out, int joins, const int64_t* n, const int* d, c
     const size_t o_o = bcarve(tcur, sizeof(int32_t) * jo.size());
     const size_t o_cm = bcarve(tcur, sizeof(int64_t) * cmeta.size());
     const size_t o_cs = bcarve(tcur, sizeof(int64_t) * cscan.size());
+    const size_t o_co = bcarve(tcur, sizeof(int64_t) * corder.size());
     hipError_t e = hipMalloc(&b->dev, tcur);
     if (e != hipSuccess) {
         delete b;
@@ -1349,6 +1368,7 @@ int tj_batch_create(tj_batch** out, int joins, const int64_t* n, const int* d, c
     b->n_small = n_small;
     b->d_cmeta = (int64_t*)(base + o_cm);
     b->d_cscan = (int64_t*)(base + o_cs);
+    b->d_corder = (int64_t*)(base + o_co);
     if ((e = hipMemcpy(b->d_joins, J.data(), sizeof(BatchJoin) * J.size(), hipMemcpyHostToDevice)) != hipSuccess ||
         (e = hipMemcpy(b->d_groups, G.data(), sizeof(BatchGroup) * G.size(), hipMemcpyHostToDevice)) != hipSuccess ||
         (e = hipMemcpy(b->d_items, it.data(), sizeof(int2) * it.size(), hipMemcpyHostToDevice)) != hipSuccess ||
@@ -1356,7 +1376,8 @@ int tj_batch_create(tj_batch** out, int joins, const int64_t* n, const int* d, c
         (e = hipMemcpy(b->d_row_local, rl.data(), sizeof(int32_t) * rl.size(), hipMemcpyHostToDevice)) != hipSuccess ||
         (e = hipMemcpy(b->d_jorder, jo.data(), sizeof(int32_t) * jo.size(), hipMemcpyHostToDevice)) != hipSuccess ||
         (e = hipMemcpy(b->d_cmeta, cmeta.data(), sizeof(int64_t) * cmeta.size(), hipMemcpyHostToDevice)) != hipSuccess ||
-        (e = hipMemcpy(b->d_cscan, cscan.data(), sizeof(int64_t) * cscan.size(), hipMemcpyHostToDevice)) != hipSuccess) {
+        (e = hipMemcpy(b->d_cscan, cscan.data(), sizeof(int64_t) * cscan.size(), hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(b->d_corder, corder.data(), sizeof(int64_t) * corder.size(), hipMemcpyHostToDevice)) != hipSuccess) {
         (void)hipFree(b->dev);
         delete b;
         return hip_fail(e, "hipMemcpy(batch tables)");
