@@ -88,10 +88,10 @@ int rows_per_wg(int combo) {
 
 // Work units of the per-row passes.  A group of GW = group_width(dp) lanes
 // handles one row, so a wave holds 64 / GW rows of one join at a time; the
-// scale scan (and the spatial order) take SCAN_SUB such row sets per unit
-// (one atomic per unit), the row-metadata pass 64 rows per unit (lane L
-// finishes row L).  Chunk prefix tables: units of joins 0..t-1 (device
-// int64 [joins + 1]).
+// scale scan takes SCAN_SUB such row sets per unit (one atomic per unit),
+// the row-metadata pass 64 rows per unit (lane L finishes row L), the
+// spatial order ORDER_ROWS rows, one per lane.  Chunk prefix tables: units
+// of joins 0..t-1 (device int64 [joins + 1]).
 constexpr int SCAN_SUB = 8;
 
 // Rows of one row-metadata unit (one wave): 16 steps of 64 / width rows.
