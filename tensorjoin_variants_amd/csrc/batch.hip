@@ -138,14 +138,24 @@ __global__ void __launch_bounds__(256) batch_scan_kernel(const float* const* __r
         uint32_t m = 0;
         auto take = [&](float f) {
             const uint32_t u = __builtin_bit_cast(uint32_t, f) & 0x7FFFFFFFu;
-            if (u >= 0x7F800000u) bad++;
-            else m = u > m ? u : m;
+            const bool nf = u >= 0x7F800000u;  // inf or NaN: counted, not a maximum
+            bad += nf ? 1u : 0u;
+            m = (!nf && u > m) ? u : m;
         };
         int64_t head = (int64_t)(((16 - ((uintptr_t)base & 15)) & 15) / 4);
         if (head > cnt) head = cnt;
         const int64_t nv = (cnt - head) / 4;
         const float4* xv = (const float4*)(base + head);
-        for (int64_t k = lane; k < nv; k += 64) {
+        // four 16-byte loads per lane in flight, then the remainder
+        int64_t k = lane;
+        for (; k + 192 < nv; k += 256) {
+            const float4 v0 = xv[k], v1 = xv[k + 64], v2 = xv[k + 128], v3 = xv[k + 192];
+            take(v0.x); take(v0.y); take(v0.z); take(v0.w);
+            take(v1.x); take(v1.y); take(v1.z); take(v1.w);
+            take(v2.x); take(v2.y); take(v2.z); take(v2.w);
+            take(v3.x); take(v3.y); take(v3.z); take(v3.w);
+        }
+        for (; k < nv; k += 64) {
             const float4 v = xv[k];
             take(v.x); take(v.y); take(v.z); take(v.w);
         }
